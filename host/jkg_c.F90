@@ -1,0 +1,103 @@
+! jkg_c.F90 -- ISO_C_BINDING interface to libjkg.so (include/jkg.h) for the
+! Fortran drop-in programs djk.exe and zjk.exe.  The derived types mirror the
+! C structs field by field (BIND(C) gives them the C layout).
+MODULE jkg_c
+  USE, INTRINSIC :: ISO_C_BINDING
+  IMPLICIT NONE
+
+  INTEGER(c_int), PARAMETER :: JKG_OK = 0_c_int
+
+  TYPE, BIND(C) :: jkg_steprec
+     INTEGER(c_int32_t) :: step, oldlen, newlen, trig, hyp, it
+     REAL(c_double) :: tw
+  END TYPE jkg_steprec
+
+  TYPE, BIND(C) :: jkg_opts
+     INTEGER(c_int32_t) :: id_ncp, id_tru, fused, max_steps, steps_per_sync, trace_cap
+     TYPE(c_ptr) :: trace
+  END TYPE jkg_opts
+
+  TYPE, BIND(C) :: jkg_result
+     INTEGER(c_int32_t) :: info, tt, npos, nneg, n2, steps
+     REAL(c_double) :: seconds
+  END TYPE jkg_result
+
+  INTERFACE
+     INTEGER(c_int) FUNCTION jkg_create(ctx, device) BIND(C, NAME='jkg_create')
+       IMPORT :: c_int, c_ptr
+       TYPE(c_ptr), INTENT(OUT) :: ctx
+       INTEGER(c_int), VALUE :: device
+     END FUNCTION jkg_create
+
+     SUBROUTINE jkg_destroy(ctx) BIND(C, NAME='jkg_destroy')
+       IMPORT :: c_ptr
+       TYPE(c_ptr), VALUE :: ctx
+     END SUBROUTINE jkg_destroy
+
+     TYPE(c_ptr) FUNCTION jkg_strerror(code) BIND(C, NAME='jkg_strerror')
+       IMPORT :: c_int, c_ptr
+       INTEGER(c_int), VALUE :: code
+     END FUNCTION jkg_strerror
+
+     INTEGER(c_int) FUNCTION jkg_jstep_len(n, n2) BIND(C, NAME='jkg_jstep_len')
+       IMPORT :: c_int
+       INTEGER(c_int), VALUE :: n, n2
+     END FUNCTION jkg_jstep_len
+
+     INTEGER(c_int) FUNCTION jkg_dsolve(ctx, n, n2, y, j, yu, zz, sy, opts, res) BIND(C, NAME='jkg_dsolve')
+       IMPORT :: c_int, c_ptr, c_double, c_int64_t, jkg_opts, jkg_result
+       TYPE(c_ptr), VALUE :: ctx
+       INTEGER(c_int), VALUE :: n, n2
+       REAL(c_double), INTENT(IN) :: y(*)
+       INTEGER(c_int64_t), INTENT(IN) :: j(*)
+       REAL(c_double), INTENT(OUT) :: yu(*), zz(*), sy(*)
+       TYPE(jkg_opts), INTENT(IN) :: opts
+       TYPE(jkg_result), INTENT(OUT) :: res
+     END FUNCTION jkg_dsolve
+
+     INTEGER(c_int) FUNCTION jkg_zsolve(ctx, n, n2, y, j, yu, zz, sy, opts, res) BIND(C, NAME='jkg_zsolve')
+       IMPORT :: c_int, c_ptr, c_double, c_double_complex, c_int64_t, jkg_opts, jkg_result
+       TYPE(c_ptr), VALUE :: ctx
+       INTEGER(c_int), VALUE :: n, n2
+       COMPLEX(c_double_complex), INTENT(IN) :: y(*)
+       INTEGER(c_int64_t), INTENT(IN) :: j(*)
+       COMPLEX(c_double_complex), INTENT(OUT) :: yu(*), zz(*)
+       REAL(c_double), INTENT(OUT) :: sy(*)
+       TYPE(jkg_opts), INTENT(IN) :: opts
+       TYPE(jkg_result), INTENT(OUT) :: res
+     END FUNCTION jkg_zsolve
+  END INTERFACE
+
+CONTAINS
+
+  ! the message of a status code, as a Fortran string
+  FUNCTION jkg_message(code) RESULT(msg)
+    INTEGER(c_int), INTENT(IN) :: code
+    CHARACTER(LEN=80) :: msg
+    CHARACTER(KIND=c_char), POINTER :: s(:)
+    INTEGER :: i
+    CALL C_F_POINTER(jkg_strerror(code), s, [80])
+    msg = ' '
+    DO i = 1, 80
+       IF (s(i) .EQ. C_NULL_CHAR) EXIT
+       msg(i:i) = s(i)
+    END DO
+  END FUNCTION jkg_message
+
+  ! the per-step CSV of the reference (src/D/dstep.F90:388-440); the GPU
+  ! steps are timed as a whole, so BUILDs carries the mean step time and
+  ! TRANSFs 0
+  SUBROUTINE write_trace(u, rec, nrec, tstep)
+    INTEGER, INTENT(IN) :: u, nrec
+    TYPE(jkg_steprec), INTENT(IN) :: rec(*)
+    REAL(c_double), INTENT(IN) :: tstep
+    INTEGER :: k
+    WRITE (u,'(A)') '"STEP","OLDLEN","BUILDs","TRANSFs","NEWLEN","NORMF2D","TRIG","HYP"'
+    DO k = 1, nrec
+       WRITE (u,'(I11,A,I11,A,F12.6,A,F12.6,A,I11,A,ES25.17E3,2(A,I11))') rec(k)%step, ',', &
+            rec(k)%oldlen, ',', tstep, ',', 0.0_c_double, ',', rec(k)%newlen, ',', rec(k)%tw, ',', &
+            rec(k)%trig, ',', rec(k)%hyp
+    END DO
+    FLUSH(u)
+  END SUBROUTINE write_trace
+END MODULE jkg_c

@@ -1,0 +1,137 @@
+/*
+ * jkg.h -- C ABI of the MI355X J-Kogbetliantz hyperbolic SVD (libjkg.so).
+ *
+ * Y = YU * diag(SY) * ZZ with YU unitary and ZZ = J Z^H J, Z J-unitary
+ * (Z^H J Z = J), J = diag(+-1): the decomposition computed by the reference's
+ * djk.exe / zjk.exe (venovako/JKogb src/D/djk.F90, src/Z/zjk.F90).  Every
+ * entry point is plain C: pointers, sizes and int status codes.  Matrices are
+ * column-major, as the reference's FN.Y/FN.YU/FN.ZZ files; complex data are
+ * interleaved (re, im) pairs of doubles (Fortran COMPLEX(KIND=8)).
+ *
+ * Return values: JKG_OK (0), a negative JKG_E* code, or (for argument
+ * checks) minus the 1-based position of the offending argument, as the
+ * reference's INFO convention does.
+ */
+#ifndef JKG_H
+#define JKG_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define JKG_OK 0
+#define JKG_EHIP (-1000)     /* a HIP runtime call failed (see jkg_last_hip_error) */
+#define JKG_ENOMEM (-1001)   /* device allocation failed */
+#define JKG_ENOTSUP (-1002)  /* option not implemented on the GPU (e.g. ID_NCP=3) */
+#define JKG_EJ (-1003)       /* a J entry outside {-1,+1} (the reference: R%TRU INFO=-2) */
+#define JKG_ENOCONV (-1004)  /* the step loop ran past its guard of 1000 N + 1000 steps */
+
+typedef struct jkg_ctx jkg_ctx;
+
+/* The AW candidate record of the reference (src/shared/atypes.F90:18-24):
+ * weight, row P < column Q (1-based), band B = Q - P, info. 24 bytes. */
+typedef struct {
+  double w;
+  int32_t p, q, b, i;
+} jkg_aw;
+
+/* One line of the reference's per-step stderr CSV (src/D/dstep.F90:410-440)
+ * minus the two timing columns: STEP, OLDLEN, NEWLEN, TRIG, HYP, NORMF2D;
+ * it = number of finite weights (IT) of the step. */
+typedef struct {
+  int32_t step, oldlen, newlen, trig, hyp, it;
+  double tw;
+} jkg_steprec;
+
+typedef struct {
+  int32_t id_ncp;         /* 0 or 1 or 2: greedy walk (AW_NCP1/2 select the same pivots); 3: JKG_ENOTSUP */
+  int32_t id_tru;         /* 0, 1 or 2 (pair enumeration; does not change results) */
+  int32_t fused;          /* 0: FMAD(a,b,c) = a*b+c, two roundings (the reference's default build); 1: fma */
+  int32_t max_steps;      /* < 0: run to convergence; else stop after this many steps */
+  int32_t steps_per_sync; /* steps launched between two host polls of the device's done flag; 0: auto */
+  int32_t trace_cap;      /* records available in trace */
+  jkg_steprec *trace;     /* host buffer for the per-step trace, or NULL */
+} jkg_opts;
+
+typedef struct {
+  int32_t info;   /* >= 0: steps with transformations (the reference's INFO); < 0: error */
+  int32_t tt;     /* number of trigonometric pairs (the reference's TT) */
+  int32_t npos;   /* J entries equal to +1 */
+  int32_t nneg;   /* J entries equal to -1 */
+  int32_t n2;     /* N_2 after JSTEP_LEN */
+  int32_t steps;  /* trace records written (steps executed, including the last) */
+  double seconds; /* device time of the step loop (HIP events) */
+} jkg_result;
+
+void jkg_opts_default(jkg_opts *o);
+const char *jkg_strerror(int code);
+int jkg_last_hip_error(void);
+
+/* A context owns one HIP stream and the device buffers of the last problem
+ * size; one context per host thread. */
+int jkg_create(jkg_ctx **ctx, int device);
+void jkg_destroy(jkg_ctx *ctx);
+
+/* ---------------------------------------------------------------------------
+ * Process level: what djk.exe/zjk.exe FN N N_2 [ID_NCP [ID_TRU]] computes
+ * between reading FN.Y/FN.J and writing FN.YU/FN.ZZ/FN.SY
+ * (src/D/djk.F90:27-121): JSTEP_LEN, R%TRU, DSTEP_LOOP (U=I, Z=I, steps until
+ * no pivot or no effective transform), SY = diag(A), ZZ = J Z^T J.
+ * y is N x N (ld = n), j holds N int64 signs; yu, zz (N x N) and sy (N) are
+ * written only on success (JKG_OK and res->info >= 0).
+ * --------------------------------------------------------------------------- */
+int jkg_dsolve(jkg_ctx *ctx, int n, int n2, const double *y, const int64_t *j, double *yu, double *zz,
+               double *sy, const jkg_opts *opts, jkg_result *res);
+int jkg_zsolve(jkg_ctx *ctx, int n, int n2, const double *y, const int64_t *j, double *yu, double *zz,
+               double *sy, const jkg_opts *opts, jkg_result *res);
+
+/* ---------------------------------------------------------------------------
+ * Strategy level: the reference's DPROC plug-in procedures
+ * (src/D/dtypes.F90:5-30), bound by DPROC_INIT (src/D/dstep.F90:76-134).
+ * jkg_srt  = ASRT: sort dz[0..nn) by AW_CMP (W descending, then B, P, Q
+ *            descending); info >= 0 on success.
+ * jkg_ncp  = ANCP (AW_NCP1/AW_NCP2): greedy disjoint pivots from the sorted
+ *            list; step[0..*sl) are 1-based list positions.
+ * jkg_tru  = ATRU (TRU1/TRU2, src/shared/jstep.F90:9-120); returns TT or -2.
+ * jkg_jstep_len = JSTEP_LEN (src/shared/jstep.F90:124-133).
+ * --------------------------------------------------------------------------- */
+int jkg_srt(jkg_ctx *ctx, int nn, int nm, jkg_aw *dz, int64_t *info);
+int jkg_ncp(jkg_ctx *ctx, int n, int nn, int nm, jkg_aw *dz, int n2, int32_t *sl, int32_t *step, int64_t *info);
+int jkg_tru(int id_tru, int n, const int32_t *j, int nn, int32_t *p, int32_t *q, int32_t pn[2]);
+int jkg_jstep_len(int n, int n2);
+
+/* ---------------------------------------------------------------------------
+ * Step level, for callers that drive the loop themselves and for parity
+ * checks of each stage against the reference.
+ * jkg_dhsvd2   : DHSVD2 (src/D/dtransf.F90:614-683) of count 2x2 blocks
+ *                (a: 4 per block, column-major; j: 2 per block); writes the
+ *                U and Z factors (4 each), the new diagonal (2) and INFO.
+ * jkg_zhsvd2   : ZHSVD2 (src/Z/ztransf.F90:925-1009), complex interleaved.
+ * jkg_dweights : the weight of every pair (DMAGF2T/DMAGF2H,
+ *                src/D/dstep.F90:12-72) as its sort key (see jkg_key_weight),
+ *                indexed by jkg_pair_index.
+ * jkg_dbuild   : DSTEP_BUILD (src/D/dstep.F90:138-282): the sorted finite
+ *                candidates sorted[0..*it) and the chosen positions step[0..*sl).
+ * jkg_dstep    : one DSTEP_EXEC on (a, u, z) in place; rec gets its CSV line.
+ *                Returns JKG_OK; rec->newlen < 0 or rec->oldlen == 0 ends a loop.
+ * --------------------------------------------------------------------------- */
+int jkg_dhsvd2(jkg_ctx *ctx, int count, const double *a, const int32_t *j, double th1, int fused, double *u,
+               double *z, double *d, int32_t *info);
+int jkg_zhsvd2(jkg_ctx *ctx, int count, const double *a, const int32_t *j, double th1, int fused, double *u,
+               double *z, double *d, int32_t *info);
+int jkg_dweights(jkg_ctx *ctx, int n, const double *a, const int64_t *j, int fused, uint64_t *keys);
+int jkg_zweights(jkg_ctx *ctx, int n, const double *a, const int64_t *j, int fused, uint64_t *keys);
+int jkg_dbuild(jkg_ctx *ctx, int n, int n2, const double *a, const int64_t *j, int fused, int32_t *it,
+               jkg_aw *sorted, int32_t *sl, int32_t *step);
+int jkg_dstep(jkg_ctx *ctx, int n, int n2, double *a, const int64_t *j, double *u, double *z, int fused,
+              jkg_steprec *rec);
+/* index of pair (p,q), 1 <= p < q <= n, in the tie-break order of AW_CMP */
+uint32_t jkg_pair_index(int n, int p, int q);
+/* the weight a sort key stands for (NaN for the key of a dropped pair) */
+double jkg_key_weight(uint64_t key);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

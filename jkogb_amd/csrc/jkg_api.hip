@@ -1,0 +1,728 @@
+// jkg_api.hip -- the C ABI of libjkg.so (include/jkg.h): contexts, device
+// buffers, the graph-captured step loop, and the step-level entry points.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../include/jkg.h"
+#include "jkg_hsvd2.hpp"
+#include "jkg_step.hpp"
+
+using namespace jkg;
+
+static thread_local int g_last_hip = 0;
+
+#define HIPCHK(call)                        \
+  do {                                      \
+    hipError_t e_ = (call);                 \
+    if (e_ != hipSuccess) {                 \
+      g_last_hip = (int)e_;                 \
+      return JKG_EHIP;                      \
+    }                                       \
+  } while (0)
+
+struct jkg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // buffers of the current problem size
+  int n = 0, cplx = -1, trace_cap = 0;
+  size_t sort_n = 0;
+  std::vector<void *> allocs;
+  DeviceStep d{};
+  double *sigma = nullptr;
+  // captured step
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graph_fused = -1;
+  // misc test buffers
+  void *tmp = nullptr;
+  size_t tmp_bytes = 0;
+};
+
+static void free_all(jkg_ctx *c) {
+  if (c->exec) (void)hipGraphExecDestroy(c->exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->exec = nullptr;
+  c->graph = nullptr;
+  c->graph_fused = -1;
+  for (void *p : c->allocs) (void)hipFree(p);
+  c->allocs.clear();
+  c->n = 0;
+  c->cplx = -1;
+  c->trace_cap = 0;
+  memset(&c->d, 0, sizeof(c->d));
+  c->sigma = nullptr;
+}
+
+template <class T>
+static int dalloc(jkg_ctx *c, T **p, size_t count) {
+  void *q = nullptr;
+  if (count == 0) count = 1;
+  if (hipMalloc(&q, count * sizeof(T)) != hipSuccess) {
+    (void)hipGetLastError();
+    return JKG_ENOMEM;
+  }
+  c->allocs.push_back(q);
+  *p = (T *)q;
+  return JKG_OK;
+}
+
+#define ACHK(x)                 \
+  do {                          \
+    int r_ = (x);               \
+    if (r_ != JKG_OK) {         \
+      free_all(c);              \
+      return r_;                \
+    }                           \
+  } while (0)
+
+// (Re)allocates the buffers of an N x N problem (real or complex).
+static int ensure_buffers(jkg_ctx *c, int n, int cplx, int trace_cap) {
+  if (trace_cap < 1) trace_cap = 1;
+  if (c->n == n && c->cplx == cplx && c->trace_cap >= trace_cap) return JKG_OK;
+  free_all(c);
+  DeviceStep &d = c->d;
+  const size_t nn = (size_t)n * (n - 1) / 2;
+  const size_t esz = cplx ? 16 : 8;
+  const int n2max = n / 2 > 0 ? n / 2 : 1;
+  d.n = n;
+  d.lda = n;
+  d.cplx = cplx;
+  d.nn = (int)nn;
+  d.n2max = n2max;
+  ACHK(dalloc(c, (char **)&d.A, esz * n * n));
+  ACHK(dalloc(c, (char **)&d.U, esz * n * n));
+  ACHK(dalloc(c, (char **)&d.Z, esz * n * n));
+  ACHK(dalloc(c, &d.J, n));
+  ACHK(dalloc(c, &d.plus, n));
+  ACHK(dalloc(c, &d.minus, n));
+  ACHK(dalloc(c, &d.sort.k0, nn));
+  ACHK(dalloc(c, &d.sort.k1, nn));
+  ACHK(dalloc(c, &d.sort.v0, nn));
+  ACHK(dalloc(c, &d.sort.v1, nn));
+  ACHK(dalloc(c, (char **)&d.sort.scratch, sort_scratch_bytes((int)nn)));
+  d.sort_skip = sort_skip_flags(d.sort, (int)nn);
+  for (int k = 0; k < 2; ++k) {
+    ACHK(dalloc(c, &d.lists.key[k], nn));
+    ACHK(dalloc(c, &d.lists.val[k], nn));
+    ACHK(dalloc(c, &d.lists.pos[k], nn));
+  }
+  ACHK(dalloc(c, &d.fcount, nn / 2048 + 2));
+  ACHK(dalloc(c, &d.used, n));
+  ACHK(dalloc(c, &d.piv_p, n2max));
+  ACHK(dalloc(c, &d.piv_q, n2max));
+  ACHK(dalloc(c, &d.piv_pos, n2max));
+  ACHK(dalloc(c, &d.piv_info, n2max));
+  ACHK(dalloc(c, &d.piv_w, n2max));
+  ACHK(dalloc(c, (char **)&d.piv_V, esz * 4 * n2max));
+  ACHK(dalloc(c, (char **)&d.piv_W, esz * 4 * n2max));
+  ACHK(dalloc(c, &d.piv_d, 2 * n2max));
+  ACHK(dalloc(c, &d.rot_row, n2max));
+  ACHK(dalloc(c, &d.rot_col, n2max));
+  ACHK(dalloc(c, &d.colpiv, n));
+  ACHK(dalloc(c, &d.st, 1));
+  ACHK(dalloc(c, &d.trace, trace_cap));
+  ACHK(dalloc(c, &c->sigma, n));
+  d.select_limit = std::max(4096, 4 * n);
+  if (step_prepare(d) != 0) {
+    free_all(c);
+    return JKG_ENOTSUP;
+  }
+  c->n = n;
+  c->cplx = cplx;
+  c->trace_cap = trace_cap;
+  return JKG_OK;
+}
+
+static int tmp_bytes(jkg_ctx *c, size_t bytes) {
+  if (c->tmp_bytes >= bytes) return JKG_OK;
+  if (c->tmp) (void)hipFree(c->tmp);
+  c->tmp = nullptr;
+  c->tmp_bytes = 0;
+  if (hipMalloc(&c->tmp, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return JKG_ENOMEM;
+  }
+  c->tmp_bytes = bytes;
+  return JKG_OK;
+}
+
+extern "C" {
+
+void jkg_opts_default(jkg_opts *o) {
+  memset(o, 0, sizeof(*o));
+  o->max_steps = -1;
+}
+
+const char *jkg_strerror(int code) {
+  switch (code) {
+    case JKG_OK: return "success";
+    case JKG_EHIP: return "HIP runtime error";
+    case JKG_ENOMEM: return "device allocation failed";
+    case JKG_ENOTSUP: return "option not supported on the GPU";
+    case JKG_EJ: return "J has an entry outside {-1,+1} (R%TRU)";
+    case JKG_ENOCONV: return "the step loop did not end within 1000 N + 1000 steps";
+    default: return (code < 0) ? "invalid argument" : "unknown";
+  }
+}
+
+int jkg_last_hip_error(void) { return g_last_hip; }
+
+int jkg_create(jkg_ctx **out, int device) {
+  if (!out) return -1;
+  *out = nullptr;
+  int count = 0;
+  HIPCHK(hipGetDeviceCount(&count));
+  if (device < 0 || device >= count) return -2;
+  HIPCHK(hipSetDevice(device));
+  jkg_ctx *c = new jkg_ctx();
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    g_last_hip = (int)hipGetLastError();
+    delete c;
+    return JKG_EHIP;
+  }
+  *out = c;
+  return JKG_OK;
+}
+
+void jkg_destroy(jkg_ctx *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  free_all(c);
+  if (c->tmp) (void)hipFree(c->tmp);
+  (void)hipEventDestroy(c->ev0);
+  (void)hipEventDestroy(c->ev1);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int jkg_jstep_len(int n, int n2) {  // src/shared/jstep.F90:124-133
+  if (n2 == 0) return (n / 2 > 0) ? n / 2 : 0;
+  int m = (n2 > 0) ? n2 : 0;
+  return (m < n / 2) ? m : n / 2;
+}
+
+int jkg_tru(int id_tru, int n, const int32_t *j, int nn, int32_t *p, int32_t *q, int32_t pn[2]) {
+  // src/shared/jstep.F90:9-120: trigonometric pairs from the front,
+  // hyperbolic pairs from the back; column-cyclic (1) or row-cyclic (2)
+  pn[0] = pn[1] = 0;
+  if (n < 0) return -1;
+  if (nn < 0) return -3;
+  if (n <= 1) return 0;
+  for (int i = 0; i < n; ++i) {
+    if (j[i] == 1) ++pn[0];
+    else if (j[i] == -1) ++pn[1];
+    else return -2;
+  }
+  int it = 1, ih = nn;
+  auto put = [&](int ip, int iq) {
+    if (j[ip - 1] == j[iq - 1]) { p[it - 1] = ip; q[it - 1] = iq; ++it; }
+    else { p[ih - 1] = ip; q[ih - 1] = iq; --ih; }
+  };
+  if (id_tru == 2) {
+    for (int ip = 1; ip <= n - 1; ++ip)
+      for (int iq = ip + 1; iq <= n; ++iq) {
+        if (it > ih) return 0;
+        put(ip, iq);
+      }
+  } else {
+    for (int iq = 2; iq <= n; ++iq)
+      for (int ip = 1; ip <= iq - 1; ++ip) {
+        if (it > ih) return 0;
+        put(ip, iq);
+      }
+  }
+  return it - 1;
+}
+
+uint32_t jkg_pair_index(int n, int p, int q) { return pair_index(n, p, q); }
+
+double jkg_key_weight(uint64_t key) {
+  if (key == KEY_NAN) return NAN;
+  uint64_t ord = ~key;
+  uint64_t u = (ord & 0x8000000000000000ull) ? (ord & 0x7FFFFFFFFFFFFFFFull) : ~ord;
+  double w;
+  memcpy(&w, &u, 8);
+  return w;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- problem setup
+static int upload_problem(jkg_ctx *c, int n, int n2, const void *y, const int64_t *j, int *npos, int *nneg,
+                          int max_steps, int trace_cap) {
+  DeviceStep &d = c->d;
+  std::vector<int> J32(n), pl, mi;
+  for (int i = 0; i < n; ++i) {
+    if (j[i] != 1 && j[i] != -1) return JKG_EJ;
+    J32[i] = (int)j[i];
+    (j[i] == 1 ? pl : mi).push_back(i);
+  }
+  *npos = (int)pl.size();
+  *nneg = (int)mi.size();
+  d.npl = (int)pl.size();
+  d.nmi = (int)mi.size();
+  const size_t esz = d.cplx ? 16 : 8;
+  HIPCHK(hipMemcpyAsync(d.A, y, esz * n * n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d.J, J32.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+  if (!pl.empty()) HIPCHK(hipMemcpyAsync(d.plus, pl.data(), sizeof(int) * pl.size(), hipMemcpyHostToDevice, c->stream));
+  if (!mi.empty()) HIPCHK(hipMemcpyAsync(d.minus, mi.data(), sizeof(int) * mi.size(), hipMemcpyHostToDevice, c->stream));
+  StepState s{};
+  s.max_steps = max_steps;
+  s.n2 = n2;
+  s.trace_cap = trace_cap;
+  HIPCHK(hipMemcpyAsync(d.st, &s, sizeof(s), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));  // the host vectors go out of scope
+  return JKG_OK;
+}
+
+static int capture_step(jkg_ctx *c, int fused) {
+  if (c->exec && c->graph_fused == fused) return JKG_OK;
+  if (c->exec) (void)hipGraphExecDestroy(c->exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->exec = nullptr;
+  c->graph = nullptr;
+  c->d.fused = fused;
+  HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  step_launch(c->d, c->stream);
+  HIPCHK(hipStreamEndCapture(c->stream, &c->graph));
+  HIPCHK(hipGraphInstantiate(&c->exec, c->graph, nullptr, nullptr, 0));
+  c->graph_fused = fused;
+  return JKG_OK;
+}
+
+static int run_loop(jkg_ctx *c, const jkg_opts *o, jkg_result *res) {
+  DeviceStep &d = c->d;
+  int rc = capture_step(c, o->fused ? 1 : 0);
+  if (rc) return rc;
+  init_identity_launch(d, c->stream);
+  HIPCHK(hipGetLastError());
+  int batch = o->steps_per_sync > 0 ? o->steps_per_sync : 16;
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  // The reference needs about 10 N steps; a run that has not ended after
+  // 1000 N + 1000 steps is stopped and reported (a guard, never reached by a
+  // converging run).
+  const long cap = (o->max_steps >= 0) ? (long)o->max_steps + 2 : 1000L * d.n + 1000;
+  long launched = 0;
+  for (;;) {
+    for (int k = 0; k < batch; ++k) HIPCHK(hipGraphLaunch(c->exec, c->stream));
+    launched += batch;
+    int done = 0;
+    HIPCHK(hipMemcpyAsync(&done, &d.st->done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (done) break;
+    if (launched > cap) return JKG_ENOCONV;
+    if (o->steps_per_sync <= 0 && batch < 64) batch *= 2;
+  }
+  HIPCHK(hipEventRecord(c->ev1, c->stream));
+  finalize_launch(d, c->stream, c->sigma);
+  HIPCHK(hipGetLastError());
+  StepState s;
+  HIPCHK(hipMemcpyAsync(&s, d.st, sizeof(s), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  res->info = s.info;
+  res->steps = s.nrec;
+  res->seconds = ms * 1e-3;
+  (void)launched;
+  return JKG_OK;
+}
+
+extern "C" int jkg_dsolve(jkg_ctx *c, int n, int n2, const double *y, const int64_t *j, double *yu, double *zz,
+                          double *sy, const jkg_opts *opts, jkg_result *res) {
+  if (!c) return -1;
+  if (n < 3) return -2;
+  if (!y) return -4;
+  if (!j) return -5;
+  if (!yu) return -6;
+  if (!zz) return -7;
+  if (!sy) return -8;
+  if (!res) return -10;
+  jkg_opts o;
+  if (opts) o = *opts;
+  else jkg_opts_default(&o);
+  if (o.id_ncp == 3) return JKG_ENOTSUP;
+  if (o.id_ncp < 0 || o.id_ncp > 3) return -9;
+  if (o.id_tru < 0 || o.id_tru > 2) return -9;
+  memset(res, 0, sizeof(*res));
+  HIPCHK(hipSetDevice(c->device));
+  n2 = jkg_jstep_len(n, n2);
+  res->n2 = n2;
+  if (n2 <= 0) return -3;
+  int cap = std::max(o.trace_cap, 1);
+  int rc = ensure_buffers(c, n, 0, cap);
+  if (rc) return rc;
+  // TT: the number of trigonometric pairs (the reference's R%TRU result)
+  int npos = 0, nneg = 0;
+  rc = upload_problem(c, n, n2, y, j, &npos, &nneg, o.max_steps, c->trace_cap);
+  if (rc) return rc;
+  res->npos = npos;
+  res->nneg = nneg;
+  res->tt = (int)((int64_t)npos * (npos - 1) / 2 + (int64_t)nneg * (nneg - 1) / 2);
+  rc = run_loop(c, &o, res);
+  if (rc) return rc;
+  DeviceStep &d = c->d;
+  if (o.trace && o.trace_cap > 0) {
+    int cnt = std::min(res->steps, o.trace_cap);
+    if (cnt > 0) HIPCHK(hipMemcpy(o.trace, d.trace, sizeof(jkg_steprec) * cnt, hipMemcpyDeviceToHost));
+  }
+  if (res->info >= 0) {
+    HIPCHK(hipMemcpy(yu, d.U, sizeof(double) * n * n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(zz, d.Z, sizeof(double) * n * n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(sy, c->sigma, sizeof(double) * n, hipMemcpyDeviceToHost));
+  }
+  return JKG_OK;
+}
+
+extern "C" int jkg_zsolve(jkg_ctx *c, int n, int n2, const double *y, const int64_t *j, double *yu, double *zz,
+                          double *sy, const jkg_opts *opts, jkg_result *res) {
+  (void)c; (void)n; (void)n2; (void)y; (void)j; (void)yu; (void)zz; (void)sy; (void)opts; (void)res;
+  return JKG_ENOTSUP;
+}
+
+// ---------------------------------------------------------------- step level
+template <bool F>
+__global__ void k_dhsvd2_batch(int count, const double *a, const int *j, double th1, double *u, double *z,
+                               double *dg, int *info) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  double A[4], U[4] = {0, 0, 0, 0}, Z[4] = {0, 0, 0, 0};
+  for (int k = 0; k < 4; ++k) A[k] = a[4 * i + k];
+  int J[2] = {j[2 * i], j[2 * i + 1]};
+  int inf = dhsvd2<F>(A, J, U, Z, th1);
+  for (int k = 0; k < 4; ++k) {
+    u[4 * i + k] = U[k];
+    z[4 * i + k] = Z[k];
+  }
+  dg[2 * i] = A[0];
+  dg[2 * i + 1] = A[3];
+  info[i] = inf;
+}
+
+template <bool F>
+__global__ void k_zhsvd2_batch(int count, const dz *a, const int *j, double th1, dz *u, dz *z, double *dg,
+                               int *info) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  dz A[4], U[4], Z[4];
+  for (int k = 0; k < 4; ++k) {
+    A[k] = a[4 * i + k];
+    U[k] = zmk(0, 0);
+    Z[k] = zmk(0, 0);
+  }
+  int J[2] = {j[2 * i], j[2 * i + 1]};
+  int inf = zhsvd2<F>(A, J, U, Z, th1);
+  for (int k = 0; k < 4; ++k) {
+    u[4 * i + k] = U[k];
+    z[4 * i + k] = Z[k];
+  }
+  dg[2 * i] = A[0].re;
+  dg[2 * i + 1] = A[3].re;
+  info[i] = inf;
+}
+
+template <class T, bool CPLX>
+static int hsvd2_batch(jkg_ctx *c, int count, const double *a, const int32_t *j, double th1, int fused, double *u,
+                       double *z, double *dg, int32_t *info) {
+  if (!c) return -1;
+  if (count < 0) return -2;
+  if (count == 0) return JKG_OK;
+  HIPCHK(hipSetDevice(c->device));
+  const size_t esz = CPLX ? 16 : 8;
+  const size_t ba = esz * 4 * count, bj = sizeof(int) * 2 * count, bd = sizeof(double) * 2 * count,
+               bi = sizeof(int) * count;
+  int rc = tmp_bytes(c, 3 * ba + bj + bd + bi + 256);
+  if (rc) return rc;
+  char *base = (char *)c->tmp;
+  char *da = base, *du = da + ba, *dzp = du + ba, *dj = dzp + ba, *dd = dj + bj, *di = dd + bd;
+  HIPCHK(hipMemcpyAsync(da, a, ba, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(dj, j, bj, hipMemcpyHostToDevice, c->stream));
+  const int g = (count + 127) / 128;
+  if (CPLX) {
+    if (fused)
+      hipLaunchKernelGGL(k_zhsvd2_batch<true>, dim3(g), dim3(128), 0, c->stream, count, (const dz *)da,
+                         (const int *)dj, th1, (dz *)du, (dz *)dzp, (double *)dd, (int *)di);
+    else
+      hipLaunchKernelGGL(k_zhsvd2_batch<false>, dim3(g), dim3(128), 0, c->stream, count, (const dz *)da,
+                         (const int *)dj, th1, (dz *)du, (dz *)dzp, (double *)dd, (int *)di);
+  } else {
+    if (fused)
+      hipLaunchKernelGGL(k_dhsvd2_batch<true>, dim3(g), dim3(128), 0, c->stream, count, (const double *)da,
+                         (const int *)dj, th1, (double *)du, (double *)dzp, (double *)dd, (int *)di);
+    else
+      hipLaunchKernelGGL(k_dhsvd2_batch<false>, dim3(g), dim3(128), 0, c->stream, count, (const double *)da,
+                         (const int *)dj, th1, (double *)du, (double *)dzp, (double *)dd, (int *)di);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(u, du, ba, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(z, dzp, ba, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(dg, dd, bd, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(info, di, bi, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return JKG_OK;
+}
+
+extern "C" int jkg_dhsvd2(jkg_ctx *c, int count, const double *a, const int32_t *j, double th1, int fused, double *u,
+                          double *z, double *d, int32_t *info) {
+  return hsvd2_batch<double, false>(c, count, a, j, th1, fused, u, z, d, info);
+}
+extern "C" int jkg_zhsvd2(jkg_ctx *c, int count, const double *a, const int32_t *j, double th1, int fused, double *u,
+                          double *z, double *d, int32_t *info) {
+  return hsvd2_batch<dz, true>(c, count, a, j, th1, fused, u, z, d, info);
+}
+
+// Prepares the device state of one step on (a, j) without running it.
+static int prepare_single(jkg_ctx *c, int n, int n2, const double *a, const int64_t *j, int fused) {
+  if (n < 3) return -2;
+  HIPCHK(hipSetDevice(c->device));
+  int rc = ensure_buffers(c, n, 0, 1);
+  if (rc) return rc;
+  c->d.fused = fused;
+  int npos, nneg;
+  return upload_problem(c, n, jkg_jstep_len(n, n2), a, j, &npos, &nneg, -1, c->trace_cap);
+}
+
+extern "C" int jkg_dweights(jkg_ctx *c, int n, const double *a, const int64_t *j, int fused, uint64_t *keys) {
+  if (!c) return -1;
+  int rc = prepare_single(c, n, 0, a, j, fused);
+  if (rc) return rc;
+  step_begin_launch(c->d, c->stream);
+  weights_launch(c->d, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(keys, c->d.sort.k0, sizeof(uint64_t) * c->d.nn, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return JKG_OK;
+}
+
+extern "C" int jkg_zweights(jkg_ctx *c, int n, const double *a, const int64_t *j, int fused, uint64_t *keys) {
+  (void)c; (void)n; (void)a; (void)j; (void)fused; (void)keys;
+  return JKG_ENOTSUP;
+}
+
+static void decode_pair(int n, uint32_t k, int &p, int &q) {
+  // host twin of pair_of(): largest m with m(m-1)/2 <= k
+  uint32_t m = (uint32_t)((1.0 + sqrt(1.0 + 8.0 * (double)k)) * 0.5);
+  while ((uint64_t)m * (m - 1) / 2 > k) --m;
+  while ((uint64_t)(m + 1) * m / 2 <= k) ++m;
+  uint32_t r = k - m * (m - 1) / 2;
+  p = (int)(m - r);
+  q = p + n - (int)m;
+}
+
+extern "C" int jkg_dbuild(jkg_ctx *c, int n, int n2, const double *a, const int64_t *j, int fused, int32_t *it,
+                          jkg_aw *sorted, int32_t *sl, int32_t *step) {
+  if (!c) return -1;
+  int rc = prepare_single(c, n, n2, a, j, fused);
+  if (rc) return rc;
+  DeviceStep &d = c->d;
+  step_begin_launch(d, c->stream);
+  weights_launch(d, c->stream);
+  sort_pairs_launch(d.sort, &d.st->nn_sort, d.nn, c->stream);
+  select_launch(d, c->stream);
+  HIPCHK(hipGetLastError());
+  StepState s;
+  int skip[9];
+  HIPCHK(hipMemcpyAsync(&s, d.st, sizeof(s), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(skip, d.sort_skip, sizeof(skip), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  int moved = 0;
+  for (int i = 0; i < 8; ++i) moved += skip[i] ? 0 : 1;
+  const uint64_t *kb = (moved & 1) ? d.sort.k1 : d.sort.k0;
+  const uint32_t *vb = (moved & 1) ? d.sort.v1 : d.sort.v0;
+  std::vector<uint64_t> keys(s.it);
+  std::vector<uint32_t> vals(s.it);
+  if (s.it > 0) {
+    HIPCHK(hipMemcpy(keys.data(), kb, sizeof(uint64_t) * s.it, hipMemcpyDeviceToHost));
+    if (moved) HIPCHK(hipMemcpy(vals.data(), vb, sizeof(uint32_t) * s.it, hipMemcpyDeviceToHost));
+    else
+      for (int i = 0; i < s.it; ++i) vals[i] = (uint32_t)i;
+  }
+  for (int i = 0; i < s.it; ++i) {
+    int p, q;
+    decode_pair(n, vals[i], p, q);
+    sorted[i].w = jkg_key_weight(keys[i]);
+    sorted[i].p = p;
+    sorted[i].q = q;
+    sorted[i].b = q - p;
+    sorted[i].i = (int32_t)vals[i];
+  }
+  *it = s.it;
+  *sl = s.sl;
+  if (s.sl > 0) HIPCHK(hipMemcpy(step, d.piv_pos, sizeof(int) * s.sl, hipMemcpyDeviceToHost));
+  return JKG_OK;
+}
+
+extern "C" int jkg_dstep(jkg_ctx *c, int n, int n2, double *a, const int64_t *j, double *u, double *z, int fused,
+                         jkg_steprec *rec) {
+  if (!c) return -1;
+  int rc = prepare_single(c, n, n2, a, j, fused);
+  if (rc) return rc;
+  DeviceStep &d = c->d;
+  HIPCHK(hipMemcpyAsync(d.U, u, sizeof(double) * n * n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d.Z, z, sizeof(double) * n * n, hipMemcpyHostToDevice, c->stream));
+  step_launch(d, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(a, d.A, sizeof(double) * n * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(u, d.U, sizeof(double) * n * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(z, d.Z, sizeof(double) * n * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(rec, d.trace, sizeof(jkg_steprec), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return JKG_OK;
+}
+
+// ---------------------------------------------------------------- DPROC plug-ins
+__global__ void k_plugin_pairkeys(int n, int nn, const int *pq, uint64_t *k0) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nn) k0[i] = pair_index(n, pq[2 * i], pq[2 * i + 1]);
+}
+__global__ void k_plugin_wkeys(int nn, const double *w, const uint64_t *kin, const uint32_t *vin, int iota,
+                               uint64_t *k0, uint32_t *v0) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  uint32_t src = iota ? (uint32_t)i : vin[i];
+  k0[i] = weight_key(w[src]);
+  v0[i] = src;
+}
+
+// ASRT: sort by (pair index) then stably by weight key == AW_CMP order.
+extern "C" int jkg_srt(jkg_ctx *c, int nn, int nm, jkg_aw *dz, int64_t *info) {
+  if (!c) return -1;
+  if (nn < 0) { *info = -1; return -2; }
+  if (nm < nn) { *info = -2; return -3; }
+  *info = 0;
+  if (nn == 0) return JKG_OK;
+  HIPCHK(hipSetDevice(c->device));
+  int n = 0;
+  for (int i = 0; i < nn; ++i) n = std::max(n, (int)dz[i].q);
+  std::vector<int> pq(2 * (size_t)nn);
+  std::vector<double> w(nn);
+  for (int i = 0; i < nn; ++i) {
+    pq[2 * i] = dz[i].p;
+    pq[2 * i + 1] = dz[i].q;
+    w[i] = dz[i].w;
+    if (dz[i].p < 1 || dz[i].q <= dz[i].p) return -4;
+  }
+  // buffers: keys/vals x2, scratch, pq, w, n
+  const size_t bk = sizeof(uint64_t) * nn, bv = sizeof(uint32_t) * nn;
+  const size_t bs = sort_scratch_bytes(nn), bpq = sizeof(int) * 2 * nn, bw = sizeof(double) * nn;
+  int rc = tmp_bytes(c, 2 * bk + 2 * bv + bs + bpq + bw + 64 * 8);
+  if (rc) return rc;
+  char *p = (char *)c->tmp;
+  auto take = [&](size_t b) {
+    char *r = p;
+    p += (b + 255) & ~(size_t)255;
+    return r;
+  };
+  SortBuffers sb;
+  sb.k0 = (uint64_t *)take(bk);
+  sb.k1 = (uint64_t *)take(bk);
+  sb.v0 = (uint32_t *)take(bv);
+  sb.v1 = (uint32_t *)take(bv);
+  sb.scratch = (int *)take(bs);
+  int *dpq = (int *)take(bpq);
+  double *dw = (double *)take(bw);
+  int *dn = (int *)take(sizeof(int));
+  int *skip = sort_skip_flags(sb, nn);
+  HIPCHK(hipMemcpyAsync(dpq, pq.data(), bpq, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(dw, w.data(), bw, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(dn, &nn, sizeof(int), hipMemcpyHostToDevice, c->stream));
+  const int g = (nn + 255) / 256;
+  hipLaunchKernelGGL(k_plugin_pairkeys, dim3(g), dim3(256), 0, c->stream, n, nn, dpq, sb.k0);
+  sort_pairs_launch(sb, dn, nn, c->stream, 0);
+  int sk[9];
+  HIPCHK(hipMemcpyAsync(sk, skip, sizeof(sk), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  int moved = 0;
+  for (int i = 0; i < 8; ++i) moved += sk[i] ? 0 : 1;
+  // second sort: weight keys over the first permutation, values given
+  const uint64_t *kin = (moved & 1) ? sb.k1 : sb.k0;
+  const uint32_t *vin = (moved & 1) ? sb.v1 : sb.v0;
+  uint64_t *kt = (moved & 1) ? sb.k0 : sb.k1;  // free buffer for the new keys
+  uint32_t *vt = (moved & 1) ? sb.v0 : sb.v1;
+  hipLaunchKernelGGL(k_plugin_wkeys, dim3(g), dim3(256), 0, c->stream, nn, dw, kin, vin, moved == 0 ? 1 : 0, kt, vt);
+  SortBuffers s2 = sb;
+  s2.k0 = kt;
+  s2.v0 = vt;
+  s2.k1 = (uint64_t *)kin;
+  s2.v1 = (uint32_t *)vin;
+  sort_pairs_launch(s2, dn, nn, c->stream, 1);
+  HIPCHK(hipMemcpyAsync(sk, skip, sizeof(sk), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  moved = 0;
+  for (int i = 0; i < 8; ++i) moved += sk[i] ? 0 : 1;
+  std::vector<uint32_t> perm(nn);
+  HIPCHK(hipMemcpy(perm.data(), (moved & 1) ? s2.v1 : s2.v0, bv, hipMemcpyDeviceToHost));
+  std::vector<jkg_aw> out(nn);
+  for (int i = 0; i < nn; ++i) out[i] = dz[perm[i]];
+  memcpy(dz, out.data(), sizeof(jkg_aw) * nn);
+  return JKG_OK;
+}
+
+// ANCP (AW_NCP1/AW_NCP2): the device's greedy walk over a given sorted list.
+extern "C" int jkg_ncp(jkg_ctx *c, int n, int nn, int nm, jkg_aw *dz, int n2, int32_t *sl, int32_t *step,
+                       int64_t *info) {
+  if (!c) return -1;
+  *sl = 0;
+  *info = 0;
+  if (n < 0) { *info = -1; return -2; }
+  if (nn < 0) { *info = -2; return -3; }
+  if (nm < nn) { *info = -3; return -4; }
+  if (n2 < 0) { *info = -5; return -7; }
+  if (n == 0 || nn == 0 || n2 == 0) return JKG_OK;
+  for (int i = 0; i < nn; ++i)
+    if (!(dz[i].w == dz[i].w) || dz[i].p < 1 || dz[i].q <= dz[i].p || dz[i].q > n) return -5;
+  HIPCHK(hipSetDevice(c->device));
+  int nb = std::max(n, 3);
+  int rc = ensure_buffers(c, nb, 0, 1);
+  if (rc) return rc;
+  if ((size_t)nn > (size_t)c->d.nn || n2 > c->d.n2max) {
+    // the candidate list holds at most n(n-1)/2 distinct pairs
+    return -3;
+  }
+  DeviceStep &d = c->d;
+  std::vector<uint64_t> key(nn);
+  std::vector<uint32_t> val(nn), pos(nn);
+  for (int i = 0; i < nn; ++i) {
+    double w = dz[i].w;
+    uint64_t u;
+    memcpy(&u, &w, 8);
+    if (w == 0.0) u = 0;
+    uint64_t ord = (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+    key[i] = ~ord;
+    val[i] = pair_index(nb, dz[i].p, dz[i].q);
+    pos[i] = (uint32_t)(i + 1);
+  }
+  HIPCHK(hipMemcpyAsync(d.lists.key[1], key.data(), 8 * (size_t)nn, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d.lists.val[1], val.data(), 4 * (size_t)nn, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d.lists.pos[1], pos.data(), 4 * (size_t)nn, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemsetAsync(d.used, 0, nb, c->stream));
+  StepState s{};
+  s.n2 = n2;
+  s.sel_round = 1;
+  s.list_len = nn;
+  s.active = 1;
+  HIPCHK(hipMemcpyAsync(d.st, &s, sizeof(s), hipMemcpyHostToDevice, c->stream));
+  // the lists of round 1 are read from index (1-1)&1 = 0: swap the pair
+  DeviceStep dd = d;
+  std::swap(dd.lists.key[0], dd.lists.key[1]);
+  std::swap(dd.lists.val[0], dd.lists.val[1]);
+  std::swap(dd.lists.pos[0], dd.lists.pos[1]);
+  select_launch(dd, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(&s, d.st, sizeof(s), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  *sl = s.sl;
+  if (s.sl > 0) HIPCHK(hipMemcpy(step, d.piv_pos, sizeof(int) * s.sl, hipMemcpyDeviceToHost));
+  return JKG_OK;
+}

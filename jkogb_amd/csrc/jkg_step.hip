@@ -1,0 +1,794 @@
+// jkg_step.hip -- one dynamic-ordering step of the real FP64 J-Kogbetliantz
+// HSVD on gfx950 (the reference's DSTEP_EXEC, src/D/dstep.F90:376-443).
+//
+//   begin   -- per-step control values, maps reset                    (1 WG)
+//   weights -- (a) trigonometric pairs: DASUM2 of the 2x2 off-diagonals;
+//              hyperbolic pairs: the 2x2 HSVD per pair, then the off-norm
+//              change of columns p,q (ABODNDF2) with the rows staged in LDS
+//   sort    -- (b) stable LSD radix sort of (key(W), pair index)
+//   select  -- (c) exact greedy walk of the sorted list (AW_NCP2): one
+//              workgroup resolves chunks in order with wave ballots against an
+//              LDS bitmap; between rounds every CU filters the rest of the list
+//              against the bitmap (order-preserving compaction), so the walk
+//              only ever visits candidates whose indices were free
+//   pivots  -- (d) batched 2x2 HSVD of the chosen pivots, rotation descriptors
+//   rotate  -- (e) one workgroup per column pair: both columns of A staged in
+//              LDS, all row transforms (BA), then the column transform (AB)
+//              and the new diagonal; columns of U and Z updated in place
+//   end     -- NEWLEN, NORMF2D, TRIG, HYP trace line; loop control  (1 WG)
+//
+// Everything follows the reference's operation order, so with identical
+// inputs the results are bit-identical to it.
+#include "jkg_hsvd2.hpp"
+#include "jkg_step.hpp"
+
+namespace jkg {
+
+// ------------------------------------------------------------------ helpers
+template <bool F>
+__device__ __forceinline__ double dasum2(double x1, double x2) {  // src/D/dtransf.F90:12-38
+  double a = __builtin_fabs(x1), b = __builtin_fabs(x2);
+  if (a >= b) {
+    if (a == 0.0) return 0.0;
+    return fmad<F>(b, (b / a), a) * a;
+  }
+  return fmad<F>(a, (a / b), b) * b;
+}
+
+// ratio-form 2x2 transform of the pair (x, y) (BA or AB, see Rot)
+template <bool F>
+__device__ __forceinline__ void rot_apply(const Rot &R, double &x, double &y) {
+  double xx = fmad<F>(R.sa ? y : x, R.r1, R.sa ? x : y) * R.m1;
+  double yy = fmad<F>(R.sb ? x : y, R.r2, R.sb ? y : x) * R.m2;
+  x = xx;
+  y = yy;
+}
+// descriptor of BA(B) (rows, src/D/dtransf.F90:121-183); equal to AB(B^T)
+__device__ __forceinline__ Rot rot_ba(const double B[4]) {
+  Rot R;
+  R.sa = __builtin_fabs(B[0]) >= __builtin_fabs(B[2]);
+  R.r1 = R.sa ? B[2] / B[0] : B[0] / B[2];
+  R.m1 = R.sa ? B[0] : B[2];
+  R.sb = __builtin_fabs(B[3]) >= __builtin_fabs(B[1]);
+  R.r2 = R.sb ? B[1] / B[3] : B[3] / B[1];
+  R.m2 = R.sb ? B[3] : B[1];
+  R.pad0 = R.pad1 = 0;
+  return R;
+}
+// descriptor of AB(B) (columns, src/D/dtransf.F90:187-242)
+__device__ __forceinline__ Rot rot_ab(const double B[4]) {
+  Rot R;
+  R.sa = __builtin_fabs(B[0]) >= __builtin_fabs(B[1]);
+  R.r1 = R.sa ? B[1] / B[0] : B[0] / B[1];
+  R.m1 = R.sa ? B[0] : B[1];
+  R.sb = __builtin_fabs(B[3]) >= __builtin_fabs(B[2]);
+  R.r2 = R.sb ? B[2] / B[3] : B[3] / B[2];
+  R.m2 = R.sb ? B[3] : B[2];
+  R.pad0 = R.pad1 = 0;
+  return R;
+}
+
+// ------------------------------------------------------------------ begin
+__global__ void k_step_begin(StepState *st, int nn, int n, int *colpiv) {
+  __shared__ int act;
+  if (threadIdx.x == 0) {
+    int a = !st->done;
+    if (a && st->max_steps >= 0 && st->step >= st->max_steps) {
+      st->done = 1;
+      st->info = st->step;
+      a = 0;
+    }
+    st->active = a;
+    st->nn_sort = a ? nn : 0;
+    st->sl = 0;
+    st->nl = 0;
+    st->err = 0;
+    st->nrow = 0;
+    st->sel_done = !a;
+    st->sel_round = 0;
+    st->consumed = 0;
+    st->list_len = 0;
+    act = a;
+  }
+  __syncthreads();
+  if (!act) return;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) colpiv[i] = -1;
+}
+
+// ------------------------------------------------------------------ (a) weights
+// DMAGF2T (src/D/dstep.F90:12-37) for the pairs with J_p == J_q.
+template <bool F>
+__global__ void __launch_bounds__(256) k_trig_weights(int n, const double *__restrict__ A, int lda,
+                                                       const int *__restrict__ J, uint64_t *__restrict__ keys,
+                                                       const StepState *st, int nn) {
+  if (!st->active) return;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < (uint32_t)nn; k += gridDim.x * blockDim.x) {
+    int p, q;
+    pair_of(n, k, p, q);
+    int jp = J[p - 1], jq = J[q - 1];
+    if (jp != jq) continue;  // hyperbolic: k_hyp_weights
+    const size_t P = p - 1, Q = q - 1;
+    double aqp = A[P * lda + Q], apq = A[Q * lda + P], app = A[P * lda + P], aqq = A[Q * lda + Q];
+    bool go = (aqp != 0.0) || (apq != 0.0) || sign_neg(app) || sign_neg(aqq) ||
+              ((jp + jq == -2) ? (aqq < app) : (app < aqq));
+    keys[k] = go ? weight_key(dasum2<F>(aqp, apq)) : KEY_NAN;
+  }
+}
+
+// DMAGF2H (src/D/dstep.F90:41-72) + ABODNDF2 (src/D/dtransf.F90:246-348) for
+// the pairs with J_p != J_q: one lane per pair of a 16 (J=+1) x 16 (J=-1)
+// column tile; rows are streamed through LDS in chunks of 64, coalesced.
+constexpr int HT = 16;
+constexpr int HR = 64;
+template <bool F>
+__global__ void __launch_bounds__(256) k_hyp_weights(int n, const double *__restrict__ A, int lda,
+                                                      const int *__restrict__ J, const int *__restrict__ plus,
+                                                      int npl, const int *__restrict__ minus, int nmi,
+                                                      uint64_t *__restrict__ keys, const StepState *st) {
+  if (!st->active) return;
+  __shared__ double L[HR][2 * HT + 1];
+  __shared__ int cols[2 * HT];
+  const int a = threadIdx.x / HT, b = threadIdx.x % HT;
+  const int ia = blockIdx.x * HT + a, ib = blockIdx.y * HT + b;
+  if (threadIdx.x < 2 * HT) {
+    int c = threadIdx.x;
+    int g = (c < HT) ? blockIdx.x * HT + c : blockIdx.y * HT + (c - HT);
+    cols[c] = (c < HT) ? (g < npl ? plus[g] : -1) : (g < nmi ? minus[g] : -1);
+  }
+  const bool valid = ia < npl && ib < nmi;
+  const int cp = valid ? plus[ia] : 0, cm = valid ? minus[ib] : 1;
+  const int p0 = cp < cm ? cp : cm, q0 = cp < cm ? cm : cp;  // 0-based, p0 < q0
+  const bool xplus = cp < cm;
+  bool need = false;
+  double d = 0.0, r1 = 0.0, r2 = 0.0, m1 = 0.0, m2 = 0.0;
+  bool sel1 = true, sel2 = true;
+  uint64_t key = KEY_NAN;
+  if (valid) {
+    const size_t P = p0, Q = q0;
+    double app = A[P * lda + P], aqp = A[P * lda + Q], apq = A[Q * lda + P], aqq = A[Q * lda + Q];
+    if (aqp != 0.0 || apq != 0.0 || sign_neg(app) || sign_neg(aqq)) {
+      double a2[4] = {app, aqp, apq, aqq}, u2[4], z2[4];
+      int j2[2] = {J[p0], J[q0]};
+      int info = dhsvd2<F>(a2, j2, u2, z2, TH1FIX);
+      if (info < 0) key = KEY_NAN;
+      else if ((info & 1) == 0) key = weight_key(0.0);
+      else if ((info & 4) == 0) key = weight_key(dasum2<F>(aqp, apq));
+      else {
+        need = true;
+        const double aq = (info & 8) ? a2[1] : 0.0, ap = (info & 8) ? a2[2] : 0.0;
+        d = fmad<F>(aqp - aq, aqp + aq, 0.0);
+        d = fmad<F>(apq - ap, apq + ap, d);
+        sel1 = __builtin_fabs(z2[0]) >= __builtin_fabs(z2[1]);
+        r1 = sel1 ? z2[1] / z2[0] : z2[0] / z2[1];
+        m1 = sel1 ? z2[0] : z2[1];
+        sel2 = __builtin_fabs(z2[3]) >= __builtin_fabs(z2[2]);
+        r2 = sel2 ? z2[2] / z2[3] : z2[3] / z2[2];
+        m2 = sel2 ? z2[3] : z2[2];
+      }
+    }
+  }
+  if (__syncthreads_or(need)) {
+    const int xc = xplus ? a : HT + b, yc = xplus ? HT + b : a;
+    const int s1c = sel1 ? yc : xc, t1c = sel1 ? xc : yc;
+    const bool eq = (sel1 == sel2);
+    for (int r0 = 0; r0 < n; r0 += HR) {
+      __syncthreads();
+      for (int e = threadIdx.x; e < HR * 2 * HT; e += 256) {
+        const int c = e / HR, r = e % HR, col = cols[c];
+        L[r][c] = (col >= 0 && r0 + r < n) ? A[(size_t)col * lda + r0 + r] : 0.0;
+      }
+      __syncthreads();
+      if (need) {
+        const int rmax = (n - r0 < HR) ? n - r0 : HR;
+        for (int r = 0; r < rmax; ++r) {
+          const double x = L[r][xc], y = L[r][yc], s1 = L[r][s1c], t1 = L[r][t1c];
+          const double s2 = eq ? t1 : s1, t2 = eq ? s1 : t1;
+          const double xx = fmad<F>(s1, r1, t1) * m1;
+          const double yy = fmad<F>(s2, r2, t2) * m2;
+          double dn = fmad<F>(x - xx, x + xx, d);
+          dn = fmad<F>(y - yy, y + yy, dn);
+          const int row = r0 + r;
+          d = (row == p0 || row == q0) ? d : dn;
+        }
+      }
+    }
+    if (need) key = (d < -1.7976931348623157e308) ? KEY_NAN : weight_key(d);  // -inf removed
+  }
+  if (valid) keys[pair_index(n, p0 + 1, q0 + 1)] = key;
+}
+
+// ------------------------------------------------------------------ (c) select
+constexpr int SEL_THREADS = 256;
+constexpr int FILT_TILE = 2048;
+
+// One workgroup walks the current list in order: chunks of 256 entries are
+// tested against the LDS bitmap in parallel, the surviving candidates are
+// compacted (in order) and wave 0 accepts them one after another with
+// ballots.  The walk stops after `limit` entries (then the list is filtered),
+// at the end of the list, or when N_2 pivots are taken.
+__global__ void __launch_bounds__(SEL_THREADS) k_sel_resolve(int n, const uint64_t *k0, const uint64_t *k1,
+                                                             const uint32_t *v0, const uint32_t *v1,
+                                                             const int *sort_skip, ListPtrs lp,
+                                                             uint8_t *used_g, int *piv_p, int *piv_q, int *piv_pos,
+                                                             double *piv_w, StepState *st, int round, int limit,
+                                                             int nn) {
+  extern __shared__ uint8_t used[];  // n bytes
+  __shared__ int cand_n;
+  __shared__ int acc;
+  __shared__ int cp[SEL_THREADS], cq[SEL_THREADS], cpos[SEL_THREADS];
+  __shared__ uint64_t ckey[SEL_THREADS];
+  __shared__ int wcount[SEL_THREADS / 64];
+  __shared__ int itv;
+  if (st->sel_done || st->sel_round != round) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  SortResult sr = sort_result_dev(k0, k1, v0, v1, sort_skip);
+  const uint64_t *keys = (round == 0) ? sr.keys : lp.key[(round - 1) & 1];
+  const uint32_t *vals = (round == 0) ? sr.vals : lp.val[(round - 1) & 1];
+  const uint32_t *poss = (round == 0) ? nullptr : lp.pos[(round - 1) & 1];
+  const bool iota = (round == 0) && sr.vals_iota;
+  int len;
+  if (round == 0) {
+    // IT = number of finite weights = position of the first NaN key
+    if (tid == 0) {
+      int lo = 0, hi = nn;  // first index with key == KEY_NAN
+      while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (keys[mid] == KEY_NAN) hi = mid;
+        else lo = mid + 1;
+      }
+      itv = lo;
+      st->it = lo;
+      st->list_len = lo;
+    }
+    for (int i = tid; i < n; i += SEL_THREADS) used[i] = 0;
+    __syncthreads();
+    len = itv;
+  } else {
+    for (int i = tid; i < n; i += SEL_THREADS) used[i] = used_g[i];
+    len = st->list_len;
+  }
+  if (tid == 0) acc = st->sl;
+  __syncthreads();
+  const int n2 = st->n2;
+  int c = st->consumed;
+  const int stop = (limit > 0 && len - c > limit) ? c + limit : len;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  while (c < stop && acc < n2) {
+    const int e = c + tid;
+    bool cand = false;
+    int p = 0, q = 0, pos = 0;
+    uint64_t key = 0;
+    if (e < stop) {
+      uint32_t kk = iota ? (uint32_t)e : vals[e];
+      pair_of(n, kk, p, q);
+      --p;
+      --q;
+      key = keys[e];
+      pos = poss ? (int)poss[e] : e + 1;
+      cand = !used[p] && !used[q];
+    }
+    // in-order compaction of the candidates of this chunk
+    uint64_t bal = __ballot(cand);
+    if (lane == 0) wcount[wave] = __popcll(bal);
+    __syncthreads();
+    int off = 0;
+    for (int w = 0; w < wave; ++w) off += wcount[w];
+    if (cand) {
+      int slot = off + __popcll(bal & lt);
+      cp[slot] = p;
+      cq[slot] = q;
+      cpos[slot] = pos;
+      ckey[slot] = key;
+    }
+    if (tid == 0) {
+      int t = 0;
+      for (int w = 0; w < SEL_THREADS / 64; ++w) t += wcount[w];
+      cand_n = t;
+    }
+    __syncthreads();
+    if (wave == 0) {
+      const int nc = cand_n;
+      int a = acc;
+      for (int g = 0; g < nc && a < n2; g += 64) {
+        const int s = g + lane;
+        int pp = -1, qq = -1;
+        bool ok = false;
+        if (s < nc) {
+          pp = cp[s];
+          qq = cq[s];
+          ok = !used[pp] && !used[qq];
+        }
+        uint64_t m = __ballot(ok);
+        while (m && a < n2) {
+          const int l0 = __ffsll((long long)m) - 1;
+          const int pf = __shfl(pp, l0), qf = __shfl(qq, l0);
+          if (lane == 0) {
+            used[pf] = 1;
+            used[qf] = 1;
+            piv_p[a] = pf;
+            piv_q[a] = qf;
+            piv_pos[a] = cpos[g + l0];
+            piv_w[a] = key_weight(ckey[g + l0]);
+          }
+          ++a;
+          ok = ok && (lane > l0) && pp != pf && pp != qf && qq != pf && qq != qf;
+          m = __ballot(ok);
+        }
+      }
+      if (lane == 0) acc = a;
+    }
+    __syncthreads();
+    c = (c + SEL_THREADS < stop) ? c + SEL_THREADS : stop;
+  }
+  for (int i = tid; i < n; i += SEL_THREADS) used_g[i] = used[i];
+  if (tid == 0) {
+    st->sl = acc;
+    st->consumed = c;
+    if (acc >= n2 || c >= len) st->sel_done = 1;
+  }
+}
+
+// Filter: survivors of [consumed, len) whose indices are both free, counted
+// per tile of FILT_TILE entries ...
+__global__ void __launch_bounds__(256) k_sel_filter_count(int n, const uint64_t *k0, const uint64_t *k1,
+                                                          const uint32_t *v0, const uint32_t *v1,
+                                                          const int *sort_skip, ListPtrs lp,
+                                                          const uint8_t *used_g, int *fcount, const StepState *st,
+                                                          int round) {
+  extern __shared__ uint8_t used[];
+  __shared__ int wc[4];
+  if (st->sel_done || st->sel_round != round) return;
+  const int c0 = st->consumed, len = st->list_len;
+  const int base = c0 + blockIdx.x * FILT_TILE;
+  if (base >= len) {
+    if (threadIdx.x == 0) fcount[blockIdx.x] = 0;
+    return;
+  }
+  for (int i = threadIdx.x; i < n; i += blockDim.x) used[i] = used_g[i];
+  __syncthreads();
+  SortResult sr = sort_result_dev(k0, k1, v0, v1, sort_skip);
+  const uint32_t *vals = (round == 0) ? sr.vals : lp.val[(round - 1) & 1];
+  const bool iota = (round == 0) && sr.vals_iota;
+  int cnt = 0;
+  for (int e = base + threadIdx.x; e < base + FILT_TILE && e < len; e += blockDim.x) {
+    int p, q;
+    pair_of(n, iota ? (uint32_t)e : vals[e], p, q);
+    cnt += (!used[p - 1] && !used[q - 1]) ? 1 : 0;
+  }
+  for (int o = 32; o > 0; o >>= 1) cnt += __shfl_down(cnt, o);
+  if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) fcount[blockIdx.x] = wc[0] + wc[1] + wc[2] + wc[3];
+}
+
+// ... exclusive scan of the tile counts (one workgroup) ...
+__global__ void __launch_bounds__(1024) k_sel_filter_scan(int *fcount, int ntiles, StepState *st, int round) {
+  __shared__ int part[1024];
+  if (st->sel_done || st->sel_round != round) return;
+  const int per = (ntiles + 1023) / 1024;
+  const int lo = threadIdx.x * per;
+  int s = 0;
+  for (int i = 0; i < per; ++i)
+    if (lo + i < ntiles) s += fcount[lo + i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    int v = (threadIdx.x >= off) ? part[threadIdx.x - off] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int run = part[threadIdx.x] - s;
+  for (int i = 0; i < per; ++i)
+    if (lo + i < ntiles) {
+      int v = fcount[lo + i];
+      fcount[lo + i] = run;
+      run += v;
+    }
+  if (threadIdx.x == 1023) fcount[ntiles] = part[1023];
+}
+
+// ... and written in order to the next list.
+__global__ void __launch_bounds__(256) k_sel_filter_write(int n, const uint64_t *k0, const uint64_t *k1,
+                                                          const uint32_t *v0, const uint32_t *v1,
+                                                          const int *sort_skip, ListPtrs lp,
+                                                          const uint8_t *used_g, const int *fcount,
+                                                          const StepState *st, int round) {
+  extern __shared__ uint8_t used[];
+  __shared__ int wc[4];
+  __shared__ int run;
+  if (st->sel_done || st->sel_round != round) return;
+  const int c0 = st->consumed, len = st->list_len;
+  const int base = c0 + blockIdx.x * FILT_TILE;
+  if (base < len) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) used[i] = used_g[i];
+    if (threadIdx.x == 0) run = fcount[blockIdx.x];
+    __syncthreads();
+    SortResult sr = sort_result_dev(k0, k1, v0, v1, sort_skip);
+    const uint64_t *keys = (round == 0) ? sr.keys : lp.key[(round - 1) & 1];
+    const uint32_t *vals = (round == 0) ? sr.vals : lp.val[(round - 1) & 1];
+    const uint32_t *poss = (round == 0) ? nullptr : lp.pos[(round - 1) & 1];
+    const bool iota = (round == 0) && sr.vals_iota;
+    uint64_t *okey = lp.key[round & 1];
+    uint32_t *oval = lp.val[round & 1], *opos = lp.pos[round & 1];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int e0 = base; e0 < base + FILT_TILE && e0 < len; e0 += blockDim.x) {
+      const int e = e0 + threadIdx.x;
+      bool keep = false;
+      uint32_t kk = 0;
+      if (e < base + FILT_TILE && e < len) {
+        kk = iota ? (uint32_t)e : vals[e];
+        int p, q;
+        pair_of(n, kk, p, q);
+        keep = !used[p - 1] && !used[q - 1];
+      }
+      uint64_t bal = __ballot(keep);
+      if (lane == 0) wc[wave] = __popcll(bal);
+      __syncthreads();
+      int off = run;
+      for (int w = 0; w < wave; ++w) off += wc[w];
+      if (keep) {
+        int o = off + __popcll(bal & lt);
+        okey[o] = keys[e];
+        oval[o] = kk;
+        opos[o] = poss ? poss[e] : (uint32_t)(e + 1);
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) run += wc[0] + wc[1] + wc[2] + wc[3];
+      __syncthreads();
+    }
+  }
+}
+
+// list bookkeeping for the next round (fcount[ntiles] holds the total)
+__global__ void k_sel_filter_end(const int *fcount, int ntiles, StepState *st, int round) {
+  if (st->sel_done || st->sel_round != round) return;
+  st->list_len = fcount[ntiles];
+  st->consumed = 0;
+  st->sel_round = round + 1;
+  if (fcount[ntiles] == 0) st->sel_done = 1;
+}
+
+// ------------------------------------------------------------------ (d) pivots
+template <bool F>
+__global__ void __launch_bounds__(256) k_pivots(int n, const double *__restrict__ A, int lda,
+                                                 const int *__restrict__ J, const int *__restrict__ piv_p,
+                                                 const int *__restrict__ piv_q, int *piv_info, double *piv_V,
+                                                 double *piv_W, double *piv_d, Rot *rot_row, Rot *rot_col,
+                                                 int *colpiv, StepState *st) {
+  if (!st->active) return;
+  const int sl = st->sl;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= sl) return;
+  const int p = piv_p[i], q = piv_q[i];
+  const size_t P = p, Q = q;
+  double b[4] = {A[P * lda + P], A[P * lda + Q], A[Q * lda + P], A[Q * lda + Q]};
+  int k[2] = {J[p], J[q]};
+  double v[4], w[4];
+  int info = dhsvd2<F>(b, k, v, w, TH1FIX);
+  piv_info[i] = info;
+  colpiv[p] = i;
+  colpiv[q] = -(i + 2);
+  if (info < 0) {
+    atomicMin(&st->err, info);
+    return;
+  }
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    piv_V[4 * i + t] = v[t];
+    piv_W[4 * i + t] = w[t];
+  }
+  piv_d[2 * i] = b[0];
+  piv_d[2 * i + 1] = b[3];
+  rot_row[i] = rot_ba(v);
+  rot_col[i] = rot_ab(w);
+  if (info & 2) atomicAdd(&st->nrow, 1);
+}
+
+// ------------------------------------------------------------------ (e) rotate
+template <bool F>
+__global__ void __launch_bounds__(512) k_rotate(int n, double *__restrict__ A, int lda, double *__restrict__ U,
+                                                double *__restrict__ Z, const int *__restrict__ piv_p,
+                                                const int *__restrict__ piv_q, const int *__restrict__ piv_info,
+                                                const double *__restrict__ piv_d, const Rot *__restrict__ rot_row,
+                                                const Rot *__restrict__ rot_col, const int *__restrict__ colpiv,
+                                                const StepState *st) {
+  extern __shared__ double L[];  // column c in L[0..n), its partner in L[n..2n)
+  if (!st->active) return;
+  const int sl = st->sl;
+  if (sl <= 0) return;
+  const int c = blockIdx.x;
+  const int j = colpiv[c];
+  if (j <= -2) return;  // the partner column of a pivot: done by its p column
+  const int err = st->err, nrow = st->nrow;
+  const int info = (j >= 0) ? piv_info[j] : -1;
+  const int c1 = (j >= 0) ? piv_q[j] : -1;
+  const bool col_rot = (j >= 0) && err == 0 && info >= 0 && (info & 4);
+  const bool diag_set = (j >= 0) && err == 0 && info >= 0 && !(info & 8);
+  const int tid = threadIdx.x, bs = blockDim.x;
+  if (nrow > 0 || col_rot || diag_set) {
+    double *col0 = A + (size_t)c * lda;
+    double *col1 = (c1 >= 0) ? A + (size_t)c1 * lda : nullptr;
+    for (int r = tid; r < n; r += bs) {
+      L[r] = col0[r];
+      if (col1) L[n + r] = col1[r];
+    }
+    __syncthreads();
+    if (nrow > 0) {
+      // phase 1 on A: rows p_i, q_i of every pivot with a row transform
+      for (int i = tid; i < sl; i += bs) {
+        const int inf = piv_info[i];
+        if (inf < 0 || !(inf & 2)) continue;
+        const Rot R = rot_row[i];
+        const int p = piv_p[i], q = piv_q[i];
+        double x = L[p], y = L[q];
+        rot_apply<F>(R, x, y);
+        L[p] = x;
+        L[q] = y;
+        if (col1) {
+          x = L[n + p];
+          y = L[n + q];
+          rot_apply<F>(R, x, y);
+          L[n + p] = x;
+          L[n + q] = y;
+        }
+      }
+      __syncthreads();
+    }
+    if (col1) {
+      // phase 2 on A: the column transform and the new 2x2 diagonal
+      const int p = c, q = c1;
+      Rot R;
+      if (col_rot) R = rot_col[j];
+      const double d0 = diag_set ? piv_d[2 * j] : 0.0, d1 = diag_set ? piv_d[2 * j + 1] : 0.0;
+      for (int r = tid; r < n; r += bs) {
+        double x = L[r], y = L[n + r];
+        if (col_rot) rot_apply<F>(R, x, y);
+        if (diag_set) {
+          if (r == p) { x = d0; y = 0.0; }
+          if (r == q) { x = 0.0; y = d1; }
+        }
+        col0[r] = x;
+        col1[r] = y;
+      }
+    } else {
+      for (int r = tid; r < n; r += bs) col0[r] = L[r];
+    }
+  }
+  // phase 1 on U (columns p,q times V^T) and Z (columns p,q times W)
+  if (j >= 0 && info >= 0) {
+    if (info & 2) {
+      const Rot R = rot_row[j];
+      double *u0 = U + (size_t)c * lda, *u1 = U + (size_t)c1 * lda;
+      for (int r = tid; r < n; r += bs) {
+        double x = u0[r], y = u1[r];
+        rot_apply<F>(R, x, y);
+        u0[r] = x;
+        u1[r] = y;
+      }
+    }
+    if (info & 4) {
+      const Rot R = rot_col[j];
+      double *z0 = Z + (size_t)c * lda, *z1 = Z + (size_t)c1 * lda;
+      for (int r = tid; r < n; r += bs) {
+        double x = z0[r], y = z1[r];
+        rot_apply<F>(R, x, y);
+        z0[r] = x;
+        z1[r] = y;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ end
+// NEWLEN, NORMF2D (summed in pivot order from -0, as one thread of the
+// reference does), TRIG, HYP; then SL = MIN(SL, NL) and the loop test.
+__global__ void __launch_bounds__(256) k_step_end(const int *__restrict__ J, const int *__restrict__ piv_p,
+                                                   const int *__restrict__ piv_q, const int *__restrict__ piv_info,
+                                                   const double *__restrict__ piv_w, StepState *st, StepRec *trace) {
+  __shared__ int cnt[3];
+  if (!st->active) return;
+  const int sl = st->sl, s = st->step;
+  StepRec rec;
+  rec.step = s;
+  rec.oldlen = sl;
+  rec.it = st->it;
+  if (sl <= 0) {
+    if (threadIdx.x == 0) {
+      rec.newlen = -1;
+      rec.tw = -0.0;
+      rec.trig = rec.hyp = 0;
+      if (s < st->trace_cap) trace[s] = rec;
+      st->nrec = s + 1;
+      st->done = 1;
+      st->info = s;
+    }
+    return;
+  }
+  const int err = st->err;
+  if (threadIdx.x < 3) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  if (err == 0) {
+    int nl = 0, nt = 0, nh = 0;
+    for (int i = threadIdx.x; i < sl; i += blockDim.x) {
+      const int l = piv_info[i];
+      if ((l & 1) && ((l & 2) || (l & 4))) ++nl;
+      if ((l & 2) || (l & 4)) {
+        if (J[piv_p[i]] == J[piv_q[i]]) ++nt;
+        else ++nh;
+      }
+    }
+    atomicAdd(&cnt[0], nl);
+    atomicAdd(&cnt[1], nt);
+    atomicAdd(&cnt[2], nh);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tw = -0.0;
+    if (err == 0)
+      for (int i = 0; i < sl; ++i) {
+        const int l = piv_info[i];
+        if ((l & 2) || (l & 4)) tw = tw + piv_w[i];
+      }
+    const int nl = (err == 0) ? cnt[0] : err;
+    rec.newlen = nl;
+    rec.tw = tw;
+    rec.trig = (err == 0) ? cnt[1] : 0;
+    rec.hyp = (err == 0) ? cnt[2] : 0;
+    if (s < st->trace_cap) trace[s] = rec;
+    st->nrec = s + 1;
+    st->nl = nl;
+    const int sl2 = (nl < sl) ? nl : sl;
+    if (sl2 <= 0) {
+      st->done = 1;
+      st->info = (sl2 < 0) ? sl2 : s;
+    } else {
+      st->step = s + 1;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ setup / finalize
+__global__ void k_identity(double *U, double *Z, int n, int lda) {
+  const size_t total = (size_t)n * n;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t c = e / n, r = e % n;
+    const double v = (r == c) ? 1.0 : 0.0;
+    U[c * lda + r] = v;
+    Z[c * lda + r] = v;
+  }
+}
+
+// SIGMA = diag(A), A(i,i) = 0, Z <- J Z^T J (src/D/dstep.F90:447-468, 594-603)
+__global__ void k_finalize(double *A, double *Z, int n, int lda, const int *J, double *sigma) {
+  const size_t total = (size_t)n * n;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const int c = (int)(e / n), r = (int)(e % n);
+    if (r == c) {
+      sigma[r] = A[(size_t)c * lda + r];
+      A[(size_t)c * lda + r] = 0.0;
+    } else if (r < c) {  // one lane per pair (r, c), r < c
+      double *zrc = &Z[(size_t)c * lda + r], *zcr = &Z[(size_t)r * lda + c];
+      double t = *zrc;
+      double a = *zcr, b = t;
+      if (J[r] != J[c]) {
+        a = -a;
+        b = -b;
+      }
+      *zrc = a;
+      *zcr = b;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ launchers
+static inline int grid_for(size_t work, int bs, int cap) {
+  size_t g = (work + bs - 1) / bs;
+  if (g < 1) g = 1;
+  return (int)((g > (size_t)cap) ? cap : g);
+}
+
+void step_begin_launch(DeviceStep &d, hipStream_t s) {
+  hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(256), 0, s, d.st, d.nn, d.n, d.colpiv);
+}
+
+void weights_launch(DeviceStep &d, hipStream_t s) {
+  const double *A = (const double *)d.A;
+  if (d.fused) {
+    hipLaunchKernelGGL(k_trig_weights<true>, dim3(grid_for(d.nn, 256, 65536)), dim3(256), 0, s, d.n, A, d.lda, d.J,
+                       d.sort.k0, d.st, d.nn);
+  } else {
+    hipLaunchKernelGGL(k_trig_weights<false>, dim3(grid_for(d.nn, 256, 65536)), dim3(256), 0, s, d.n, A, d.lda, d.J,
+                       d.sort.k0, d.st, d.nn);
+  }
+  if (d.npl > 0 && d.nmi > 0) {
+    dim3 g((d.npl + HT - 1) / HT, (d.nmi + HT - 1) / HT);
+    if (d.fused)
+      hipLaunchKernelGGL(k_hyp_weights<true>, g, dim3(256), 0, s, d.n, A, d.lda, d.J, d.plus, d.npl, d.minus, d.nmi,
+                         d.sort.k0, d.st);
+    else
+      hipLaunchKernelGGL(k_hyp_weights<false>, g, dim3(256), 0, s, d.n, A, d.lda, d.J, d.plus, d.npl, d.minus,
+                         d.nmi, d.sort.k0, d.st);
+  }
+}
+
+constexpr int SEL_ROUNDS = 4;
+
+void select_launch(DeviceStep &d, hipStream_t s) {
+  const int ntiles = (d.nn + FILT_TILE - 1) / FILT_TILE;
+  const size_t lds = (size_t)d.n;
+  for (int r = 0; r < SEL_ROUNDS; ++r) {
+    const int limit = (r == SEL_ROUNDS - 1) ? 0 : d.select_limit;
+    hipLaunchKernelGGL(k_sel_resolve, dim3(1), dim3(SEL_THREADS), lds, s, d.n, d.sort.k0, d.sort.k1, d.sort.v0,
+                       d.sort.v1, d.sort_skip, d.lists, d.used, d.piv_p, d.piv_q, d.piv_pos, d.piv_w, d.st, r,
+                       limit, d.nn);
+    if (r == SEL_ROUNDS - 1) break;
+    hipLaunchKernelGGL(k_sel_filter_count, dim3(ntiles), dim3(256), lds, s, d.n, d.sort.k0, d.sort.k1, d.sort.v0,
+                       d.sort.v1, d.sort_skip, d.lists, d.used, d.fcount, d.st, r);
+    hipLaunchKernelGGL(k_sel_filter_scan, dim3(1), dim3(1024), 0, s, d.fcount, ntiles, d.st, r);
+    hipLaunchKernelGGL(k_sel_filter_write, dim3(ntiles), dim3(256), lds, s, d.n, d.sort.k0, d.sort.k1, d.sort.v0,
+                       d.sort.v1, d.sort_skip, d.lists, d.used, d.fcount, d.st, r);
+    hipLaunchKernelGGL(k_sel_filter_end, dim3(1), dim3(1), 0, s, d.fcount, ntiles, d.st, r);
+  }
+}
+
+void pivots_launch(DeviceStep &d, hipStream_t s) {
+  const int g = (d.n2max + 255) / 256;
+  if (d.fused)
+    hipLaunchKernelGGL(k_pivots<true>, dim3(g), dim3(256), 0, s, d.n, (const double *)d.A, d.lda, d.J, d.piv_p,
+                       d.piv_q, d.piv_info, (double *)d.piv_V, (double *)d.piv_W, d.piv_d, d.rot_row, d.rot_col,
+                       d.colpiv, d.st);
+  else
+    hipLaunchKernelGGL(k_pivots<false>, dim3(g), dim3(256), 0, s, d.n, (const double *)d.A, d.lda, d.J, d.piv_p,
+                       d.piv_q, d.piv_info, (double *)d.piv_V, (double *)d.piv_W, d.piv_d, d.rot_row, d.rot_col,
+                       d.colpiv, d.st);
+}
+
+void rotate_launch(DeviceStep &d, hipStream_t s) {
+  const size_t lds = (size_t)2 * d.n * sizeof(double);
+  if (d.fused)
+    hipLaunchKernelGGL(k_rotate<true>, dim3(d.n), dim3(512), lds, s, d.n, (double *)d.A, d.lda, (double *)d.U,
+                       (double *)d.Z, d.piv_p, d.piv_q, d.piv_info, d.piv_d, d.rot_row, d.rot_col, d.colpiv, d.st);
+  else
+    hipLaunchKernelGGL(k_rotate<false>, dim3(d.n), dim3(512), lds, s, d.n, (double *)d.A, d.lda, (double *)d.U,
+                       (double *)d.Z, d.piv_p, d.piv_q, d.piv_info, d.piv_d, d.rot_row, d.rot_col, d.colpiv, d.st);
+}
+
+void step_end_launch(DeviceStep &d, hipStream_t s) {
+  hipLaunchKernelGGL(k_step_end, dim3(1), dim3(256), 0, s, d.J, d.piv_p, d.piv_q, d.piv_info, d.piv_w, d.st,
+                     d.trace);
+}
+
+int step_prepare(DeviceStep &d) {
+  const size_t lds = (size_t)2 * d.n * sizeof(double);
+  if (lds > 160 * 1024) return -1;  // two columns must fit the LDS of one CU
+  if (hipFuncSetAttribute((const void *)k_rotate<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
+          hipSuccess ||
+      hipFuncSetAttribute((const void *)k_rotate<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
+          hipSuccess)
+    return -2;
+  return 0;
+}
+
+void step_launch(DeviceStep &d, hipStream_t s) {
+  step_begin_launch(d, s);
+  weights_launch(d, s);
+  sort_pairs_launch(d.sort, &d.st->nn_sort, d.nn, s);
+  select_launch(d, s);
+  pivots_launch(d, s);
+  rotate_launch(d, s);
+  step_end_launch(d, s);
+}
+
+void init_identity_launch(DeviceStep &d, hipStream_t s) {
+  hipLaunchKernelGGL(k_identity, dim3(grid_for((size_t)d.n * d.n, 256, 8192)), dim3(256), 0, s, (double *)d.U,
+                     (double *)d.Z, d.n, d.lda);
+}
+
+void finalize_launch(DeviceStep &d, hipStream_t s, double *sigma) {
+  hipLaunchKernelGGL(k_finalize, dim3(grid_for((size_t)d.n * d.n, 256, 8192)), dim3(256), 0, s, (double *)d.A,
+                     (double *)d.Z, d.n, d.lda, d.J, sigma);
+}
+
+}  // namespace jkg

@@ -1,0 +1,99 @@
+// jkg_step.hpp -- device buffers and launch interface of one dynamic-ordering
+// step (src/D/dstep.F90:138-443 and src/Z/zstep.F90:140-447, restated for
+// the GPU).  All per-step control values live in device memory (StepState),
+// so a step is a fixed sequence of launches that a HIP graph can capture and
+// that never needs the host between steps.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "jkg_sort.hpp"
+
+namespace jkg {
+
+// Candidate lists of the selection rounds (ping-pong), passed by value.
+struct ListPtrs {
+  uint64_t *key[2];
+  uint32_t *val[2];
+  uint32_t *pos[2];
+};
+
+// One CSV line of the reference's per-step trace (src/D/dstep.F90:410-440).
+struct StepRec {
+  int32_t step, oldlen, newlen, trig, hyp, it;
+  double tw;
+};
+
+// Column-rotation / row-rotation descriptor of a 2x2 factor B in the ratio
+// form of BA/AB (src/D/dtransf.F90:121-242):
+//   xx = FMAD(sa ? y : x, r1, sa ? x : y) * m1
+//   yy = FMAD(sb ? x : y, r2, sb ? y : x) * m2
+struct Rot {
+  double r1, r2, m1, m2;
+  int sa, sb, pad0, pad1;
+};
+
+struct StepState {
+  int step;       // index S of the step being executed
+  int active;     // this step runs (set at step begin: !done)
+  int done;       // the loop has ended
+  int info;       // final INFO of the loop (steps, or < 0)
+  int max_steps;  // < 0: unlimited
+  int n2;         // N_2 after JSTEP_LEN
+  int nn_sort;    // number of keys to sort this step (NN, or 0 when inactive)
+  int it;         // finite weights this step (IT)
+  int sl;         // pivots chosen this step (SL)
+  int nl;         // effective transforms this step (NEWLEN)
+  int err;        // a pivot's 2x2 failed: min INFO (< 0) or 0
+  int nrow;       // pivots with a row transform (INFO & 2)
+  // selection bookkeeping
+  int sel_done;   // the greedy walk has finished for this step
+  int sel_round;  // round of the current list
+  int list_len;   // length of the current candidate list
+  int consumed;   // entries of the current list already walked
+  int trace_cap;  // capacity of the trace buffer
+  int nrec;       // trace records written (last step index + 1)
+  int ntrig, nhyp;
+  double tw;
+};
+
+struct DeviceStep {
+  int n, lda, cplx, fused;
+  int nn, npl, nmi;
+  int n2max;
+  // matrices (column-major, leading dimension lda): real or complex128
+  void *A, *U, *Z;
+  int *J;                 // int32 signs
+  int *plus, *minus;      // 0-based column indices with J = +1 / -1
+  SortBuffers sort;
+  int *sort_skip;
+  // selection lists (round >= 1): keys, pair indices, 1-based sorted positions
+  ListPtrs lists;
+  int *fcount;            // per-tile survivor counts of the filter
+  uint8_t *used;          // N bytes: index already in a chosen pivot
+  // pivots
+  int *piv_p, *piv_q, *piv_pos, *piv_info;
+  double *piv_w;          // weight of the pivot (for NORMF2D)
+  void *piv_V, *piv_W;    // 2x2 U- and Z-factors (4 real or 4 complex each)
+  double *piv_d;          // new diagonal (2 per pivot)
+  Rot *rot_row, *rot_col; // descriptors (real path)
+  int *colpiv;            // N: j for c = p_j, -(j+2) for c = q_j, -1 otherwise
+  StepState *st;
+  StepRec *trace;
+  int select_limit;       // entries walked per selection round before filtering
+};
+
+// sets kernel attributes (dynamic LDS) for the problem size; before capture
+int step_prepare(DeviceStep &d);
+void step_launch(DeviceStep &d, hipStream_t s);
+void step_begin_launch(DeviceStep &d, hipStream_t s);
+void finalize_launch(DeviceStep &d, hipStream_t s, double *sigma);
+void init_identity_launch(DeviceStep &d, hipStream_t s);
+
+// the pieces, exposed for the step-level C-ABI and its parity tests
+void weights_launch(DeviceStep &d, hipStream_t s);
+void select_launch(DeviceStep &d, hipStream_t s);
+void pivots_launch(DeviceStep &d, hipStream_t s);
+void rotate_launch(DeviceStep &d, hipStream_t s);
+void step_end_launch(DeviceStep &d, hipStream_t s);
+
+}  // namespace jkg

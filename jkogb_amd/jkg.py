@@ -1,0 +1,249 @@
+"""ctypes front-end of libjkg.so (the C ABI of include/jkg.h).
+
+The reference's interfaces on the hot path, in Python:
+  Context.dsolve      -- djk.exe FN N N_2 [ID_NCP [ID_TRU]] between reading
+                         FN.Y/FN.J and writing FN.YU/FN.ZZ/FN.SY
+                         (src/D/djk.F90:27-121)
+  Context.srt / .ncp  -- the DPROC strategy procedures ASRT / ANCP
+                         (src/D/dtypes.F90:5-30)
+  tru / jstep_len     -- ATRU (src/shared/jstep.F90:9-120), JSTEP_LEN (:124-133)
+  Context.dhsvd2, .dweights, .dbuild, .dstep
+                      -- DHSVD2, DMAGF2T/DMAGF2H, DSTEP_BUILD, DSTEP_EXEC
+There is no CPU fallback: a missing libjkg.so raises at import.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libjkg.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError("libjkg.so is not built (run __graft_entry__.build() or make -C jkogb_amd/csrc): %s"
+                      % LIB_PATH)
+_lib = C.CDLL(LIB_PATH)
+
+JKG_OK = 0
+JKG_EHIP = -1000
+JKG_ENOMEM = -1001
+JKG_ENOTSUP = -1002
+JKG_EJ = -1003
+
+AW_DTYPE = np.dtype([("w", "<f8"), ("p", "<i4"), ("q", "<i4"), ("b", "<i4"), ("i", "<i4")])
+STEPREC_DTYPE = np.dtype([("step", "<i4"), ("oldlen", "<i4"), ("newlen", "<i4"), ("trig", "<i4"),
+                          ("hyp", "<i4"), ("it", "<i4"), ("tw", "<f8")])
+
+
+class Opts(C.Structure):
+    _fields_ = [("id_ncp", C.c_int32), ("id_tru", C.c_int32), ("fused", C.c_int32), ("max_steps", C.c_int32),
+                ("steps_per_sync", C.c_int32), ("trace_cap", C.c_int32), ("trace", C.c_void_p)]
+
+
+class Result(C.Structure):
+    _fields_ = [("info", C.c_int32), ("tt", C.c_int32), ("npos", C.c_int32), ("nneg", C.c_int32),
+                ("n2", C.c_int32), ("steps", C.c_int32), ("seconds", C.c_double)]
+
+
+def _sig(name, res, args):
+    f = getattr(_lib, name)
+    f.restype = res
+    f.argtypes = args
+    return f
+
+
+_vp, _i, _d = C.c_void_p, C.c_int, C.c_double
+_create = _sig("jkg_create", _i, [C.POINTER(C.c_void_p), _i])
+_destroy = _sig("jkg_destroy", None, [_vp])
+_strerror = _sig("jkg_strerror", C.c_char_p, [_i])
+_last_hip = _sig("jkg_last_hip_error", _i, [])
+_dsolve = _sig("jkg_dsolve", _i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, C.POINTER(Opts), C.POINTER(Result)])
+_zsolve = _sig("jkg_zsolve", _i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, C.POINTER(Opts), C.POINTER(Result)])
+_srt = _sig("jkg_srt", _i, [_vp, _i, _i, _vp, C.POINTER(C.c_int64)])
+_ncp = _sig("jkg_ncp", _i, [_vp, _i, _i, _i, _vp, _i, C.POINTER(C.c_int32), _vp, C.POINTER(C.c_int64)])
+_tru = _sig("jkg_tru", _i, [_i, _i, _vp, _i, _vp, _vp, _vp])
+_jstep_len = _sig("jkg_jstep_len", _i, [_i, _i])
+_dhsvd2 = _sig("jkg_dhsvd2", _i, [_vp, _i, _vp, _vp, _d, _i, _vp, _vp, _vp, _vp])
+_zhsvd2 = _sig("jkg_zhsvd2", _i, [_vp, _i, _vp, _vp, _d, _i, _vp, _vp, _vp, _vp])
+_dweights = _sig("jkg_dweights", _i, [_vp, _i, _vp, _vp, _i, _vp])
+_zweights = _sig("jkg_zweights", _i, [_vp, _i, _vp, _vp, _i, _vp])
+_dbuild = _sig("jkg_dbuild", _i, [_vp, _i, _i, _vp, _vp, _i, C.POINTER(C.c_int32), _vp, C.POINTER(C.c_int32), _vp])
+_dstep = _sig("jkg_dstep", _i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _i, _vp])
+_pair_index = _sig("jkg_pair_index", C.c_uint32, [_i, _i, _i])
+_key_weight = _sig("jkg_key_weight", _d, [C.c_uint64])
+
+EXPORTS = ["jkg_create", "jkg_destroy", "jkg_strerror", "jkg_last_hip_error", "jkg_opts_default", "jkg_dsolve",
+           "jkg_zsolve", "jkg_srt", "jkg_ncp", "jkg_tru", "jkg_jstep_len", "jkg_dhsvd2", "jkg_zhsvd2",
+           "jkg_dweights", "jkg_zweights", "jkg_dbuild", "jkg_dstep", "jkg_pair_index", "jkg_key_weight"]
+
+
+class JKGError(RuntimeError):
+    def __init__(self, code, where):
+        msg = "%s: %s (%d)" % (where, _strerror(code).decode(), code)
+        if code == JKG_EHIP:
+            msg += " [hipError %d]" % _last_hip()
+        super().__init__(msg)
+        self.code = code
+
+
+def _chk(rc, where):
+    if rc != JKG_OK:
+        raise JKGError(rc, where)
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def jstep_len(n, n2):
+    return _jstep_len(n, n2)
+
+
+def tru(J, id_tru=1):
+    n = len(J)
+    nn = n * (n - 1) // 2
+    J32 = np.ascontiguousarray(J, dtype=np.int32)
+    P = np.zeros(max(nn, 1), np.int32)
+    Q = np.zeros(max(nn, 1), np.int32)
+    PN = np.zeros(2, np.int32)
+    tt = _tru(id_tru, n, _p(J32), nn, _p(P), _p(Q), _p(PN))
+    return tt, P[:nn], Q[:nn], PN
+
+
+def pair_index(n, p, q):
+    return _pair_index(n, p, q)
+
+
+def key_weight(key):
+    return _key_weight(int(key))
+
+
+def keys_to_weights(keys):
+    """Vectorised jkg_key_weight: the weights the sort keys stand for."""
+    keys = np.asarray(keys, dtype=np.uint64)
+    ordv = ~keys
+    neg = (ordv & np.uint64(1 << 63)) == 0
+    u = np.where(neg, ~ordv, ordv & np.uint64((1 << 63) - 1))
+    w = u.view(np.float64).copy()
+    w[keys == np.uint64(0xFFFFFFFFFFFFFFFF)] = np.nan
+    return w
+
+
+class Context:
+    """One HIP stream and the device buffers of the last problem size."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        _chk(_create(C.byref(h), device), "jkg_create")
+        self._h = h
+
+    def close(self):
+        if self._h:
+            _destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---------------------------------------------------------------- solve
+    def dsolve(self, Y, J, n2=0, id_ncp=0, id_tru=0, fused=False, max_steps=-1, trace_cap=None,
+               steps_per_sync=0):
+        """HSVD of a real Y (N x N): dict(YU, ZZ, SY, info, trace, tt, pn, n2, seconds)."""
+        cplx = np.iscomplexobj(Y)
+        dt = np.complex128 if cplx else np.float64
+        Y = np.asfortranarray(Y, dtype=dt)
+        n = Y.shape[0]
+        if Y.shape != (n, n):
+            raise ValueError("Y must be square")
+        J64 = np.ascontiguousarray(J, dtype=np.int64)
+        YU = np.zeros((n, n), dt, order="F")
+        ZZ = np.zeros((n, n), dt, order="F")
+        SY = np.zeros(n)
+        if trace_cap is None:
+            trace_cap = 64 * n + 64 if max_steps < 0 else max_steps + 1
+        trace = np.zeros(max(trace_cap, 1), STEPREC_DTYPE)
+        o = Opts(id_ncp, id_tru, 1 if fused else 0, max_steps, steps_per_sync, trace_cap, _p(trace))
+        r = Result()
+        f = _zsolve if cplx else _dsolve
+        _chk(f(self._h, n, n2, _p(Y), _p(J64), _p(YU), _p(ZZ), _p(SY), C.byref(o), C.byref(r)),
+             "jkg_zsolve" if cplx else "jkg_dsolve")
+        return dict(YU=YU, ZZ=ZZ, SY=SY, info=r.info, trace=trace[:min(r.steps, trace_cap)], tt=r.tt,
+                    pn=(r.npos, r.nneg), n2=r.n2, seconds=r.seconds, steps=r.steps)
+
+    # ---------------------------------------------------------------- plug-ins
+    def srt(self, dz, nn=None):
+        """ASRT: sorts dz[:nn] (AW records) in place by AW_CMP."""
+        nn = len(dz) if nn is None else nn
+        info = C.c_int64(0)
+        _chk(_srt(self._h, nn, len(dz), _p(dz), C.byref(info)), "jkg_srt")
+        return info.value
+
+    def ncp(self, n, dz, n2, nn=None):
+        """ANCP: greedy disjoint pivots of the sorted dz[:nn]; returns 1-based positions."""
+        nn = len(dz) if nn is None else nn
+        sl = C.c_int32(0)
+        step = np.zeros(max(n2, 1), np.int32)
+        info = C.c_int64(0)
+        _chk(_ncp(self._h, n, nn, len(dz), _p(dz), n2, C.byref(sl), _p(step), C.byref(info)), "jkg_ncp")
+        return step[:sl.value].copy()
+
+    # ---------------------------------------------------------------- step level
+    def dhsvd2(self, A, J, th1=0.75, fused=False):
+        """DHSVD2 of a batch: A (count, 4) column-major 2x2 blocks, J (count, 2).
+        Returns U (count,4), Z (count,4), D (count,2), INFO (count)."""
+        cplx = np.iscomplexobj(A)
+        dt = np.complex128 if cplx else np.float64
+        A = np.ascontiguousarray(A, dtype=dt).reshape(-1, 4)
+        J = np.ascontiguousarray(J, dtype=np.int32).reshape(-1, 2)
+        cnt = A.shape[0]
+        U = np.zeros((cnt, 4), dt)
+        Z = np.zeros((cnt, 4), dt)
+        D = np.zeros((cnt, 2))
+        info = np.zeros(cnt, np.int32)
+        f = _zhsvd2 if cplx else _dhsvd2
+        _chk(f(self._h, cnt, _p(A), _p(J), th1, 1 if fused else 0, _p(U), _p(Z), _p(D), _p(info)), "jkg_dhsvd2")
+        return U, Z, D, info
+
+    def dweights(self, A, J, fused=False):
+        """Sort keys of every pair weight, indexed by pair_index."""
+        cplx = np.iscomplexobj(A)
+        A = np.asfortranarray(A, dtype=np.complex128 if cplx else np.float64)
+        n = A.shape[0]
+        keys = np.zeros(n * (n - 1) // 2, np.uint64)
+        f = _zweights if cplx else _dweights
+        _chk(f(self._h, n, _p(A), _p(np.ascontiguousarray(J, dtype=np.int64)), 1 if fused else 0, _p(keys)),
+             "jkg_dweights")
+        return keys
+
+    def dbuild(self, A, J, n2, fused=False):
+        """DSTEP_BUILD: (sorted finite candidates as AW records, chosen 1-based positions)."""
+        A = np.asfortranarray(A, dtype=np.float64)
+        n = A.shape[0]
+        nn = n * (n - 1) // 2
+        srt = np.zeros(nn, AW_DTYPE)
+        step = np.zeros(max(n // 2, 1), np.int32)
+        it = C.c_int32(0)
+        sl = C.c_int32(0)
+        _chk(_dbuild(self._h, n, n2, _p(A), _p(np.ascontiguousarray(J, dtype=np.int64)), 1 if fused else 0,
+                     C.byref(it), _p(srt), C.byref(sl), _p(step)), "jkg_dbuild")
+        return srt[:it.value].copy(), step[:sl.value].copy()
+
+    def dstep(self, A, J, U, Z, n2, fused=False):
+        """One DSTEP_EXEC on copies of (A, U, Z); returns (A, U, Z, trace record)."""
+        A = np.array(A, dtype=np.float64, order="F", copy=True)
+        U = np.array(U, dtype=np.float64, order="F", copy=True)
+        Z = np.array(Z, dtype=np.float64, order="F", copy=True)
+        n = A.shape[0]
+        rec = np.zeros(1, STEPREC_DTYPE)
+        _chk(_dstep(self._h, n, n2, _p(A), _p(np.ascontiguousarray(J, dtype=np.int64)), _p(U), _p(Z),
+                    1 if fused else 0, _p(rec)), "jkg_dstep")
+        return A, U, Z, rec[0]

@@ -161,3 +161,30 @@ def solve(Y, J, n2=0, id_ncp=0, id_tru=1, fused=False, max_steps=-1, max_trace=N
     nsteps = info + 1 if info >= 0 else None
     return dict(YU=U, ZZ=Z, SY=S[:n].copy(), info=info,
                 trace=trace[:min(nsteps if nsteps else max_trace, max_trace)], tt=tt, pn=PN, A=A)
+
+
+def pair_weights(A, J, fused=False):
+    """Weight of every pair (DMAGF2T for J_p == J_q, DMAGF2H otherwise, with
+    the hyperbolic -inf dropped to NaN as DSTEP_BUILD does), indexed by the
+    tie-break order of AW_CMP: k = m(m-1)/2 + (m - p), m = N - (q - p)."""
+    cplx = np.iscomplexobj(A)
+    A = np.asfortranarray(A, dtype=np.complex128 if cplx else np.float64)
+    n = A.shape[0]
+    J32 = np.ascontiguousarray(J, dtype=np.int32)
+    ft = getattr(lib(), pfx(fused) + ("zmagf2t" if cplx else "dmagf2t"))
+    fh = getattr(lib(), pfx(fused) + ("zmagf2h" if cplx else "dmagf2h"))
+    for f in (ft, fh):
+        f.restype = C.c_double
+        f.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p]
+    w = np.zeros(n * (n - 1) // 2)
+    pa, ja = _p(A), _p(J32)
+    for m in range(1, n):
+        for p in range(m, 0, -1):
+            q = p + n - m
+            k = m * (m - 1) // 2 + (m - p)
+            if J32[p - 1] == J32[q - 1]:
+                w[k] = ft(n, p, q, pa, n, ja)
+            else:
+                v = fh(n, p, q, pa, n, ja)
+                w[k] = np.nan if v < -np.finfo(np.float64).max else v
+    return w
