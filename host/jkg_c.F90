@@ -20,6 +20,7 @@ MODULE jkg_c
   TYPE, BIND(C) :: jkg_result
      INTEGER(c_int32_t) :: info, tt, npos, nneg, n2, steps
      REAL(c_double) :: seconds
+     INTEGER(c_int64_t) :: sum_sl, sum_it, sum_b2, sum_b4, sum_abod
   END TYPE jkg_result
 
   INTERFACE

@@ -62,6 +62,10 @@ typedef struct {
   int32_t n2;     /* N_2 after JSTEP_LEN */
   int32_t steps;  /* trace records written (steps executed, including the last) */
   double seconds; /* device time of the step loop (HIP events) */
+  /* work done, summed over the steps: chosen pivots, finite weights, pivots
+   * with a row transform (INFO&2), with a column transform (INFO&4), and
+   * hyperbolic pairs whose weight scanned their two columns (ABODNDF2) */
+  int64_t sum_sl, sum_it, sum_b2, sum_b4, sum_abod;
 } jkg_result;
 
 void jkg_opts_default(jkg_opts *o);
@@ -126,6 +130,12 @@ int jkg_dbuild(jkg_ctx *ctx, int n, int n2, const double *a, const int64_t *j, i
                jkg_aw *sorted, int32_t *sl, int32_t *step);
 int jkg_dstep(jkg_ctx *ctx, int n, int n2, double *a, const int64_t *j, double *u, double *z, int fused,
               jkg_steprec *rec);
+/* jkg_dprofile: `steps` steps of the real solver from (y, j), each stage
+ * launched on its own and timed with HIP events on the context's stream;
+ * stage_ms[0..6] = begin, weights, sort, select, pivots, rotate, end
+ * (milliseconds summed over the steps); res as for jkg_dsolve. */
+int jkg_dprofile(jkg_ctx *ctx, int n, int n2, const double *y, const int64_t *j, int fused, int steps,
+                 double *stage_ms, jkg_result *res);
 /* index of pair (p,q), 1 <= p < q <= n, in the tie-break order of AW_CMP */
 uint32_t jkg_pair_index(int n, int p, int q);
 /* the weight a sort key stands for (NaN for the key of a dropped pair) */

@@ -86,6 +86,7 @@ static int dalloc(jkg_ctx *c, T **p, size_t count) {
 // (Re)allocates the buffers of an N x N problem (real or complex).
 static int ensure_buffers(jkg_ctx *c, int n, int cplx, int trace_cap) {
   if (trace_cap < 1) trace_cap = 1;
+  if (n > 65535) return JKG_ENOTSUP;  // pairs are packed as two 16-bit indices
   if (c->n == n && c->cplx == cplx && c->trace_cap >= trace_cap) return JKG_OK;
   free_all(c);
   DeviceStep &d = c->d;
@@ -130,7 +131,7 @@ static int ensure_buffers(jkg_ctx *c, int n, int cplx, int trace_cap) {
   ACHK(dalloc(c, &d.st, 1));
   ACHK(dalloc(c, &d.trace, trace_cap));
   ACHK(dalloc(c, &c->sigma, n));
-  d.select_limit = std::max(4096, 4 * n);
+  d.select_limit = 65536;  // walked before the first filter of the list
   if (step_prepare(d) != 0) {
     free_all(c);
     return JKG_ENOTSUP;
@@ -335,6 +336,11 @@ static int run_loop(jkg_ctx *c, const jkg_opts *o, jkg_result *res) {
   res->info = s.info;
   res->steps = s.nrec;
   res->seconds = ms * 1e-3;
+  res->sum_sl = (int64_t)s.acc_sl;
+  res->sum_it = (int64_t)s.acc_it;
+  res->sum_b2 = (int64_t)s.acc_b2;
+  res->sum_b4 = (int64_t)s.acc_b4;
+  res->sum_abod = (int64_t)s.acc_abod;
   (void)launched;
   return JKG_OK;
 }
@@ -510,15 +516,6 @@ extern "C" int jkg_zweights(jkg_ctx *c, int n, const double *a, const int64_t *j
   return JKG_ENOTSUP;
 }
 
-static void decode_pair(int n, uint32_t k, int &p, int &q) {
-  // host twin of pair_of(): largest m with m(m-1)/2 <= k
-  uint32_t m = (uint32_t)((1.0 + sqrt(1.0 + 8.0 * (double)k)) * 0.5);
-  while ((uint64_t)m * (m - 1) / 2 > k) --m;
-  while ((uint64_t)(m + 1) * m / 2 <= k) ++m;
-  uint32_t r = k - m * (m - 1) / 2;
-  p = (int)(m - r);
-  q = p + n - (int)m;
-}
 
 extern "C" int jkg_dbuild(jkg_ctx *c, int n, int n2, const double *a, const int64_t *j, int fused, int32_t *it,
                           jkg_aw *sorted, int32_t *sl, int32_t *step) {
@@ -528,7 +525,7 @@ extern "C" int jkg_dbuild(jkg_ctx *c, int n, int n2, const double *a, const int6
   DeviceStep &d = c->d;
   step_begin_launch(d, c->stream);
   weights_launch(d, c->stream);
-  sort_pairs_launch(d.sort, &d.st->nn_sort, d.nn, c->stream);
+  sort_pairs_launch(d.sort, &d.st->nn_sort, d.nn, c->stream, 1);
   select_launch(d, c->stream);
   HIPCHK(hipGetLastError());
   StepState s;
@@ -544,18 +541,15 @@ extern "C" int jkg_dbuild(jkg_ctx *c, int n, int n2, const double *a, const int6
   std::vector<uint32_t> vals(s.it);
   if (s.it > 0) {
     HIPCHK(hipMemcpy(keys.data(), kb, sizeof(uint64_t) * s.it, hipMemcpyDeviceToHost));
-    if (moved) HIPCHK(hipMemcpy(vals.data(), vb, sizeof(uint32_t) * s.it, hipMemcpyDeviceToHost));
-    else
-      for (int i = 0; i < s.it; ++i) vals[i] = (uint32_t)i;
+    HIPCHK(hipMemcpy(vals.data(), vb, sizeof(uint32_t) * s.it, hipMemcpyDeviceToHost));
   }
   for (int i = 0; i < s.it; ++i) {
-    int p, q;
-    decode_pair(n, vals[i], p, q);
+    const int p = pq_p(vals[i]) + 1, q = pq_q(vals[i]) + 1;
     sorted[i].w = jkg_key_weight(keys[i]);
     sorted[i].p = p;
     sorted[i].q = q;
     sorted[i].b = q - p;
-    sorted[i].i = (int32_t)vals[i];
+    sorted[i].i = (int32_t)pair_index(n, p, q);
   }
   *it = s.it;
   *sl = s.sl;
@@ -700,7 +694,7 @@ extern "C" int jkg_ncp(jkg_ctx *c, int n, int nn, int nm, jkg_aw *dz, int n2, in
     if (w == 0.0) u = 0;
     uint64_t ord = (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
     key[i] = ~ord;
-    val[i] = pair_index(nb, dz[i].p, dz[i].q);
+    val[i] = pack_pq(dz[i].p - 1, dz[i].q - 1);
     pos[i] = (uint32_t)(i + 1);
   }
   HIPCHK(hipMemcpyAsync(d.lists.key[1], key.data(), 8 * (size_t)nn, hipMemcpyHostToDevice, c->stream));
@@ -724,5 +718,68 @@ extern "C" int jkg_ncp(jkg_ctx *c, int n, int nn, int nm, jkg_aw *dz, int n2, in
   HIPCHK(hipStreamSynchronize(c->stream));
   *sl = s.sl;
   if (s.sl > 0) HIPCHK(hipMemcpy(step, d.piv_pos, sizeof(int) * s.sl, hipMemcpyDeviceToHost));
+  return JKG_OK;
+}
+
+// ---------------------------------------------------------------- profiling
+extern "C" int jkg_dprofile(jkg_ctx *c, int n, int n2, const double *y, const int64_t *j, int fused, int steps,
+                            double *stage_ms, jkg_result *res) {
+  if (!c) return -1;
+  if (n < 3) return -2;
+  if (steps < 1) return -7;
+  memset(res, 0, sizeof(*res));
+  HIPCHK(hipSetDevice(c->device));
+  n2 = jkg_jstep_len(n, n2);
+  int rc = ensure_buffers(c, n, 0, steps + 1);
+  if (rc) return rc;
+  int npos, nneg;
+  rc = upload_problem(c, n, n2, y, j, &npos, &nneg, steps, c->trace_cap);
+  if (rc) return rc;
+  DeviceStep &d = c->d;
+  d.fused = fused ? 1 : 0;
+  init_identity_launch(d, c->stream);
+  const int NST = 7;
+  hipEvent_t ev[NST + 1];
+  for (int k = 0; k <= NST; ++k) HIPCHK(hipEventCreate(&ev[k]));
+  for (int k = 0; k < NST; ++k) stage_ms[k] = 0.0;
+  for (int s = 0; s < steps; ++s) {
+    HIPCHK(hipEventRecord(ev[0], c->stream));
+    step_begin_launch(d, c->stream);
+    HIPCHK(hipEventRecord(ev[1], c->stream));
+    weights_launch(d, c->stream);
+    HIPCHK(hipEventRecord(ev[2], c->stream));
+    sort_pairs_launch(d.sort, &d.st->nn_sort, d.nn, c->stream, 1);
+    HIPCHK(hipEventRecord(ev[3], c->stream));
+    select_launch(d, c->stream);
+    HIPCHK(hipEventRecord(ev[4], c->stream));
+    pivots_launch(d, c->stream);
+    HIPCHK(hipEventRecord(ev[5], c->stream));
+    rotate_launch(d, c->stream);
+    HIPCHK(hipEventRecord(ev[6], c->stream));
+    step_end_launch(d, c->stream);
+    HIPCHK(hipEventRecord(ev[7], c->stream));
+    HIPCHK(hipEventSynchronize(ev[7]));
+    for (int k = 0; k < NST; ++k) {
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+      stage_ms[k] += ms;
+    }
+  }
+  for (int k = 0; k <= NST; ++k) (void)hipEventDestroy(ev[k]);
+  StepState st;
+  HIPCHK(hipMemcpy(&st, d.st, sizeof(st), hipMemcpyDeviceToHost));
+  res->info = st.done ? st.info : st.step;
+  res->steps = st.nrec;
+  res->n2 = n2;
+  res->npos = npos;
+  res->nneg = nneg;
+  res->sum_sl = (int64_t)st.acc_sl;
+  res->sum_it = (int64_t)st.acc_it;
+  res->sum_b2 = (int64_t)st.acc_b2;
+  res->sum_b4 = (int64_t)st.acc_b4;
+  res->sum_abod = (int64_t)st.acc_abod;
+  double tot = 0;
+  for (int k = 0; k < NST; ++k) tot += stage_ms[k];
+  res->seconds = tot * 1e-3;
   return JKG_OK;
 }

@@ -127,6 +127,11 @@ __device__ __forceinline__ void pair_of(int n, uint32_t k, int &p, int &q) {
   q = p + n - (int)m;
 }
 
+// A pair as the sort carries it: 0-based p in the low, q in the high 16 bits.
+__host__ __device__ __forceinline__ uint32_t pack_pq(int p0, int q0) { return (uint32_t)p0 | ((uint32_t)q0 << 16); }
+__host__ __device__ __forceinline__ int pq_p(uint32_t v) { return (int)(v & 0xFFFFu); }
+__host__ __device__ __forceinline__ int pq_q(uint32_t v) { return (int)(v >> 16); }
+
 // Complex FP64 value in the reference's COMPLEX(KIND=8) layout.
 struct dz {
   double re, im;

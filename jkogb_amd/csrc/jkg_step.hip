@@ -100,7 +100,7 @@ __global__ void k_step_begin(StepState *st, int nn, int n, int *colpiv) {
 template <bool F>
 __global__ void __launch_bounds__(256) k_trig_weights(int n, const double *__restrict__ A, int lda,
                                                        const int *__restrict__ J, uint64_t *__restrict__ keys,
-                                                       const StepState *st, int nn) {
+                                                       uint32_t *__restrict__ vals, const StepState *st, int nn) {
   if (!st->active) return;
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < (uint32_t)nn; k += gridDim.x * blockDim.x) {
     int p, q;
@@ -112,6 +112,7 @@ __global__ void __launch_bounds__(256) k_trig_weights(int n, const double *__res
     bool go = (aqp != 0.0) || (apq != 0.0) || sign_neg(app) || sign_neg(aqq) ||
               ((jp + jq == -2) ? (aqq < app) : (app < aqq));
     keys[k] = go ? weight_key(dasum2<F>(aqp, apq)) : KEY_NAN;
+    vals[k] = pack_pq(p - 1, q - 1);
   }
 }
 
@@ -124,7 +125,8 @@ template <bool F>
 __global__ void __launch_bounds__(256) k_hyp_weights(int n, const double *__restrict__ A, int lda,
                                                       const int *__restrict__ J, const int *__restrict__ plus,
                                                       int npl, const int *__restrict__ minus, int nmi,
-                                                      uint64_t *__restrict__ keys, const StepState *st) {
+                                                      uint64_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                                                      StepState *st) {
   if (!st->active) return;
   __shared__ double L[HR][2 * HT + 1];
   __shared__ int cols[2 * HT];
@@ -167,7 +169,12 @@ __global__ void __launch_bounds__(256) k_hyp_weights(int n, const double *__rest
       }
     }
   }
+  __shared__ int nneed;
+  if (threadIdx.x == 0) nneed = 0;
+  __syncthreads();
+  if (need) atomicAdd(&nneed, 1);
   if (__syncthreads_or(need)) {
+    if (threadIdx.x == 0) atomicAdd((unsigned long long *)&st->acc_abod, (unsigned long long)nneed);
     const int xc = xplus ? a : HT + b, yc = xplus ? HT + b : a;
     const int s1c = sel1 ? yc : xc, t1c = sel1 ? xc : yc;
     const bool eq = (sel1 == sel2);
@@ -194,43 +201,57 @@ __global__ void __launch_bounds__(256) k_hyp_weights(int n, const double *__rest
     }
     if (need) key = (d < -1.7976931348623157e308) ? KEY_NAN : weight_key(d);  // -inf removed
   }
-  if (valid) keys[pair_index(n, p0 + 1, q0 + 1)] = key;
+  if (valid) {
+    const uint32_t k = pair_index(n, p0 + 1, q0 + 1);
+    keys[k] = key;
+    vals[k] = pack_pq(p0, q0);
+  }
 }
 
 // ------------------------------------------------------------------ (c) select
-constexpr int SEL_THREADS = 256;
+constexpr int SEL_THREADS_RES = 512;
 constexpr int FILT_TILE = 2048;
 
-// One workgroup walks the current list in order: chunks of 256 entries are
-// tested against the LDS bitmap in parallel, the surviving candidates are
-// compacted (in order) and wave 0 accepts them one after another with
-// ballots.  The walk stops after `limit` entries (then the list is filtered),
-// at the end of the list, or when N_2 pivots are taken.
-__global__ void __launch_bounds__(SEL_THREADS) k_sel_resolve(int n, const uint64_t *k0, const uint64_t *k1,
-                                                             const uint32_t *v0, const uint32_t *v1,
-                                                             const int *sort_skip, ListPtrs lp,
-                                                             uint8_t *used_g, int *piv_p, int *piv_q, int *piv_pos,
-                                                             double *piv_w, StepState *st, int round, int limit,
-                                                             int nn) {
+// One workgroup of 1024 lanes walks the current list in order, 16384 entries
+// (16 consecutive per lane) at a time.  Each lane keeps the (p, q) of its 16
+// entries in registers with a 16-bit mask of the ones still free.  One
+// iteration accepts the first free entry of the chunk: a wave minimum of the
+// lanes' first free index, one LDS slot per wave, one barrier, then every lane
+// reads the 16 slots, knows the accepted pair and clears its colliding
+// entries -- no LDS traffic per entry.  The next chunk's indices are loaded
+// while the current one is resolved.  The walk stops after `limit` entries,
+// at the end of the list, or when N_2 pivots are taken (AW_NCP2 semantics).
+constexpr int RES_ITEMS = 16;
+constexpr int RES_CHUNK = SEL_THREADS_RES * RES_ITEMS;
+
+struct ResSlot {
+  int idx, p, q, pad;
+};
+
+extern "C" __device__ int __ockl_wfred_min_i32(int);  // wave-wide minimum (DPP)
+
+template <bool HAS_POS>
+__global__ void __launch_bounds__(SEL_THREADS_RES) k_sel_resolve(int n, const uint64_t *k0, const uint64_t *k1,
+                                                                 const uint32_t *v0, const uint32_t *v1,
+                                                                 const int *sort_skip, ListPtrs lp,
+                                                                 uint8_t *used_g, int *piv_p, int *piv_q,
+                                                                 int *piv_pos, StepState *st, int round, int limit,
+                                                                 int nn) {
   extern __shared__ uint8_t used[];  // n bytes
-  __shared__ int cand_n;
-  __shared__ int acc;
-  __shared__ int cp[SEL_THREADS], cq[SEL_THREADS], cpos[SEL_THREADS];
-  __shared__ uint64_t ckey[SEL_THREADS];
-  __shared__ int wcount[SEL_THREADS / 64];
+  __shared__ ResSlot slot[2][SEL_THREADS_RES / 64];
   __shared__ int itv;
   if (st->sel_done || st->sel_round != round) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nwaves = SEL_THREADS_RES / 64;
   SortResult sr = sort_result_dev(k0, k1, v0, v1, sort_skip);
-  const uint64_t *keys = (round == 0) ? sr.keys : lp.key[(round - 1) & 1];
   const uint32_t *vals = (round == 0) ? sr.vals : lp.val[(round - 1) & 1];
-  const uint32_t *poss = (round == 0) ? nullptr : lp.pos[(round - 1) & 1];
-  const bool iota = (round == 0) && sr.vals_iota;
+  const uint32_t *poss = HAS_POS ? lp.pos[(round - 1) & 1] : nullptr;
   int len;
   if (round == 0) {
-    // IT = number of finite weights = position of the first NaN key
     if (tid == 0) {
-      int lo = 0, hi = nn;  // first index with key == KEY_NAN
+      // IT = number of finite weights = position of the first NaN key
+      const uint64_t *keys = sr.keys;
+      int lo = 0, hi = nn;
       while (lo < hi) {
         int mid = (lo + hi) >> 1;
         if (keys[mid] == KEY_NAN) hi = mid;
@@ -240,87 +261,91 @@ __global__ void __launch_bounds__(SEL_THREADS) k_sel_resolve(int n, const uint64
       st->it = lo;
       st->list_len = lo;
     }
-    for (int i = tid; i < n; i += SEL_THREADS) used[i] = 0;
+    for (int i = tid; i < n; i += SEL_THREADS_RES) used[i] = 0;
     __syncthreads();
     len = itv;
   } else {
-    for (int i = tid; i < n; i += SEL_THREADS) used[i] = used_g[i];
+    for (int i = tid; i < n; i += SEL_THREADS_RES) used[i] = used_g[i];
+    __syncthreads();
     len = st->list_len;
   }
-  if (tid == 0) acc = st->sl;
-  __syncthreads();
   const int n2 = st->n2;
+  int acc = st->sl;
   int c = st->consumed;
   const int stop = (limit > 0 && len - c > limit) ? c + limit : len;
-  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  int it_par = 0;
+  uint32_t nxt[RES_ITEMS];
+  auto load_chunk = [&](int base, uint32_t *dst) {
+#pragma unroll
+    for (int i = 0; i < RES_ITEMS; ++i) {
+      const int e = base + tid * RES_ITEMS + i;
+      dst[i] = (e < stop) ? vals[e] : 0xFFFFFFFFu;
+    }
+  };
+  if (c < stop) load_chunk(c, nxt);
   while (c < stop && acc < n2) {
-    const int e = c + tid;
-    bool cand = false;
-    int p = 0, q = 0, pos = 0;
-    uint64_t key = 0;
-    if (e < stop) {
-      uint32_t kk = iota ? (uint32_t)e : vals[e];
-      pair_of(n, kk, p, q);
-      --p;
-      --q;
-      key = keys[e];
-      pos = poss ? (int)poss[e] : e + 1;
-      cand = !used[p] && !used[q];
+    int pp[RES_ITEMS], qq[RES_ITEMS];
+    uint32_t alive = 0;
+#pragma unroll
+    for (int i = 0; i < RES_ITEMS; ++i) {
+      int p = 0, q = 0;
+      if (nxt[i] != 0xFFFFFFFFu) {
+        p = pq_p(nxt[i]);
+        q = pq_q(nxt[i]);
+        if (!used[p] && !used[q]) alive |= 1u << i;
+      }
+      pp[i] = p;
+      qq[i] = q;
     }
-    // in-order compaction of the candidates of this chunk
-    uint64_t bal = __ballot(cand);
-    if (lane == 0) wcount[wave] = __popcll(bal);
-    __syncthreads();
-    int off = 0;
-    for (int w = 0; w < wave; ++w) off += wcount[w];
-    if (cand) {
-      int slot = off + __popcll(bal & lt);
-      cp[slot] = p;
-      cq[slot] = q;
-      cpos[slot] = pos;
-      ckey[slot] = key;
-    }
-    if (tid == 0) {
-      int t = 0;
-      for (int w = 0; w < SEL_THREADS / 64; ++w) t += wcount[w];
-      cand_n = t;
-    }
-    __syncthreads();
-    if (wave == 0) {
-      const int nc = cand_n;
-      int a = acc;
-      for (int g = 0; g < nc && a < n2; g += 64) {
-        const int s = g + lane;
-        int pp = -1, qq = -1;
-        bool ok = false;
-        if (s < nc) {
-          pp = cp[s];
-          qq = cq[s];
-          ok = !used[pp] && !used[qq];
-        }
-        uint64_t m = __ballot(ok);
-        while (m && a < n2) {
-          const int l0 = __ffsll((long long)m) - 1;
-          const int pf = __shfl(pp, l0), qf = __shfl(qq, l0);
-          if (lane == 0) {
-            used[pf] = 1;
-            used[qf] = 1;
-            piv_p[a] = pf;
-            piv_q[a] = qf;
-            piv_pos[a] = cpos[g + l0];
-            piv_w[a] = key_weight(ckey[g + l0]);
+    const int base = c;
+    c = (c + RES_CHUNK < stop) ? c + RES_CHUNK : stop;
+    if (c < stop) load_chunk(c, nxt);  // in flight while this chunk resolves
+    for (;;) {
+      int first = alive ? tid * RES_ITEMS + (__ffs(alive) - 1) : 0x7FFFFFFF;
+      const int wmin = __ockl_wfred_min_i32(first);
+      ResSlot *sl = slot[it_par];
+      if (wmin == 0x7FFFFFFF) {
+        if (lane == 0) sl[wave] = ResSlot{wmin, 0, 0, 0};
+      } else if (first == wmin) {
+        const int li = wmin - tid * RES_ITEMS;
+        int pw = 0, qw = 0;
+#pragma unroll
+        for (int i = 0; i < RES_ITEMS; ++i)
+          if (i == li) {
+            pw = pp[i];
+            qw = qq[i];
           }
-          ++a;
-          ok = ok && (lane > l0) && pp != pf && pp != qf && qq != pf && qq != qf;
-          m = __ballot(ok);
+        sl[wave] = ResSlot{wmin, pw, qw, 0};
+      }
+      __syncthreads();
+      int g = 0x7FFFFFFF, pg = 0, qg = 0;
+      for (int w = 0; w < nwaves; ++w) {
+        const ResSlot r = sl[w];
+        if (r.idx < g) {
+          g = r.idx;
+          pg = r.p;
+          qg = r.q;
         }
       }
-      if (lane == 0) acc = a;
+      it_par ^= 1;
+      if (g == 0x7FFFFFFF) break;
+      if (tid == 0) {
+        used[pg] = 1;
+        used[qg] = 1;
+        piv_p[acc] = pg;
+        piv_q[acc] = qg;
+        const int e = base + g;
+        piv_pos[acc] = HAS_POS ? (int)poss[e] : e + 1;
+      }
+      ++acc;
+      if (acc >= n2) break;
+#pragma unroll
+      for (int i = 0; i < RES_ITEMS; ++i)
+        if (pp[i] == pg || pp[i] == qg || qq[i] == pg || qq[i] == qg) alive &= ~(1u << i);
     }
-    __syncthreads();
-    c = (c + SEL_THREADS < stop) ? c + SEL_THREADS : stop;
+    __syncthreads();  // used[] of this chunk visible to the next one
   }
-  for (int i = tid; i < n; i += SEL_THREADS) used_g[i] = used[i];
+  for (int i = tid; i < n; i += SEL_THREADS_RES) used_g[i] = used[i];
   if (tid == 0) {
     st->sl = acc;
     st->consumed = c;
@@ -348,12 +373,10 @@ __global__ void __launch_bounds__(256) k_sel_filter_count(int n, const uint64_t 
   __syncthreads();
   SortResult sr = sort_result_dev(k0, k1, v0, v1, sort_skip);
   const uint32_t *vals = (round == 0) ? sr.vals : lp.val[(round - 1) & 1];
-  const bool iota = (round == 0) && sr.vals_iota;
   int cnt = 0;
   for (int e = base + threadIdx.x; e < base + FILT_TILE && e < len; e += blockDim.x) {
-    int p, q;
-    pair_of(n, iota ? (uint32_t)e : vals[e], p, q);
-    cnt += (!used[p - 1] && !used[q - 1]) ? 1 : 0;
+    const uint32_t v = vals[e];
+    cnt += (!used[pq_p(v)] && !used[pq_q(v)]) ? 1 : 0;
   }
   for (int o = 32; o > 0; o >>= 1) cnt += __shfl_down(cnt, o);
   if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = cnt;
@@ -408,7 +431,6 @@ __global__ void __launch_bounds__(256) k_sel_filter_write(int n, const uint64_t 
     const uint64_t *keys = (round == 0) ? sr.keys : lp.key[(round - 1) & 1];
     const uint32_t *vals = (round == 0) ? sr.vals : lp.val[(round - 1) & 1];
     const uint32_t *poss = (round == 0) ? nullptr : lp.pos[(round - 1) & 1];
-    const bool iota = (round == 0) && sr.vals_iota;
     uint64_t *okey = lp.key[round & 1];
     uint32_t *oval = lp.val[round & 1], *opos = lp.pos[round & 1];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -418,10 +440,8 @@ __global__ void __launch_bounds__(256) k_sel_filter_write(int n, const uint64_t 
       bool keep = false;
       uint32_t kk = 0;
       if (e < base + FILT_TILE && e < len) {
-        kk = iota ? (uint32_t)e : vals[e];
-        int p, q;
-        pair_of(n, kk, p, q);
-        keep = !used[p - 1] && !used[q - 1];
+        kk = vals[e];
+        keep = !used[pq_p(kk)] && !used[pq_q(kk)];
       }
       uint64_t bal = __ballot(keep);
       if (lane == 0) wc[wave] = __popcll(bal);
@@ -454,7 +474,10 @@ __global__ void k_sel_filter_end(const int *fcount, int ntiles, StepState *st, i
 template <bool F>
 __global__ void __launch_bounds__(256) k_pivots(int n, const double *__restrict__ A, int lda,
                                                  const int *__restrict__ J, const int *__restrict__ piv_p,
-                                                 const int *__restrict__ piv_q, int *piv_info, double *piv_V,
+                                                 const int *__restrict__ piv_q, const int *__restrict__ piv_pos,
+                                                 const uint64_t *k0, const uint64_t *k1, const uint32_t *v0,
+                                                 const uint32_t *v1, const int *sort_skip, double *piv_w,
+                                                 int *piv_info, double *piv_V,
                                                  double *piv_W, double *piv_d, Rot *rot_row, Rot *rot_col,
                                                  int *colpiv, StepState *st) {
   if (!st->active) return;
@@ -463,6 +486,11 @@ __global__ void __launch_bounds__(256) k_pivots(int n, const double *__restrict_
   if (i >= sl) return;
   const int p = piv_p[i], q = piv_q[i];
   const size_t P = p, Q = q;
+  {
+    // the pivot's weight (for NORMF2D) from its position in the sorted list
+    SortResult sr = sort_result_dev(k0, k1, v0, v1, sort_skip);
+    piv_w[i] = key_weight(sr.keys[piv_pos[i] - 1]);
+  }
   double b[4] = {A[P * lda + P], A[P * lda + Q], A[Q * lda + P], A[Q * lda + Q]};
   int k[2] = {J[p], J[q]};
   double v[4], w[4];
@@ -587,9 +615,13 @@ __global__ void __launch_bounds__(512) k_rotate(int n, double *__restrict__ A, i
 __global__ void __launch_bounds__(256) k_step_end(const int *__restrict__ J, const int *__restrict__ piv_p,
                                                    const int *__restrict__ piv_q, const int *__restrict__ piv_info,
                                                    const double *__restrict__ piv_w, StepState *st, StepRec *trace) {
-  __shared__ int cnt[3];
+  __shared__ int cnt[5];
   if (!st->active) return;
   const int sl = st->sl, s = st->step;
+  if (threadIdx.x == 0) {
+    st->acc_sl += (unsigned long long)sl;
+    st->acc_it += (unsigned long long)st->it;
+  }
   StepRec rec;
   rec.step = s;
   rec.oldlen = sl;
@@ -607,8 +639,18 @@ __global__ void __launch_bounds__(256) k_step_end(const int *__restrict__ J, con
     return;
   }
   const int err = st->err;
-  if (threadIdx.x < 3) cnt[threadIdx.x] = 0;
+  if (threadIdx.x < 5) cnt[threadIdx.x] = 0;
   __syncthreads();
+  {
+    int nb2 = 0, nb4 = 0;
+    for (int i = threadIdx.x; i < sl; i += blockDim.x) {
+      const int l = piv_info[i];
+      if (l >= 0 && (l & 2)) ++nb2;
+      if (l >= 0 && (l & 4)) ++nb4;
+    }
+    atomicAdd(&cnt[3], nb2);
+    atomicAdd(&cnt[4], nb4);
+  }
   if (err == 0) {
     int nl = 0, nt = 0, nh = 0;
     for (int i = threadIdx.x; i < sl; i += blockDim.x) {
@@ -638,6 +680,8 @@ __global__ void __launch_bounds__(256) k_step_end(const int *__restrict__ J, con
     rec.hyp = (err == 0) ? cnt[2] : 0;
     if (s < st->trace_cap) trace[s] = rec;
     st->nrec = s + 1;
+    st->acc_b2 += (unsigned long long)cnt[3];
+    st->acc_b4 += (unsigned long long)cnt[4];
     st->nl = nl;
     const int sl2 = (nl < sl) ? nl : sl;
     if (sl2 <= 0) {
@@ -697,32 +741,37 @@ void weights_launch(DeviceStep &d, hipStream_t s) {
   const double *A = (const double *)d.A;
   if (d.fused) {
     hipLaunchKernelGGL(k_trig_weights<true>, dim3(grid_for(d.nn, 256, 65536)), dim3(256), 0, s, d.n, A, d.lda, d.J,
-                       d.sort.k0, d.st, d.nn);
+                       d.sort.k0, d.sort.v0, d.st, d.nn);
   } else {
     hipLaunchKernelGGL(k_trig_weights<false>, dim3(grid_for(d.nn, 256, 65536)), dim3(256), 0, s, d.n, A, d.lda, d.J,
-                       d.sort.k0, d.st, d.nn);
+                       d.sort.k0, d.sort.v0, d.st, d.nn);
   }
   if (d.npl > 0 && d.nmi > 0) {
     dim3 g((d.npl + HT - 1) / HT, (d.nmi + HT - 1) / HT);
     if (d.fused)
       hipLaunchKernelGGL(k_hyp_weights<true>, g, dim3(256), 0, s, d.n, A, d.lda, d.J, d.plus, d.npl, d.minus, d.nmi,
-                         d.sort.k0, d.st);
+                         d.sort.k0, d.sort.v0, d.st);
     else
       hipLaunchKernelGGL(k_hyp_weights<false>, g, dim3(256), 0, s, d.n, A, d.lda, d.J, d.plus, d.npl, d.minus,
-                         d.nmi, d.sort.k0, d.st);
+                         d.nmi, d.sort.k0, d.sort.v0, d.st);
   }
 }
 
-constexpr int SEL_ROUNDS = 4;
+constexpr int SEL_ROUNDS = 2;
 
 void select_launch(DeviceStep &d, hipStream_t s) {
   const int ntiles = (d.nn + FILT_TILE - 1) / FILT_TILE;
   const size_t lds = (size_t)d.n;
   for (int r = 0; r < SEL_ROUNDS; ++r) {
     const int limit = (r == SEL_ROUNDS - 1) ? 0 : d.select_limit;
-    hipLaunchKernelGGL(k_sel_resolve, dim3(1), dim3(SEL_THREADS), lds, s, d.n, d.sort.k0, d.sort.k1, d.sort.v0,
-                       d.sort.v1, d.sort_skip, d.lists, d.used, d.piv_p, d.piv_q, d.piv_pos, d.piv_w, d.st, r,
-                       limit, d.nn);
+    if (r == 0)
+      hipLaunchKernelGGL(k_sel_resolve<false>, dim3(1), dim3(SEL_THREADS_RES), lds, s, d.n, d.sort.k0, d.sort.k1,
+                         d.sort.v0, d.sort.v1, d.sort_skip, d.lists, d.used, d.piv_p, d.piv_q, d.piv_pos, d.st, r,
+                         limit, d.nn);
+    else
+      hipLaunchKernelGGL(k_sel_resolve<true>, dim3(1), dim3(SEL_THREADS_RES), lds, s, d.n, d.sort.k0, d.sort.k1,
+                         d.sort.v0, d.sort.v1, d.sort_skip, d.lists, d.used, d.piv_p, d.piv_q, d.piv_pos, d.st, r,
+                         limit, d.nn);
     if (r == SEL_ROUNDS - 1) break;
     hipLaunchKernelGGL(k_sel_filter_count, dim3(ntiles), dim3(256), lds, s, d.n, d.sort.k0, d.sort.k1, d.sort.v0,
                        d.sort.v1, d.sort_skip, d.lists, d.used, d.fcount, d.st, r);
@@ -737,11 +786,13 @@ void pivots_launch(DeviceStep &d, hipStream_t s) {
   const int g = (d.n2max + 255) / 256;
   if (d.fused)
     hipLaunchKernelGGL(k_pivots<true>, dim3(g), dim3(256), 0, s, d.n, (const double *)d.A, d.lda, d.J, d.piv_p,
-                       d.piv_q, d.piv_info, (double *)d.piv_V, (double *)d.piv_W, d.piv_d, d.rot_row, d.rot_col,
+                       d.piv_q, d.piv_pos, d.sort.k0, d.sort.k1, d.sort.v0, d.sort.v1, d.sort_skip, d.piv_w,
+                       d.piv_info, (double *)d.piv_V, (double *)d.piv_W, d.piv_d, d.rot_row, d.rot_col,
                        d.colpiv, d.st);
   else
     hipLaunchKernelGGL(k_pivots<false>, dim3(g), dim3(256), 0, s, d.n, (const double *)d.A, d.lda, d.J, d.piv_p,
-                       d.piv_q, d.piv_info, (double *)d.piv_V, (double *)d.piv_W, d.piv_d, d.rot_row, d.rot_col,
+                       d.piv_q, d.piv_pos, d.sort.k0, d.sort.k1, d.sort.v0, d.sort.v1, d.sort_skip, d.piv_w,
+                       d.piv_info, (double *)d.piv_V, (double *)d.piv_W, d.piv_d, d.rot_row, d.rot_col,
                        d.colpiv, d.st);
 }
 
@@ -774,7 +825,7 @@ int step_prepare(DeviceStep &d) {
 void step_launch(DeviceStep &d, hipStream_t s) {
   step_begin_launch(d, s);
   weights_launch(d, s);
-  sort_pairs_launch(d.sort, &d.st->nn_sort, d.nn, s);
+  sort_pairs_launch(d.sort, &d.st->nn_sort, d.nn, s, 1);
   select_launch(d, s);
   pivots_launch(d, s);
   rotate_launch(d, s);

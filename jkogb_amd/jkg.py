@@ -42,7 +42,15 @@ class Opts(C.Structure):
 
 class Result(C.Structure):
     _fields_ = [("info", C.c_int32), ("tt", C.c_int32), ("npos", C.c_int32), ("nneg", C.c_int32),
-                ("n2", C.c_int32), ("steps", C.c_int32), ("seconds", C.c_double)]
+                ("n2", C.c_int32), ("steps", C.c_int32), ("seconds", C.c_double), ("sum_sl", C.c_int64),
+                ("sum_it", C.c_int64), ("sum_b2", C.c_int64), ("sum_b4", C.c_int64), ("sum_abod", C.c_int64)]
+
+
+STAGES = ["begin", "weights", "sort", "select", "pivots", "rotate", "end"]
+
+
+def _counts(r):
+    return dict(sum_sl=r.sum_sl, sum_it=r.sum_it, sum_b2=r.sum_b2, sum_b4=r.sum_b4, sum_abod=r.sum_abod)
 
 
 def _sig(name, res, args):
@@ -69,12 +77,14 @@ _dweights = _sig("jkg_dweights", _i, [_vp, _i, _vp, _vp, _i, _vp])
 _zweights = _sig("jkg_zweights", _i, [_vp, _i, _vp, _vp, _i, _vp])
 _dbuild = _sig("jkg_dbuild", _i, [_vp, _i, _i, _vp, _vp, _i, C.POINTER(C.c_int32), _vp, C.POINTER(C.c_int32), _vp])
 _dstep = _sig("jkg_dstep", _i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _i, _vp])
+_dprofile = _sig("jkg_dprofile", _i, [_vp, _i, _i, _vp, _vp, _i, _i, _vp, C.POINTER(Result)])
 _pair_index = _sig("jkg_pair_index", C.c_uint32, [_i, _i, _i])
 _key_weight = _sig("jkg_key_weight", _d, [C.c_uint64])
 
 EXPORTS = ["jkg_create", "jkg_destroy", "jkg_strerror", "jkg_last_hip_error", "jkg_opts_default", "jkg_dsolve",
            "jkg_zsolve", "jkg_srt", "jkg_ncp", "jkg_tru", "jkg_jstep_len", "jkg_dhsvd2", "jkg_zhsvd2",
-           "jkg_dweights", "jkg_zweights", "jkg_dbuild", "jkg_dstep", "jkg_pair_index", "jkg_key_weight"]
+           "jkg_dweights", "jkg_zweights", "jkg_dbuild", "jkg_dstep", "jkg_dprofile", "jkg_pair_index",
+           "jkg_key_weight"]
 
 
 class JKGError(RuntimeError):
@@ -177,7 +187,18 @@ class Context:
         _chk(f(self._h, n, n2, _p(Y), _p(J64), _p(YU), _p(ZZ), _p(SY), C.byref(o), C.byref(r)),
              "jkg_zsolve" if cplx else "jkg_dsolve")
         return dict(YU=YU, ZZ=ZZ, SY=SY, info=r.info, trace=trace[:min(r.steps, trace_cap)], tt=r.tt,
-                    pn=(r.npos, r.nneg), n2=r.n2, seconds=r.seconds, steps=r.steps)
+                    pn=(r.npos, r.nneg), n2=r.n2, seconds=r.seconds, steps=r.steps, **_counts(r))
+
+    def dprofile(self, Y, J, n2=0, steps=4, fused=False):
+        """Per-stage device time (ms summed over `steps` steps from Y) and work counts."""
+        Y = np.asfortranarray(Y, dtype=np.float64)
+        n = Y.shape[0]
+        ms = np.zeros(len(STAGES))
+        r = Result()
+        _chk(_dprofile(self._h, n, n2, _p(Y), _p(np.ascontiguousarray(J, dtype=np.int64)), 1 if fused else 0,
+                       steps, _p(ms), C.byref(r)), "jkg_dprofile")
+        return dict(stage_ms=dict(zip(STAGES, ms.tolist())), steps=r.steps, n2=r.n2, seconds=r.seconds,
+                    **_counts(r))
 
     # ---------------------------------------------------------------- plug-ins
     def srt(self, dz, nn=None):
