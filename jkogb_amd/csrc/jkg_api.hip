@@ -127,6 +127,7 @@ static int ensure_buffers(jkg_ctx *c, int n, int cplx, int trace_cap) {
   ACHK(dalloc(c, &d.piv_d, 2 * n2max));
   ACHK(dalloc(c, &d.rot_row, n2max));
   ACHK(dalloc(c, &d.rot_col, n2max));
+  if (cplx) ACHK(dalloc(c, (char **)&d.zrot, 3 * zrot_bytes() * n2max));
   ACHK(dalloc(c, &d.colpiv, n));
   ACHK(dalloc(c, &d.st, 1));
   ACHK(dalloc(c, &d.trace, trace_cap));
@@ -262,6 +263,7 @@ double jkg_key_weight(uint64_t key) {
 // ---------------------------------------------------------------- problem setup
 static int upload_problem(jkg_ctx *c, int n, int n2, const void *y, const int64_t *j, int *npos, int *nneg,
                           int max_steps, int trace_cap) {
+  // (the element size follows the buffers: c->d.cplx)
   DeviceStep &d = c->d;
   std::vector<int> J32(n), pl, mi;
   for (int i = 0; i < n; ++i) {
@@ -345,8 +347,8 @@ static int run_loop(jkg_ctx *c, const jkg_opts *o, jkg_result *res) {
   return JKG_OK;
 }
 
-extern "C" int jkg_dsolve(jkg_ctx *c, int n, int n2, const double *y, const int64_t *j, double *yu, double *zz,
-                          double *sy, const jkg_opts *opts, jkg_result *res) {
+static int solve_common(jkg_ctx *c, int cplx, int n, int n2, const void *y, const int64_t *j, void *yu, void *zz,
+                        double *sy, const jkg_opts *opts, jkg_result *res) {
   if (!c) return -1;
   if (n < 3) return -2;
   if (!y) return -4;
@@ -367,7 +369,7 @@ extern "C" int jkg_dsolve(jkg_ctx *c, int n, int n2, const double *y, const int6
   res->n2 = n2;
   if (n2 <= 0) return -3;
   int cap = std::max(o.trace_cap, 1);
-  int rc = ensure_buffers(c, n, 0, cap);
+  int rc = ensure_buffers(c, n, cplx, cap);
   if (rc) return rc;
   // TT: the number of trigonometric pairs (the reference's R%TRU result)
   int npos = 0, nneg = 0;
@@ -384,17 +386,22 @@ extern "C" int jkg_dsolve(jkg_ctx *c, int n, int n2, const double *y, const int6
     if (cnt > 0) HIPCHK(hipMemcpy(o.trace, d.trace, sizeof(jkg_steprec) * cnt, hipMemcpyDeviceToHost));
   }
   if (res->info >= 0) {
-    HIPCHK(hipMemcpy(yu, d.U, sizeof(double) * n * n, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(zz, d.Z, sizeof(double) * n * n, hipMemcpyDeviceToHost));
+    const size_t esz = cplx ? 16 : 8;
+    HIPCHK(hipMemcpy(yu, d.U, esz * n * n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(zz, d.Z, esz * n * n, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(sy, c->sigma, sizeof(double) * n, hipMemcpyDeviceToHost));
   }
   return JKG_OK;
 }
 
+extern "C" int jkg_dsolve(jkg_ctx *c, int n, int n2, const double *y, const int64_t *j, double *yu, double *zz,
+                          double *sy, const jkg_opts *opts, jkg_result *res) {
+  return solve_common(c, 0, n, n2, y, j, yu, zz, sy, opts, res);
+}
+
 extern "C" int jkg_zsolve(jkg_ctx *c, int n, int n2, const double *y, const int64_t *j, double *yu, double *zz,
                           double *sy, const jkg_opts *opts, jkg_result *res) {
-  (void)c; (void)n; (void)n2; (void)y; (void)j; (void)yu; (void)zz; (void)sy; (void)opts; (void)res;
-  return JKG_ENOTSUP;
+  return solve_common(c, 1, n, n2, y, j, yu, zz, sy, opts, res);
 }
 
 // ---------------------------------------------------------------- step level
@@ -489,10 +496,10 @@ extern "C" int jkg_zhsvd2(jkg_ctx *c, int count, const double *a, const int32_t 
 }
 
 // Prepares the device state of one step on (a, j) without running it.
-static int prepare_single(jkg_ctx *c, int n, int n2, const double *a, const int64_t *j, int fused) {
+static int prepare_single(jkg_ctx *c, int n, int n2, const void *a, const int64_t *j, int fused, int cplx = 0) {
   if (n < 3) return -2;
   HIPCHK(hipSetDevice(c->device));
-  int rc = ensure_buffers(c, n, 0, 1);
+  int rc = ensure_buffers(c, n, cplx, 1);
   if (rc) return rc;
   c->d.fused = fused;
   int npos, nneg;
@@ -512,8 +519,15 @@ extern "C" int jkg_dweights(jkg_ctx *c, int n, const double *a, const int64_t *j
 }
 
 extern "C" int jkg_zweights(jkg_ctx *c, int n, const double *a, const int64_t *j, int fused, uint64_t *keys) {
-  (void)c; (void)n; (void)a; (void)j; (void)fused; (void)keys;
-  return JKG_ENOTSUP;
+  if (!c) return -1;
+  int rc = prepare_single(c, n, 0, a, j, fused, 1);
+  if (rc) return rc;
+  step_begin_launch(c->d, c->stream);
+  weights_launch(c->d, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(keys, c->d.sort.k0, sizeof(uint64_t) * c->d.nn, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return JKG_OK;
 }
 
 

@@ -738,6 +738,10 @@ void step_begin_launch(DeviceStep &d, hipStream_t s) {
 }
 
 void weights_launch(DeviceStep &d, hipStream_t s) {
+  if (d.cplx) {
+    zweights_launch(d, s);
+    return;
+  }
   const double *A = (const double *)d.A;
   if (d.fused) {
     hipLaunchKernelGGL(k_trig_weights<true>, dim3(grid_for(d.nn, 256, 65536)), dim3(256), 0, s, d.n, A, d.lda, d.J,
@@ -783,6 +787,10 @@ void select_launch(DeviceStep &d, hipStream_t s) {
 }
 
 void pivots_launch(DeviceStep &d, hipStream_t s) {
+  if (d.cplx) {
+    zpivots_launch(d, s);
+    return;
+  }
   const int g = (d.n2max + 255) / 256;
   if (d.fused)
     hipLaunchKernelGGL(k_pivots<true>, dim3(g), dim3(256), 0, s, d.n, (const double *)d.A, d.lda, d.J, d.piv_p,
@@ -797,6 +805,10 @@ void pivots_launch(DeviceStep &d, hipStream_t s) {
 }
 
 void rotate_launch(DeviceStep &d, hipStream_t s) {
+  if (d.cplx) {
+    zrotate_launch(d, s);
+    return;
+  }
   const size_t lds = (size_t)2 * d.n * sizeof(double);
   if (d.fused)
     hipLaunchKernelGGL(k_rotate<true>, dim3(d.n), dim3(512), lds, s, d.n, (double *)d.A, d.lda, (double *)d.U,
@@ -812,6 +824,7 @@ void step_end_launch(DeviceStep &d, hipStream_t s) {
 }
 
 int step_prepare(DeviceStep &d) {
+  if (d.cplx) return zstep_prepare(d);
   const size_t lds = (size_t)2 * d.n * sizeof(double);
   if (lds > 160 * 1024) return -1;  // two columns must fit the LDS of one CU
   if (hipFuncSetAttribute((const void *)k_rotate<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
@@ -833,11 +846,19 @@ void step_launch(DeviceStep &d, hipStream_t s) {
 }
 
 void init_identity_launch(DeviceStep &d, hipStream_t s) {
+  if (d.cplx) {
+    zinit_identity_launch(d, s);
+    return;
+  }
   hipLaunchKernelGGL(k_identity, dim3(grid_for((size_t)d.n * d.n, 256, 8192)), dim3(256), 0, s, (double *)d.U,
                      (double *)d.Z, d.n, d.lda);
 }
 
 void finalize_launch(DeviceStep &d, hipStream_t s, double *sigma) {
+  if (d.cplx) {
+    zfinalize_launch(d, s, sigma);
+    return;
+  }
   hipLaunchKernelGGL(k_finalize, dim3(grid_for((size_t)d.n * d.n, 256, 8192)), dim3(256), 0, s, (double *)d.A,
                      (double *)d.Z, d.n, d.lda, d.J, sigma);
 }

@@ -78,6 +78,7 @@ struct DeviceStep {
   void *piv_V, *piv_W;    // 2x2 U- and Z-factors (4 real or 4 complex each)
   double *piv_d;          // new diagonal (2 per pivot)
   Rot *rot_row, *rot_col; // descriptors (real path)
+  void *zrot;             // descriptors (complex path): BA(V), AB(V^H), AB(W) per pivot
   int *colpiv;            // N: j for c = p_j, -(j+2) for c = q_j, -1 otherwise
   StepState *st;
   StepRec *trace;
@@ -92,7 +93,15 @@ void finalize_launch(DeviceStep &d, hipStream_t s, double *sigma);
 void init_identity_launch(DeviceStep &d, hipStream_t s);
 
 // the pieces, exposed for the step-level C-ABI and its parity tests
+// (weights/pivots/rotate/identity/finalize dispatch on d.cplx)
 void weights_launch(DeviceStep &d, hipStream_t s);
+size_t zrot_bytes();
+void zweights_launch(DeviceStep &d, hipStream_t s);
+void zpivots_launch(DeviceStep &d, hipStream_t s);
+void zrotate_launch(DeviceStep &d, hipStream_t s);
+int zstep_prepare(DeviceStep &d);
+void zinit_identity_launch(DeviceStep &d, hipStream_t s);
+void zfinalize_launch(DeviceStep &d, hipStream_t s, double *sigma);
 void select_launch(DeviceStep &d, hipStream_t s);
 void pivots_launch(DeviceStep &d, hipStream_t s);
 void rotate_launch(DeviceStep &d, hipStream_t s);

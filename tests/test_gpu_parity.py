@@ -260,3 +260,61 @@ def test_one_step_entry_point(ctx, orc):
     assert rec["oldlen"] == o["trace"]["oldlen"][0] and rec["newlen"] == o["trace"]["newlen"][0]
     assert same_bits(np.diag(a), o["SY"])
     assert same_bits(u, o["YU"])
+
+
+# --------------------------------------------------------------- complex (zjk)
+@pytest.mark.parametrize("fused", [False, True])
+def test_zhsvd2_batch_bit_exact(ctx, orc, fused):
+    rng = np.random.default_rng(5)
+    g = golden("kat_zhsvd2.npz")
+    blocks = [a for a in g["A"]]
+    for k in range(2000):
+        e = 10.0 ** rng.uniform(-20, 20)
+        a = (rng.standard_normal(4) + 1j * rng.standard_normal(4) * (k % 5 != 0)) * e
+        if k % 7 == 0:
+            a[rng.integers(0, 4)] = 0
+        blocks.append(a)
+    A = np.array(blocks, dtype=np.complex128)
+    J = np.array([(1, -1) if k % 3 else ((1, 1) if k % 2 else (-1, -1)) for k in range(len(A))], np.int32)
+    J[: len(g["J"])] = g["J"]
+    for th1 in (0.75, 1.0):
+        U, Z, D, info = ctx.dhsvd2(A, J, th1=th1, fused=fused)
+        for k in range(len(A)):
+            Ao, Uo, Zo, io = orc.zhsvd2(A[k].reshape(2, 2, order="F"), J[k], th1=th1, fused=fused)
+            assert info[k] == io, k
+            if io < 0:
+                continue
+            assert same_bits(U[k], Uo.reshape(4, order="F")), k
+            assert same_bits(Z[k], Zo.reshape(4, order="F")), k
+            assert same_bits(D[k], np.array([Ao[0, 0].real, Ao[1, 1].real])), k
+
+
+@pytest.mark.parametrize("n,seed", [(24, 1), (41, 2)])
+@pytest.mark.parametrize("fused", [False, True])
+def test_zweights_bit_exact(ctx, orc, n, seed, fused):
+    from jkogb_amd import jkg
+    y, j = inputs("c3", n, seed)
+    w_gpu = jkg.keys_to_weights(ctx.dweights(y, j, fused=fused))
+    w_orc = orc.pair_weights(y, j, fused=fused)
+    nan = np.isnan(w_orc)
+    assert np.array_equal(np.isnan(w_gpu), nan)
+    assert same_bits(w_gpu[~nan], np.where(w_orc == 0, 0.0, w_orc)[~nan])
+
+
+@pytest.mark.parametrize("name", ["c3_z16", "c3_z32"])
+def test_zsolve_bit_exact_vs_reference(ctx, name):
+    g = golden("solve_%s.npz" % name)
+    r = ctx.dsolve(g["Y"], g["J"], int(g["n2"]))
+    assert r["tt"] == int(g["TT"])
+    check_solve(r, g["SY"], g["YU"], g["ZZ"], g["trace"], int(g["steps"]))
+
+
+def test_zsolve_fused_and_larger_vs_oracle(ctx, orc):
+    y, j = inputs("c3", 40, 61)
+    r = ctx.dsolve(y, j, 20, fused=True)
+    o = orc.solve(y, j, 20, fused=True)
+    check_solve(r, o["SY"], o["YU"], o["ZZ"], steps=o["info"])
+    y, j = inputs("c3", 200, 62)
+    r = ctx.dsolve(y, j, 100, max_steps=5)
+    o = orc.solve(y, j, 100, max_steps=5)
+    check_solve(r, o["SY"], o["YU"], o["ZZ"], steps=5)
