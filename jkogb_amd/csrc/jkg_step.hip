@@ -210,25 +210,42 @@ __global__ void __launch_bounds__(256) k_hyp_weights(int n, const double *__rest
 
 // ------------------------------------------------------------------ (c) select
 constexpr int SEL_THREADS_RES = 512;
+constexpr int SEL_WAVES = SEL_THREADS_RES / 64;
 constexpr int FILT_TILE = 2048;
 
-// One workgroup of 1024 lanes walks the current list in order, 16384 entries
-// (16 consecutive per lane) at a time.  Each lane keeps the (p, q) of its 16
-// entries in registers with a 16-bit mask of the ones still free.  One
-// iteration accepts the first free entry of the chunk: a wave minimum of the
-// lanes' first free index, one LDS slot per wave, one barrier, then every lane
-// reads the 16 slots, knows the accepted pair and clears its colliding
-// entries -- no LDS traffic per entry.  The next chunk's indices are loaded
-// while the current one is resolved.  The walk stops after `limit` entries,
-// at the end of the list, or when N_2 pivots are taken (AW_NCP2 semantics).
-constexpr int RES_ITEMS = 16;
+// One workgroup walks the current list in order, RES_CHUNK entries at a time
+// (RES_ITEMS consecutive ones per lane).  At the start of a chunk each lane
+// tests its entries against used[] (n + 1 bytes; used[n] = 1 is the sentinel
+// that past-the-end entries, 0xFFFFFFFF, map to) and keeps a bit mask of the
+// free ones.  One iteration accepts the first free entry of the chunk:
+//   - a wave finds its first free entry with one ballot (lanes hold increasing
+//     positions), reads it with readlane, and writes (index, pair[, pos]) to an
+//     LDS slot;
+//   - one barrier; every lane reads the SEL_WAVES slots (one LDS round trip)
+//     and takes the first non-empty one;
+//   - lanes clear the entries that share an index with the accepted pair
+//     (packed 16-bit compare: a half of v ^ (x * 0x10001) is zero iff that
+//     half of v equals x); waves with nothing free skip this.
+// A chunk ends with an iteration that finds nothing; its barrier also
+// publishes the last used[] writes to the next chunk's test.  The barriers
+// fence LDS only, so the next chunk's loads stay in flight.  The walk stops
+// after `limit` entries, at the end of the list, or at N_2 pivots (AW_NCP2
+// semantics, src/shared/atypes.F90:191-230).
+constexpr int RES_ITEMS = 4;
 constexpr int RES_CHUNK = SEL_THREADS_RES * RES_ITEMS;
 
-struct ResSlot {
-  int idx, p, q, pad;
-};
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
 
-extern "C" __device__ int __ockl_wfred_min_i32(int);  // wave-wide minimum (DPP)
+// Nonzero iff pair v (p | q << 16) shares an index with the pair whose
+// indices, replicated to both halves, are P and Q.
+__device__ __forceinline__ uint32_t pair_hits(uint32_t v, uint32_t P, uint32_t Q) {
+  const uint32_t x = v ^ P, y = v ^ Q;
+  return (((x - 0x10001u) & ~x) | ((y - 0x10001u) & ~y)) & 0x80008000u;
+}
 
 template <bool HAS_POS>
 __global__ void __launch_bounds__(SEL_THREADS_RES) k_sel_resolve(int n, const uint64_t *k0, const uint64_t *k1,
@@ -237,12 +254,11 @@ __global__ void __launch_bounds__(SEL_THREADS_RES) k_sel_resolve(int n, const ui
                                                                  uint8_t *used_g, int *piv_p, int *piv_q,
                                                                  int *piv_pos, StepState *st, int round, int limit,
                                                                  int nn) {
-  extern __shared__ uint8_t used[];  // n bytes
-  __shared__ ResSlot slot[2][SEL_THREADS_RES / 64];
+  extern __shared__ uint8_t used[];  // n + 1 bytes
+  __shared__ __attribute__((aligned(16))) uint32_t slot[2][3 * SEL_WAVES];  // index | pair | pos
   __shared__ int itv;
   if (st->sel_done || st->sel_round != round) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int nwaves = SEL_THREADS_RES / 64;
   SortResult sr = sort_result_dev(k0, k1, v0, v1, sort_skip);
   const uint32_t *vals = (round == 0) ? sr.vals : lp.val[(round - 1) & 1];
   const uint32_t *poss = HAS_POS ? lp.pos[(round - 1) & 1] : nullptr;
@@ -261,11 +277,11 @@ __global__ void __launch_bounds__(SEL_THREADS_RES) k_sel_resolve(int n, const ui
       st->it = lo;
       st->list_len = lo;
     }
-    for (int i = tid; i < n; i += SEL_THREADS_RES) used[i] = 0;
+    for (int i = tid; i <= n; i += SEL_THREADS_RES) used[i] = (i == n);
     __syncthreads();
     len = itv;
   } else {
-    for (int i = tid; i < n; i += SEL_THREADS_RES) used[i] = used_g[i];
+    for (int i = tid; i <= n; i += SEL_THREADS_RES) used[i] = (i == n) ? 1 : used_g[i];
     __syncthreads();
     len = st->list_len;
   }
@@ -273,78 +289,84 @@ __global__ void __launch_bounds__(SEL_THREADS_RES) k_sel_resolve(int n, const ui
   int acc = st->sl;
   int c = st->consumed;
   const int stop = (limit > 0 && len - c > limit) ? c + limit : len;
-  int it_par = 0;
-  uint32_t nxt[RES_ITEMS];
-  auto load_chunk = [&](int base, uint32_t *dst) {
+  int par = 0;
+  uint32_t nv[RES_ITEMS], np[RES_ITEMS];
+  auto load_chunk = [&](int base) {
 #pragma unroll
     for (int i = 0; i < RES_ITEMS; ++i) {
       const int e = base + tid * RES_ITEMS + i;
-      dst[i] = (e < stop) ? vals[e] : 0xFFFFFFFFu;
+      nv[i] = (e < stop) ? vals[e] : 0xFFFFFFFFu;
+      if constexpr (HAS_POS) np[i] = (e < stop) ? poss[e] : 0u;
     }
   };
-  if (c < stop) load_chunk(c, nxt);
+  if (c < stop) load_chunk(c);
   while (c < stop && acc < n2) {
-    int pp[RES_ITEMS], qq[RES_ITEMS];
+    uint32_t vv[RES_ITEMS], pv[RES_ITEMS];
     uint32_t alive = 0;
 #pragma unroll
     for (int i = 0; i < RES_ITEMS; ++i) {
-      int p = 0, q = 0;
-      if (nxt[i] != 0xFFFFFFFFu) {
-        p = pq_p(nxt[i]);
-        q = pq_q(nxt[i]);
-        if (!used[p] && !used[q]) alive |= 1u << i;
-      }
-      pp[i] = p;
-      qq[i] = q;
+      vv[i] = nv[i];
+      if constexpr (HAS_POS) pv[i] = np[i];
+      const uint32_t p = min(vv[i] & 0xFFFFu, (uint32_t)n), q = min(vv[i] >> 16, (uint32_t)n);
+      alive |= ((used[p] | used[q]) == 0 ? 1u : 0u) << i;
     }
     const int base = c;
     c = (c + RES_CHUNK < stop) ? c + RES_CHUNK : stop;
-    if (c < stop) load_chunk(c, nxt);  // in flight while this chunk resolves
+    if (c < stop) load_chunk(c);  // in flight while this chunk resolves
     for (;;) {
-      int first = alive ? tid * RES_ITEMS + (__ffs(alive) - 1) : 0x7FFFFFFF;
-      const int wmin = __ockl_wfred_min_i32(first);
-      ResSlot *sl = slot[it_par];
-      if (wmin == 0x7FFFFFFF) {
-        if (lane == 0) sl[wave] = ResSlot{wmin, 0, 0, 0};
-      } else if (first == wmin) {
-        const int li = wmin - tid * RES_ITEMS;
-        int pw = 0, qw = 0;
+      const int fi = __ffs(alive) - 1;
+      uint32_t fv = vv[0], fp = HAS_POS ? pv[0] : 0u;
 #pragma unroll
-        for (int i = 0; i < RES_ITEMS; ++i)
-          if (i == li) {
-            pw = pp[i];
-            qw = qq[i];
-          }
-        sl[wave] = ResSlot{wmin, pw, qw, 0};
+      for (int i = 1; i < RES_ITEMS; ++i) {
+        fv = (fi == i) ? vv[i] : fv;
+        if constexpr (HAS_POS) fp = (fi == i) ? pv[i] : fp;
       }
-      __syncthreads();
-      int g = 0x7FFFFFFF, pg = 0, qg = 0;
-      for (int w = 0; w < nwaves; ++w) {
-        const ResSlot r = sl[w];
-        if (r.idx < g) {
-          g = r.idx;
-          pg = r.p;
-          qg = r.q;
-        }
+      const uint64_t bal = __ballot(alive != 0);
+      uint32_t widx = 0x7FFFFFFFu, wval = 0, wpos = 0;
+      if (bal != 0) {
+        const int L = __ffsll((unsigned long long)bal) - 1;
+        widx = (uint32_t)((wave * 64 + L) * RES_ITEMS + __builtin_amdgcn_readlane(fi, L));
+        wval = __builtin_amdgcn_readlane(fv, L);
+        if constexpr (HAS_POS) wpos = __builtin_amdgcn_readlane(fp, L);
       }
-      it_par ^= 1;
-      if (g == 0x7FFFFFFF) break;
+      if (lane == 0) {
+        slot[par][wave] = widx;
+        slot[par][SEL_WAVES + wave] = wval;
+        if constexpr (HAS_POS) slot[par][2 * SEL_WAVES + wave] = wpos;
+      }
+      lds_barrier();
+      uint32_t g = slot[par][SEL_WAVES - 1], gv = slot[par][2 * SEL_WAVES - 1];
+      uint32_t gp = HAS_POS ? slot[par][3 * SEL_WAVES - 1] : 0u;
+#pragma unroll
+      for (int w = SEL_WAVES - 2; w >= 0; --w) {  // first non-empty wave (positions grow with the wave)
+        const uint32_t si = slot[par][w];
+        const bool ok = si != 0x7FFFFFFFu;
+        g = ok ? si : g;
+        gv = ok ? slot[par][SEL_WAVES + w] : gv;
+        if constexpr (HAS_POS) gp = ok ? slot[par][2 * SEL_WAVES + w] : gp;
+      }
+      par ^= 1;
+      if (g == 0x7FFFFFFFu) break;
+      const uint32_t pg = pq_p(gv), qg = pq_q(gv);
       if (tid == 0) {
         used[pg] = 1;
         used[qg] = 1;
-        piv_p[acc] = pg;
-        piv_q[acc] = qg;
-        const int e = base + g;
-        piv_pos[acc] = HAS_POS ? (int)poss[e] : e + 1;
+      } else if (tid == 64) {
+        piv_p[acc] = (int)pg;
+        piv_q[acc] = (int)qg;
+        piv_pos[acc] = HAS_POS ? (int)gp : base + (int)g + 1;
       }
       ++acc;
       if (acc >= n2) break;
+      if (bal == 0) continue;
+      const uint32_t P = pg * 0x10001u, Q = qg * 0x10001u;
+      uint32_t kill = 0;
 #pragma unroll
-      for (int i = 0; i < RES_ITEMS; ++i)
-        if (pp[i] == pg || pp[i] == qg || qq[i] == pg || qq[i] == qg) alive &= ~(1u << i);
+      for (int i = 0; i < RES_ITEMS; ++i) kill |= (pair_hits(vv[i], P, Q) != 0 ? 1u : 0u) << i;
+      alive &= ~kill;
     }
-    __syncthreads();  // used[] of this chunk visible to the next one
   }
+  __syncthreads();
   for (int i = tid; i < n; i += SEL_THREADS_RES) used_g[i] = used[i];
   if (tid == 0) {
     st->sl = acc;
@@ -765,7 +787,7 @@ constexpr int SEL_ROUNDS = 2;
 
 void select_launch(DeviceStep &d, hipStream_t s) {
   const int ntiles = (d.nn + FILT_TILE - 1) / FILT_TILE;
-  const size_t lds = (size_t)d.n;
+  const size_t lds = (size_t)d.n + 1;
   for (int r = 0; r < SEL_ROUNDS; ++r) {
     const int limit = (r == SEL_ROUNDS - 1) ? 0 : d.select_limit;
     if (r == 0)
