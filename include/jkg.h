@@ -136,6 +136,37 @@ int jkg_dstep(jkg_ctx *ctx, int n, int n2, double *a, const int64_t *j, double *
  * (milliseconds summed over the steps); res as for jkg_dsolve. */
 int jkg_dprofile(jkg_ctx *ctx, int n, int n2, const double *y, const int64_t *j, int fused, int steps,
                  double *stage_ms, jkg_result *res);
+/* ---------------------------------------------------------------------------
+ * The mixed solver.  No counterpart in the reference: the fast path of this
+ * library for callers that need the decomposition, not the reference's exact
+ * trajectory.  Same dynamic ordering, termination and outputs as jkg_dsolve,
+ * but the hyperbolic weights come from the Gram form: one FP64 matrix-core
+ * product O(:,J=+1)^T O(:,J=-1) per step (O = A with a zero diagonal) and
+ * O(1) work per pair, instead of a scan of two columns per pair (ABODNDF2).
+ * Optionally an FP32 warm start, shortened candidate lists and a learned step
+ * controller.  Results agree with jkg_dsolve's to rounding, not bit for bit.
+ * --------------------------------------------------------------------------- */
+#define JKG_W_PAIR 0 /* per-pair column scans, as jkg_dsolve */
+#define JKG_W_GRAM 1 /* Gram form (default) */
+typedef struct {
+  int32_t weights;    /* JKG_W_PAIR or JKG_W_GRAM */
+  int32_t fp32_warm;  /* 1: FP32 steps until the off-norm stalls or falls below switch_tol, then FP64 */
+  double switch_tol;  /* relative off-norm that ends the FP32 phase (default 1e-5) */
+  int32_t key_bits;   /* 64: full sort keys (default); 32: the top 32 bits, ties by pair index */
+  int32_t controller; /* 0: N_2 and no threshold every step; 1: learned controller (ctl) */
+  const float *ctl;   /* controller parameters, or NULL */
+  double threshold;   /* fixed threshold: weights below threshold * (largest weight) dropped; 0: none */
+  int32_t reserved[8];
+} jkg_mixed_opts;
+void jkg_mixed_opts_default(jkg_mixed_opts *m);
+int jkg_dsolve_mixed(jkg_ctx *ctx, int n, int n2, const double *y, const int64_t *j, double *yu, double *zz,
+                     double *sy, const jkg_opts *opts, const jkg_mixed_opts *m, jkg_result *res);
+/* jkg_dweights_mixed: as jkg_dweights, with the weights of the mixed solver */
+int jkg_dweights_mixed(jkg_ctx *ctx, int n, const double *a, const int64_t *j, int fused, uint64_t *keys);
+/* jkg_dprofile_mixed: as jkg_dprofile, for the steps of the mixed solver */
+int jkg_dprofile_mixed(jkg_ctx *ctx, int n, int n2, const double *y, const int64_t *j, int fused, int steps,
+                       const jkg_mixed_opts *m, double *stage_ms, jkg_result *res);
+
 /* index of pair (p,q), 1 <= p < q <= n, in the tie-break order of AW_CMP */
 uint32_t jkg_pair_index(int n, int p, int q);
 /* the weight a sort key stands for (NaN for the key of a dropped pair) */

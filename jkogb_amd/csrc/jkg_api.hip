@@ -37,10 +37,13 @@ struct jkg_ctx {
   std::vector<void *> allocs;
   DeviceStep d{};
   double *sigma = nullptr;
-  // captured step
+  // captured step (for the FMA policy and weight mode it was captured with)
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  int graph_fused = -1;
+  int graph_fused = -1, graph_wmode = -1;
+  // mixed solver buffers (allocated on first use)
+  std::vector<void *> mallocs;
+  int mixed_n = 0;
   // misc test buffers
   void *tmp = nullptr;
   size_t tmp_bytes = 0;
@@ -52,8 +55,12 @@ static void free_all(jkg_ctx *c) {
   c->exec = nullptr;
   c->graph = nullptr;
   c->graph_fused = -1;
+  c->graph_wmode = -1;
   for (void *p : c->allocs) (void)hipFree(p);
   c->allocs.clear();
+  for (void *p : c->mallocs) (void)hipFree(p);
+  c->mallocs.clear();
+  c->mixed_n = 0;
   c->n = 0;
   c->cplx = -1;
   c->trace_cap = 0;
@@ -140,6 +147,26 @@ static int ensure_buffers(jkg_ctx *c, int n, int cplx, int trace_cap) {
   c->n = n;
   c->cplx = cplx;
   c->trace_cap = trace_cap;
+  return JKG_OK;
+}
+
+// Buffers of the Gram-form weights: (O^T O)(plus, minus) and the column norms.
+static int ensure_mixed(jkg_ctx *c) {
+  DeviceStep &d = c->d;
+  if (c->mixed_n == d.n) return JKG_OK;
+  const size_t n = (size_t)d.n;
+  void *g = nullptr, *cn = nullptr;
+  if (hipMalloc(&g, sizeof(double) * (n / 2) * (n - n / 2)) != hipSuccess ||
+      hipMalloc(&cn, sizeof(double) * n) != hipSuccess) {
+    (void)hipGetLastError();
+    if (g) (void)hipFree(g);
+    return JKG_ENOMEM;
+  }
+  c->mallocs.push_back(g);
+  c->mallocs.push_back(cn);
+  d.gram = (double *)g;
+  d.coln = (double *)cn;
+  c->mixed_n = d.n;
   return JKG_OK;
 }
 
@@ -290,7 +317,7 @@ static int upload_problem(jkg_ctx *c, int n, int n2, const void *y, const int64_
 }
 
 static int capture_step(jkg_ctx *c, int fused) {
-  if (c->exec && c->graph_fused == fused) return JKG_OK;
+  if (c->exec && c->graph_fused == fused && c->graph_wmode == c->d.wmode) return JKG_OK;
   if (c->exec) (void)hipGraphExecDestroy(c->exec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
   c->exec = nullptr;
@@ -301,6 +328,7 @@ static int capture_step(jkg_ctx *c, int fused) {
   HIPCHK(hipStreamEndCapture(c->stream, &c->graph));
   HIPCHK(hipGraphInstantiate(&c->exec, c->graph, nullptr, nullptr, 0));
   c->graph_fused = fused;
+  c->graph_wmode = c->d.wmode;
   return JKG_OK;
 }
 
@@ -348,7 +376,7 @@ static int run_loop(jkg_ctx *c, const jkg_opts *o, jkg_result *res) {
 }
 
 static int solve_common(jkg_ctx *c, int cplx, int n, int n2, const void *y, const int64_t *j, void *yu, void *zz,
-                        double *sy, const jkg_opts *opts, jkg_result *res) {
+                        double *sy, const jkg_opts *opts, jkg_result *res, int wmode = 0) {
   if (!c) return -1;
   if (n < 3) return -2;
   if (!y) return -4;
@@ -371,6 +399,11 @@ static int solve_common(jkg_ctx *c, int cplx, int n, int n2, const void *y, cons
   int cap = std::max(o.trace_cap, 1);
   int rc = ensure_buffers(c, n, cplx, cap);
   if (rc) return rc;
+  c->d.wmode = wmode;
+  if (wmode) {
+    rc = ensure_mixed(c);
+    if (rc) return rc;
+  }
   // TT: the number of trigonometric pairs (the reference's R%TRU result)
   int npos = 0, nneg = 0;
   rc = upload_problem(c, n, n2, y, j, &npos, &nneg, o.max_steps, c->trace_cap);
@@ -397,6 +430,24 @@ static int solve_common(jkg_ctx *c, int cplx, int n, int n2, const void *y, cons
 extern "C" int jkg_dsolve(jkg_ctx *c, int n, int n2, const double *y, const int64_t *j, double *yu, double *zz,
                           double *sy, const jkg_opts *opts, jkg_result *res) {
   return solve_common(c, 0, n, n2, y, j, yu, zz, sy, opts, res);
+}
+
+extern "C" void jkg_mixed_opts_default(jkg_mixed_opts *m) {
+  memset(m, 0, sizeof(*m));
+  m->weights = JKG_W_GRAM;
+  m->switch_tol = 1e-5;
+  m->key_bits = 64;
+}
+
+extern "C" int jkg_dsolve_mixed(jkg_ctx *c, int n, int n2, const double *y, const int64_t *j, double *yu,
+                                double *zz, double *sy, const jkg_opts *opts, const jkg_mixed_opts *mopts,
+                                jkg_result *res) {
+  jkg_mixed_opts m;
+  if (mopts) m = *mopts;
+  else jkg_mixed_opts_default(&m);
+  if (m.weights != JKG_W_PAIR && m.weights != JKG_W_GRAM) return -10;
+  if (m.fp32_warm != 0 || m.key_bits != 64 || m.controller != 0 || m.threshold != 0.0) return JKG_ENOTSUP;
+  return solve_common(c, 0, n, n2, y, j, yu, zz, sy, opts, res, m.weights == JKG_W_GRAM ? 1 : 0);
 }
 
 extern "C" int jkg_zsolve(jkg_ctx *c, int n, int n2, const double *y, const int64_t *j, double *yu, double *zz,
@@ -502,6 +553,7 @@ static int prepare_single(jkg_ctx *c, int n, int n2, const void *a, const int64_
   int rc = ensure_buffers(c, n, cplx, 1);
   if (rc) return rc;
   c->d.fused = fused;
+  c->d.wmode = 0;
   int npos, nneg;
   return upload_problem(c, n, jkg_jstep_len(n, n2), a, j, &npos, &nneg, -1, c->trace_cap);
 }
@@ -512,6 +564,23 @@ extern "C" int jkg_dweights(jkg_ctx *c, int n, const double *a, const int64_t *j
   if (rc) return rc;
   step_begin_launch(c->d, c->stream);
   weights_launch(c->d, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(keys, c->d.sort.k0, sizeof(uint64_t) * c->d.nn, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return JKG_OK;
+}
+
+extern "C" int jkg_dweights_mixed(jkg_ctx *c, int n, const double *a, const int64_t *j, int fused,
+                                  uint64_t *keys) {
+  if (!c) return -1;
+  int rc = prepare_single(c, n, 0, a, j, fused);
+  if (rc) return rc;
+  rc = ensure_mixed(c);
+  if (rc) return rc;
+  c->d.wmode = 1;
+  step_begin_launch(c->d, c->stream);
+  weights_launch(c->d, c->stream);
+  c->d.wmode = 0;
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(keys, c->d.sort.k0, sizeof(uint64_t) * c->d.nn, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -736,8 +805,8 @@ extern "C" int jkg_ncp(jkg_ctx *c, int n, int nn, int nm, jkg_aw *dz, int n2, in
 }
 
 // ---------------------------------------------------------------- profiling
-extern "C" int jkg_dprofile(jkg_ctx *c, int n, int n2, const double *y, const int64_t *j, int fused, int steps,
-                            double *stage_ms, jkg_result *res) {
+static int profile_common(jkg_ctx *c, int n, int n2, const double *y, const int64_t *j, int fused, int steps,
+                          double *stage_ms, jkg_result *res, int wmode) {
   if (!c) return -1;
   if (n < 3) return -2;
   if (steps < 1) return -7;
@@ -746,6 +815,11 @@ extern "C" int jkg_dprofile(jkg_ctx *c, int n, int n2, const double *y, const in
   n2 = jkg_jstep_len(n, n2);
   int rc = ensure_buffers(c, n, 0, steps + 1);
   if (rc) return rc;
+  c->d.wmode = wmode;
+  if (wmode) {
+    rc = ensure_mixed(c);
+    if (rc) return rc;
+  }
   int npos, nneg;
   rc = upload_problem(c, n, n2, y, j, &npos, &nneg, steps, c->trace_cap);
   if (rc) return rc;
@@ -796,4 +870,19 @@ extern "C" int jkg_dprofile(jkg_ctx *c, int n, int n2, const double *y, const in
   for (int k = 0; k < NST; ++k) tot += stage_ms[k];
   res->seconds = tot * 1e-3;
   return JKG_OK;
+}
+
+extern "C" int jkg_dprofile(jkg_ctx *c, int n, int n2, const double *y, const int64_t *j, int fused, int steps,
+                            double *stage_ms, jkg_result *res) {
+  return profile_common(c, n, n2, y, j, fused, steps, stage_ms, res, 0);
+}
+
+extern "C" int jkg_dprofile_mixed(jkg_ctx *c, int n, int n2, const double *y, const int64_t *j, int fused,
+                                  int steps, const jkg_mixed_opts *mopts, double *stage_ms, jkg_result *res) {
+  jkg_mixed_opts m;
+  if (mopts) m = *mopts;
+  else jkg_mixed_opts_default(&m);
+  if (m.weights != JKG_W_PAIR && m.weights != JKG_W_GRAM) return -8;
+  if (m.fp32_warm != 0 || m.key_bits != 64 || m.controller != 0 || m.threshold != 0.0) return JKG_ENOTSUP;
+  return profile_common(c, n, n2, y, j, fused, steps, stage_ms, res, m.weights == JKG_W_GRAM ? 1 : 0);
 }

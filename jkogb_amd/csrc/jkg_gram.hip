@@ -1,0 +1,205 @@
+// jkg_gram.hip -- pair weights of the mixed solver in Gram form, and the FP64
+// MFMA TN product they rest on.
+//
+// The reference's hyperbolic weight (DMAGF2H + ABODNDF2, src/D/dstep.F90:41-72,
+// src/D/dtransf.F90:246-348) is the off-diagonal Frobenius^2 of columns p, q
+// before minus after the 2x2 column transform B, rows p and q excepted, plus
+// two block terms.  With x = A(:,p), y = A(:,q) over the rows r not in {p, q},
+// x' = B11 x + B21 y and y' = B12 x + B22 y, that sum is a quadratic form in
+//   gxx = sum x_r^2,  gyy = sum y_r^2,  gxy = sum x_r y_r:
+//   W = d0 + gxx (1 - B11^2 - B12^2) + gyy (1 - B21^2 - B22^2)
+//          - 2 gxy (B11 B21 + B12 B22).
+// gxy for every hyperbolic pair is one product O(:,plus)^T O(:,minus), where O
+// is A with its diagonal zeroed: (O^T O)_pq sums exactly the rows r != p, q,
+// so no diagonal term has to be subtracted (no cancellation near convergence).
+// gxx = c_p - A_qp^2 with c_p the off-diagonal column norm^2.  The product
+// runs on the FP64 matrix cores; the weights are the reference's up to
+// rounding (not bit for bit), which is what the mixed solver needs.
+#include "jkg_device.hpp"
+#include "jkg_hsvd2.hpp"
+#include "jkg_step.hpp"
+
+namespace jkg {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// ------------------------------------------------------------------ TN product
+// C[i * csi + j * csj] = sum_{r < k} X[r, xc(i)] * Y[r, yc(j)]
+//   xc(i) = xcols ? xcols[i] : i (i < m), yc(j) = ycols ? ycols[j] : j (j < nc);
+// with ZD, X[r, c] and Y[r, c] read as 0 for r == c (the diagonal of O).
+// A 256-lane workgroup computes a 128 x 128 tile; each wave 64 x 64 as 4 x 4
+// MFMA f64 16x16x4 tiles.  Rows are staged 16 at a time through LDS (two
+// buffers: the next stage's global loads are in flight during the MFMAs).
+// MFMA f64 16x16x4 lane layout: A[i = l % 16][k = l / 16], B[k = l / 16][j =
+// l % 16], D item t: [i = 4 t + l / 16][j = l % 16].
+constexpr int GB_M = 128, GB_N = 128, GB_K = 16, GB_LD = GB_M + 4;
+
+template <bool ZD>
+__global__ void __launch_bounds__(256) k_gemm_tn(int k, const double *__restrict__ X, int ldx,
+                                                 const int *__restrict__ xcols, int m, const double *__restrict__ Y,
+                                                 int ldy, const int *__restrict__ ycols, int nc, double *__restrict__ C,
+                                                 long csi, long csj, const StepState *st) {
+  if (st && !st->active) return;
+  __shared__ double Xs[2][GB_K][GB_LD];
+  __shared__ double Ys[2][GB_K][GB_LD];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int i0 = blockIdx.y * GB_M, j0 = blockIdx.x * GB_N;
+  // loader: lane pair (c = tid / 2) holds rows (tid & 1) * 8 .. + 7 of column c
+  const int lc = tid >> 1, lh = (tid & 1) * 8;
+  const int xi = i0 + lc, yj = j0 + lc;
+  const int xcol = (xi < m) ? (xcols ? xcols[xi] : xi) : -1;
+  const int ycol = (yj < nc) ? (ycols ? ycols[yj] : yj) : -1;
+  const double *xp = X + (size_t)(xcol < 0 ? 0 : xcol) * ldx;
+  const double *yp = Y + (size_t)(ycol < 0 ? 0 : ycol) * ldy;
+  double xr[8], yr[8];
+  auto gload = [&](int r0) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int r = r0 + lh + u;
+      xr[u] = (xcol >= 0 && r < k && !(ZD && r == xcol)) ? xp[r] : 0.0;
+      yr[u] = (ycol >= 0 && r < k && !(ZD && r == ycol)) ? yp[r] : 0.0;
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      Xs[buf][lh + u][lc] = xr[u];
+      Ys[buf][lh + u][lc] = yr[u];
+    }
+  };
+  d4 acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = (d4){0.0, 0.0, 0.0, 0.0};
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  const int nst = (k + GB_K - 1) / GB_K;
+  for (int s = 0; s < nst; ++s) {
+    const int buf = s & 1;
+    if (s + 1 < nst) gload((s + 1) * GB_K);
+#pragma unroll
+    for (int kk = 0; kk < GB_K; kk += 4) {
+      double af[4], bf[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) af[a] = Xs[buf][kk + (lane >> 4)][wm * 64 + a * 16 + (lane & 15)];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) bf[b] = Ys[buf][kk + (lane >> 4)][wn * 64 + b * 16 + (lane & 15)];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[a], bf[b], acc[a][b], 0, 0, 0);
+    }
+    if (s + 1 < nst) sstore(buf ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int i = i0 + wm * 64 + a * 16 + 4 * t + (lane >> 4);
+        const int j = j0 + wn * 64 + b * 16 + (lane & 15);
+        if (i < m && j < nc) C[i * csi + j * csj] = acc[a][b][t];
+      }
+}
+
+void gemm_tn_launch(int k, const double *X, int ldx, const int *xcols, int m, const double *Y, int ldy,
+                    const int *ycols, int nc, double *C, long csi, long csj, bool zero_diag, const StepState *st,
+                    hipStream_t s) {
+  if (m <= 0 || nc <= 0) return;
+  dim3 grid((nc + GB_N - 1) / GB_N, (m + GB_M - 1) / GB_M);
+  if (zero_diag)
+    hipLaunchKernelGGL(k_gemm_tn<true>, grid, dim3(256), 0, s, k, X, ldx, xcols, m, Y, ldy, ycols, nc, C, csi, csj,
+                       st);
+  else
+    hipLaunchKernelGGL(k_gemm_tn<false>, grid, dim3(256), 0, s, k, X, ldx, xcols, m, Y, ldy, ycols, nc, C, csi,
+                       csj, st);
+}
+
+// ------------------------------------------------------------------ column norms
+// c[col] = sum_{r != col} A[r, col]^2, one wave per column.
+__global__ void __launch_bounds__(256) k_colnorm_off(int n, const double *__restrict__ A, int lda,
+                                                     double *__restrict__ c, const StepState *st) {
+  if (!st->active) return;
+  const int col = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (col >= n) return;
+  const double *a = A + (size_t)col * lda;
+  double s = 0.0;
+  for (int r = lane; r < n; r += 64) {
+    const double v = (r == col) ? 0.0 : a[r];
+    s = __builtin_fma(v, v, s);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (lane == 0) c[col] = s;
+}
+
+// ------------------------------------------------------------------ weights
+// One lane per hyperbolic pair (plus index i, minus index j).  The branch
+// structure is DMAGF2H's; only the ABODNDF2 sum is replaced by the quadratic
+// form above.
+template <bool F>
+__global__ void __launch_bounds__(256) k_hyp_weights_gram(int n, const double *__restrict__ A, int lda,
+                                                          const int *__restrict__ J, const int *__restrict__ plus,
+                                                          int npl, const int *__restrict__ minus, int nmi,
+                                                          const double *__restrict__ G, int ldg,
+                                                          const double *__restrict__ coln,
+                                                          uint64_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                                                          StepState *st) {
+  if (!st->active) return;
+  const int j = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const bool valid = i < npl && j < nmi;
+  bool full = false;
+  if (valid) {
+    const int cp = plus[i], cm = minus[j];
+    const int p0 = cp < cm ? cp : cm, q0 = cp < cm ? cm : cp;
+    const size_t P = p0, Q = q0;
+    const double app = A[P * lda + P], aqp = A[P * lda + Q], apq = A[Q * lda + P], aqq = A[Q * lda + Q];
+    uint64_t key = KEY_NAN;
+    if (aqp != 0.0 || apq != 0.0 || sign_neg(app) || sign_neg(aqq)) {
+      double a2[4] = {app, aqp, apq, aqq}, u2[4], z2[4];
+      int j2[2] = {J[p0], J[q0]};
+      const int info = dhsvd2<F>(a2, j2, u2, z2, TH1FIX);
+      if (info < 0) key = KEY_NAN;
+      else if ((info & 1) == 0) key = weight_key(0.0);
+      else if ((info & 4) == 0) key = weight_key(dasum2<F>(aqp, apq));
+      else {
+        full = true;
+        const double aq = (info & 8) ? a2[1] : 0.0, ap = (info & 8) ? a2[2] : 0.0;
+        double d = fmad<F>(aqp - aq, aqp + aq, 0.0);
+        d = fmad<F>(apq - ap, apq + ap, d);
+        const double gxx = coln[p0] - aqp * aqp, gyy = coln[q0] - apq * apq, gxy = G[(size_t)i * ldg + j];
+        const double b11 = z2[0], b21 = z2[1], b12 = z2[2], b22 = z2[3];
+        const double w = d + gxx * (1.0 - b11 * b11 - b12 * b12) + gyy * (1.0 - b21 * b21 - b22 * b22) -
+                         2.0 * gxy * (b11 * b21 + b12 * b22);
+        key = (w < -1.7976931348623157e308 || !(w == w)) ? KEY_NAN : weight_key(w);
+      }
+    }
+    const uint32_t k = pair_index(n, p0 + 1, q0 + 1);
+    keys[k] = key;
+    vals[k] = pack_pq(p0, q0);
+  }
+  const uint64_t b = __ballot(full);
+  if ((threadIdx.x & 63) == 0 && b) atomicAdd((unsigned long long *)&st->acc_abod, (unsigned long long)__popcll(b));
+}
+
+void gram_hyp_weights_launch(DeviceStep &d, hipStream_t s) {
+  if (d.npl <= 0 || d.nmi <= 0) return;
+  const double *A = (const double *)d.A;
+  hipLaunchKernelGGL(k_colnorm_off, dim3((d.n + 3) / 4), dim3(256), 0, s, d.n, A, d.lda, d.coln, d.st);
+  gemm_tn_launch(d.n, A, d.lda, d.plus, d.npl, A, d.lda, d.minus, d.nmi, d.gram, (long)d.nmi, 1L, true, d.st, s);
+  dim3 g((d.nmi + 63) / 64, (d.npl + 3) / 4);
+  if (d.fused)
+    hipLaunchKernelGGL(k_hyp_weights_gram<true>, g, dim3(256), 0, s, d.n, A, d.lda, d.J, d.plus, d.npl, d.minus,
+                       d.nmi, d.gram, d.nmi, d.coln, d.sort.k0, d.sort.v0, d.st);
+  else
+    hipLaunchKernelGGL(k_hyp_weights_gram<false>, g, dim3(256), 0, s, d.n, A, d.lda, d.J, d.plus, d.npl, d.minus,
+                       d.nmi, d.gram, d.nmi, d.coln, d.sort.k0, d.sort.v0, d.st);
+}
+
+}  // namespace jkg

@@ -461,4 +461,15 @@ __device__ int zhsvd2(dz A[4], const int J[2], dz U[4], dz Z[4], double th1) {
   return info;
 }
 
+// |x1|^2 + |x2|^2 scaled by the larger magnitude (DASUM2, src/D/dtransf.F90:12-38)
+template <bool F>
+__device__ __forceinline__ double dasum2(double x1, double x2) {
+  double a = __builtin_fabs(x1), b = __builtin_fabs(x2);
+  if (a >= b) {
+    if (a == 0.0) return 0.0;
+    return fmad<F>(b, (b / a), a) * a;
+  }
+  return fmad<F>(a, (a / b), b) * b;
+}
+
 }  // namespace jkg

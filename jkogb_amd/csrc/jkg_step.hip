@@ -25,16 +25,6 @@
 namespace jkg {
 
 // ------------------------------------------------------------------ helpers
-template <bool F>
-__device__ __forceinline__ double dasum2(double x1, double x2) {  // src/D/dtransf.F90:12-38
-  double a = __builtin_fabs(x1), b = __builtin_fabs(x2);
-  if (a >= b) {
-    if (a == 0.0) return 0.0;
-    return fmad<F>(b, (b / a), a) * a;
-  }
-  return fmad<F>(a, (a / b), b) * b;
-}
-
 // ratio-form 2x2 transform of the pair (x, y) (BA or AB, see Rot)
 template <bool F>
 __device__ __forceinline__ void rot_apply(const Rot &R, double &x, double &y) {
@@ -771,6 +761,10 @@ void weights_launch(DeviceStep &d, hipStream_t s) {
   } else {
     hipLaunchKernelGGL(k_trig_weights<false>, dim3(grid_for(d.nn, 256, 65536)), dim3(256), 0, s, d.n, A, d.lda, d.J,
                        d.sort.k0, d.sort.v0, d.st, d.nn);
+  }
+  if (d.wmode == 1) {
+    gram_hyp_weights_launch(d, s);
+    return;
   }
   if (d.npl > 0 && d.nmi > 0) {
     dim3 g((d.npl + HT - 1) / HT, (d.nmi + HT - 1) / HT);

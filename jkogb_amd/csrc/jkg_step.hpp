@@ -83,6 +83,10 @@ struct DeviceStep {
   StepState *st;
   StepRec *trace;
   int select_limit;       // entries walked per selection round before filtering
+  // mixed solver
+  int wmode;              // hyperbolic weights: 0 per pair (ABODNDF2), 1 Gram form
+  double *gram;           // npl x nmi: (O^T O)(plus, minus), O = A with a zero diagonal
+  double *coln;           // N: off-diagonal column norms^2
 };
 
 // sets kernel attributes (dynamic LDS) for the problem size; before capture
@@ -106,5 +110,13 @@ void select_launch(DeviceStep &d, hipStream_t s);
 void pivots_launch(DeviceStep &d, hipStream_t s);
 void rotate_launch(DeviceStep &d, hipStream_t s);
 void step_end_launch(DeviceStep &d, hipStream_t s);
+
+// Gram-form hyperbolic weights (jkg_gram.hip) and the FP64 MFMA TN product
+//   C[i * csi + j * csj] = sum_{r < k} X[r, xc(i)] * Y[r, yc(j)]
+// (xc(i) = xcols ? xcols[i] : i; zero_diag reads X[c, c] and Y[c, c] as 0).
+void gram_hyp_weights_launch(DeviceStep &d, hipStream_t s);
+void gemm_tn_launch(int k, const double *X, int ldx, const int *xcols, int m, const double *Y, int ldy,
+                    const int *ycols, int nc, double *C, long csi, long csj, bool zero_diag, const StepState *st,
+                    hipStream_t s);
 
 }  // namespace jkg

@@ -46,6 +46,15 @@ class Result(C.Structure):
                 ("sum_it", C.c_int64), ("sum_b2", C.c_int64), ("sum_b4", C.c_int64), ("sum_abod", C.c_int64)]
 
 
+class MixedOpts(C.Structure):
+    """jkg_mixed_opts: options of the mixed solver (no reference counterpart)."""
+    _fields_ = [("weights", C.c_int32), ("fp32_warm", C.c_int32), ("switch_tol", C.c_double),
+                ("key_bits", C.c_int32), ("controller", C.c_int32), ("ctl", C.c_void_p),
+                ("threshold", C.c_double), ("reserved", C.c_int32 * 8)]
+
+
+W_PAIR, W_GRAM = 0, 1
+
 STAGES = ["begin", "weights", "sort", "select", "pivots", "rotate", "end"]
 
 
@@ -78,13 +87,31 @@ _zweights = _sig("jkg_zweights", _i, [_vp, _i, _vp, _vp, _i, _vp])
 _dbuild = _sig("jkg_dbuild", _i, [_vp, _i, _i, _vp, _vp, _i, C.POINTER(C.c_int32), _vp, C.POINTER(C.c_int32), _vp])
 _dstep = _sig("jkg_dstep", _i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _i, _vp])
 _dprofile = _sig("jkg_dprofile", _i, [_vp, _i, _i, _vp, _vp, _i, _i, _vp, C.POINTER(Result)])
+_mixed_default = _sig("jkg_mixed_opts_default", None, [C.POINTER(MixedOpts)])
+_dsolve_mixed = _sig("jkg_dsolve_mixed", _i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, C.POINTER(Opts),
+                                              C.POINTER(MixedOpts), C.POINTER(Result)])
+_dweights_mixed = _sig("jkg_dweights_mixed", _i, [_vp, _i, _vp, _vp, _i, _vp])
+_dprofile_mixed = _sig("jkg_dprofile_mixed", _i, [_vp, _i, _i, _vp, _vp, _i, _i, C.POINTER(MixedOpts), _vp,
+                                                  C.POINTER(Result)])
 _pair_index = _sig("jkg_pair_index", C.c_uint32, [_i, _i, _i])
 _key_weight = _sig("jkg_key_weight", _d, [C.c_uint64])
 
 EXPORTS = ["jkg_create", "jkg_destroy", "jkg_strerror", "jkg_last_hip_error", "jkg_opts_default", "jkg_dsolve",
            "jkg_zsolve", "jkg_srt", "jkg_ncp", "jkg_tru", "jkg_jstep_len", "jkg_dhsvd2", "jkg_zhsvd2",
            "jkg_dweights", "jkg_zweights", "jkg_dbuild", "jkg_dstep", "jkg_dprofile", "jkg_pair_index",
-           "jkg_key_weight"]
+           "jkg_key_weight", "jkg_mixed_opts_default", "jkg_dsolve_mixed", "jkg_dweights_mixed",
+           "jkg_dprofile_mixed"]
+
+
+def mixed_opts(**kw):
+    """jkg_mixed_opts with the library's defaults, then the given fields."""
+    m = MixedOpts()
+    _mixed_default(C.byref(m))
+    for k, v in kw.items():
+        if not hasattr(m, k):
+            raise KeyError(k)
+        setattr(m, k, v)
+    return m
 
 
 class JKGError(RuntimeError):
@@ -188,6 +215,48 @@ class Context:
              "jkg_zsolve" if cplx else "jkg_dsolve")
         return dict(YU=YU, ZZ=ZZ, SY=SY, info=r.info, trace=trace[:min(r.steps, trace_cap)], tt=r.tt,
                     pn=(r.npos, r.nneg), n2=r.n2, seconds=r.seconds, steps=r.steps, **_counts(r))
+
+    def dsolve_mixed(self, Y, J, n2=0, fused=False, max_steps=-1, trace_cap=None, steps_per_sync=0, **mixed):
+        """The mixed solver (jkg_dsolve_mixed): same outputs as dsolve, agreeing to rounding."""
+        Y = np.asfortranarray(Y, dtype=np.float64)
+        n = Y.shape[0]
+        if Y.shape != (n, n):
+            raise ValueError("Y must be square")
+        J64 = np.ascontiguousarray(J, dtype=np.int64)
+        YU = np.zeros((n, n), order="F")
+        ZZ = np.zeros((n, n), order="F")
+        SY = np.zeros(n)
+        if trace_cap is None:
+            trace_cap = 64 * n + 64 if max_steps < 0 else max_steps + 1
+        trace = np.zeros(max(trace_cap, 1), STEPREC_DTYPE)
+        o = Opts(0, 0, 1 if fused else 0, max_steps, steps_per_sync, trace_cap, _p(trace))
+        m = mixed_opts(**mixed)
+        r = Result()
+        _chk(_dsolve_mixed(self._h, n, n2, _p(Y), _p(J64), _p(YU), _p(ZZ), _p(SY), C.byref(o), C.byref(m),
+                           C.byref(r)), "jkg_dsolve_mixed")
+        return dict(YU=YU, ZZ=ZZ, SY=SY, info=r.info, trace=trace[:min(r.steps, trace_cap)], tt=r.tt,
+                    pn=(r.npos, r.nneg), n2=r.n2, seconds=r.seconds, steps=r.steps, **_counts(r))
+
+    def dprofile_mixed(self, Y, J, n2=0, steps=4, fused=False, **mixed):
+        """As dprofile, for the steps of the mixed solver."""
+        Y = np.asfortranarray(Y, dtype=np.float64)
+        n = Y.shape[0]
+        ms = np.zeros(len(STAGES))
+        r = Result()
+        m = mixed_opts(**mixed)
+        _chk(_dprofile_mixed(self._h, n, n2, _p(Y), _p(np.ascontiguousarray(J, dtype=np.int64)), 1 if fused else 0,
+                             steps, C.byref(m), _p(ms), C.byref(r)), "jkg_dprofile_mixed")
+        return dict(stage_ms=dict(zip(STAGES, ms.tolist())), steps=r.steps, n2=r.n2, seconds=r.seconds,
+                    **_counts(r))
+
+    def dweights_mixed(self, A, J, fused=False):
+        """Sort keys of the mixed solver's pair weights, indexed by pair_index."""
+        A = np.asfortranarray(A, dtype=np.float64)
+        n = A.shape[0]
+        keys = np.zeros(n * (n - 1) // 2, np.uint64)
+        _chk(_dweights_mixed(self._h, n, _p(A), _p(np.ascontiguousarray(J, dtype=np.int64)), 1 if fused else 0,
+                             _p(keys)), "jkg_dweights_mixed")
+        return keys
 
     def dprofile(self, Y, J, n2=0, steps=4, fused=False):
         """Per-stage device time (ms summed over `steps` steps from Y) and work counts."""

@@ -1,0 +1,135 @@
+"""The mixed solver (jkg_dsolve_mixed) through the C ABI.
+
+Its weights are the reference's up to rounding, so the checks here are
+tolerances, written in each test:
+  - pair weights: every pair whose weight the mixed solver computes with the
+    reference's own formula (trigonometric pairs, DMAGF2H's early returns) is
+    bit-identical to the oracle; the Gram-form hyperbolic weights are within
+    32 N eps of a per-pair scale s_pq = (c_p + c_q)(1 + |B|_F^2) + A_qp^2 + A_pq^2,
+    c = off-diagonal column norms^2, B = the pair's 2x2 column factor;
+  - whole solves: the hyperbolic singular values, sorted within each J class,
+    agree with the plain solver's (bit-exact with the reference) to 64 N eps
+    relatively; the reconstruction, orthogonality and J-orthogonality residuals
+    stay within 4x the plain solver's + 2 N eps (the first two also within
+    64 N eps); the step count stays within 25 % of the plain solver's.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+EPS = np.finfo(np.float64).eps
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from jkogb_amd import jkg
+    c = jkg.Context(0)
+    yield c
+    c.close()
+
+
+def inputs(cfg, n, seed):
+    from jkogb_amd import inputs as gen
+    return gen.make(cfg, n=n, seed=seed)
+
+
+def check_weights(ctx, orc, A, J):
+    from jkogb_amd import jkg
+    A = np.asfortranarray(A)
+    n = A.shape[0]
+    wf = orc.pair_weights(A, J)
+    km = ctx.dweights_mixed(A, J)
+    wm = jkg.keys_to_weights(km)
+    O = A.copy()
+    np.fill_diagonal(O, 0.0)
+    c = np.einsum("ij,ij->j", O, O)
+    ngram = 0
+    for m in range(1, n):
+        for p in range(m, 0, -1):
+            q = p + n - m
+            k = m * (m - 1) // 2 + (m - p)
+            if np.isnan(wf[k]):
+                assert np.isnan(wm[k]), (p, q)
+                continue
+            P, Q = p - 1, q - 1
+            if J[P] == J[Q]:
+                assert wm[k] == wf[k] and np.signbit(wm[k]) == np.signbit(wf[k]), (p, q)
+                continue
+            a2, u2, z2, info = orc.dhsvd2([[A[P, P], A[P, Q]], [A[Q, P], A[Q, Q]]], [J[P], J[Q]])
+            if info < 0 or (info & 1) == 0 or (info & 4) == 0:
+                assert wm[k] == wf[k], (p, q, info)
+                continue
+            ngram += 1
+            s = (c[P] + c[Q]) * (1.0 + np.sum(z2 * z2)) + A[Q, P] ** 2 + A[P, Q] ** 2
+            assert abs(wm[k] - wf[k]) <= 32 * n * EPS * s, (p, q, wm[k], wf[k], s)
+    return ngram
+
+
+@pytest.mark.parametrize("cfg,n,steps", [("c1", 40, 0), ("c2", 48, 30), ("c5", 56, 80), ("c5", 33, 200)])
+def test_gram_weights_vs_oracle(ctx, orc, cfg, n, steps):
+    Y, J = inputs(cfg, n, 11)
+    A = orc.solve(Y, J, n // 2, max_steps=steps)["A"] if steps else Y
+    ngram = check_weights(ctx, orc, A, J)
+    assert ngram > 0 or steps > 100
+
+
+def test_gram_weights_tile_edges(ctx, orc):
+    # J classes of sizes that leave partial 128-column tiles and 64-lane rows
+    rng = np.random.default_rng(5)
+    n = 150
+    Y = rng.standard_normal((n, n)) * 10.0 ** rng.uniform(-3, 3, n)
+    J = np.where(rng.random(n) < 0.8, 1, -1)
+    assert check_weights(ctx, orc, Y, J) > 0
+
+
+def residuals(Y, J, out):
+    n = Y.shape[0]
+    YU, ZZ, SY = out["YU"], out["ZZ"], out["SY"]
+    rec = np.linalg.norm(Y - (YU * SY) @ ZZ) / np.linalg.norm(Y)
+    orth = np.linalg.norm(YU.T @ YU - np.eye(n))
+    Jd = np.diag(np.asarray(J, dtype=np.float64))
+    jorth = np.linalg.norm(ZZ @ Jd @ ZZ.T - Jd)
+    return rec, orth, jorth
+
+
+def sorted_by_class(SY, J):
+    J = np.asarray(J)
+    return np.sort(SY[J > 0]), np.sort(SY[J < 0])
+
+
+@pytest.mark.parametrize("cfg,n", [("c1", 96), ("c2", 128), ("c5", 160), ("c1", 203)])
+def test_mixed_solve_matches_plain(ctx, cfg, n):
+    Y, J = inputs(cfg, n, 3)
+    plain = ctx.dsolve(Y, J, n // 2)
+    mix = ctx.dsolve_mixed(Y, J, n // 2)
+    assert plain["info"] >= 0 and mix["info"] >= 0
+    for a, b in zip(sorted_by_class(mix["SY"], J), sorted_by_class(plain["SY"], J)):
+        rel = np.abs(a - b) / np.abs(b)
+        assert rel.max() <= 64 * n * EPS, rel.max()
+    rec, orth, jorth = residuals(Y, J, mix)
+    prec, porth, pjorth = residuals(Y, J, plain)
+    # as good as the plain solver's (the reference's own level), to a factor, and 64 N eps at most
+    assert rec <= min(4 * prec + 2 * n * EPS, 64 * n * EPS), (rec, prec)
+    assert orth <= min(4 * porth + 2 * n * EPS, 64 * n * EPS), (orth, porth)
+    # J-orthogonality: hyperbolic factors may be ill-conditioned, so relative to the plain solver's only
+    assert jorth <= 4 * pjorth + 2 * n * EPS, (jorth, pjorth)
+    assert abs(mix["steps"] - plain["steps"]) <= 0.25 * plain["steps"], (mix["steps"], plain["steps"])
+
+
+def test_mixed_pair_weights_mode_is_the_plain_solver(ctx):
+    # weights=JKG_W_PAIR runs the reference's weights: the plain trajectory bit for bit
+    Y, J = inputs("c2", 64, 9)
+    plain = ctx.dsolve(Y, J, 32)
+    mix = ctx.dsolve_mixed(Y, J, 32, weights=0)
+    assert np.array_equal(plain["SY"].view(np.uint64), mix["SY"].view(np.uint64))
+    assert np.array_equal(plain["YU"].view(np.uint64), mix["YU"].view(np.uint64))
+    assert plain["steps"] == mix["steps"]
+
+
+def test_mixed_rejects_unsupported(ctx):
+    from jkogb_amd import jkg
+    Y, J = inputs("c1", 16, 1)
+    with pytest.raises(jkg.JKGError) as e:
+        ctx.dsolve_mixed(Y, J, 8, weights=7)
+    assert e.value.code == -10
