@@ -40,7 +40,7 @@ struct jkg_ctx {
   // captured step (for the FMA policy and weight mode it was captured with)
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  int graph_fused = -1, graph_wmode = -1;
+  int graph_fused = -1, graph_wmode = -1, graph_npl = -1;  // launch shapes depend on the J split
   // mixed solver buffers (allocated on first use)
   std::vector<void *> mallocs;
   int mixed_n = 0;
@@ -56,6 +56,7 @@ static void free_all(jkg_ctx *c) {
   c->graph = nullptr;
   c->graph_fused = -1;
   c->graph_wmode = -1;
+  c->graph_npl = -1;
   for (void *p : c->allocs) (void)hipFree(p);
   c->allocs.clear();
   for (void *p : c->mallocs) (void)hipFree(p);
@@ -317,7 +318,8 @@ static int upload_problem(jkg_ctx *c, int n, int n2, const void *y, const int64_
 }
 
 static int capture_step(jkg_ctx *c, int fused) {
-  if (c->exec && c->graph_fused == fused && c->graph_wmode == c->d.wmode) return JKG_OK;
+  // the graph holds the grids and arguments of this problem's J split (npl, nmi)
+  if (c->exec && c->graph_fused == fused && c->graph_wmode == c->d.wmode && c->graph_npl == c->d.npl) return JKG_OK;
   if (c->exec) (void)hipGraphExecDestroy(c->exec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
   c->exec = nullptr;
@@ -329,6 +331,7 @@ static int capture_step(jkg_ctx *c, int fused) {
   HIPCHK(hipGraphInstantiate(&c->exec, c->graph, nullptr, nullptr, 0));
   c->graph_fused = fused;
   c->graph_wmode = c->d.wmode;
+  c->graph_npl = c->d.npl;
   return JKG_OK;
 }
 

@@ -318,3 +318,18 @@ def test_zsolve_fused_and_larger_vs_oracle(ctx, orc):
     r = ctx.dsolve(y, j, 100, max_steps=5)
     o = orc.solve(y, j, 100, max_steps=5)
     check_solve(r, o["SY"], o["YU"], o["ZZ"], steps=5)
+
+
+def test_context_reuse_with_another_j_split(ctx, orc):
+    # a second problem of the same N with a different number of +1 signs in the same context
+    # (the captured step must follow the new split)
+    Y1, J1 = inputs("c2", 48, 4)
+    Y2 = np.random.default_rng(8).standard_normal((48, 48))
+    J2 = np.array([1] * 30 + [-1] * 18)
+    np.random.default_rng(9).shuffle(J2)
+    for Y, J in [(Y1, J1), (Y2, J2), (Y1, J1)]:
+        out = ctx.dsolve(Y, J, 24)
+        ref = orc.solve(Y, J, 24)
+        assert out["info"] == ref["info"]
+        assert same_bits(out["SY"], ref["SY"])
+        assert same_bits(out["YU"], ref["YU"])
