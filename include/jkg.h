@@ -156,6 +156,9 @@ typedef struct {
   int32_t controller; /* 0: N_2 and no threshold every step; 1: learned controller (ctl) */
   const float *ctl;   /* controller parameters, or NULL */
   double threshold;   /* fixed threshold: weights below threshold * (largest weight) dropped; 0: none */
+  double tol;         /* convergence test: a pair with max(|a_pq|,|a_qp|) <= tol * sqrt(|a_pp| |a_qq|) whose
+                         transform would otherwise be the identity is dropped (default: DBL_EPSILON);
+                         0: the reference's test only */
   int32_t reserved[8];
 } jkg_mixed_opts;
 void jkg_mixed_opts_default(jkg_mixed_opts *m);

@@ -40,7 +40,14 @@ struct jkg_ctx {
   // captured step (for the FMA policy and weight mode it was captured with)
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  int graph_fused = -1, graph_wmode = -1, graph_npl = -1;  // launch shapes depend on the J split
+  // host values the captured graph holds as launch shapes or kernel arguments
+  struct GraphKey {
+    int fused = -1, wmode = -1, npl = -1;
+    double neg_tol = -1.0;
+    bool operator==(const GraphKey &o) const {
+      return fused == o.fused && wmode == o.wmode && npl == o.npl && neg_tol == o.neg_tol;
+    }
+  } gkey;
   // mixed solver buffers (allocated on first use)
   std::vector<void *> mallocs;
   int mixed_n = 0;
@@ -54,9 +61,7 @@ static void free_all(jkg_ctx *c) {
   if (c->graph) (void)hipGraphDestroy(c->graph);
   c->exec = nullptr;
   c->graph = nullptr;
-  c->graph_fused = -1;
-  c->graph_wmode = -1;
-  c->graph_npl = -1;
+  c->gkey = jkg_ctx::GraphKey{};
   for (void *p : c->allocs) (void)hipFree(p);
   c->allocs.clear();
   for (void *p : c->mallocs) (void)hipFree(p);
@@ -318,8 +323,12 @@ static int upload_problem(jkg_ctx *c, int n, int n2, const void *y, const int64_
 }
 
 static int capture_step(jkg_ctx *c, int fused) {
-  // the graph holds the grids and arguments of this problem's J split (npl, nmi)
-  if (c->exec && c->graph_fused == fused && c->graph_wmode == c->d.wmode && c->graph_npl == c->d.npl) return JKG_OK;
+  jkg_ctx::GraphKey key;
+  key.fused = fused;
+  key.wmode = c->d.wmode;
+  key.npl = c->d.npl;  // grids and arguments follow this problem's J split (npl, nmi)
+  key.neg_tol = c->d.neg_tol;
+  if (c->exec && c->gkey == key) return JKG_OK;
   if (c->exec) (void)hipGraphExecDestroy(c->exec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
   c->exec = nullptr;
@@ -329,9 +338,7 @@ static int capture_step(jkg_ctx *c, int fused) {
   step_launch(c->d, c->stream);
   HIPCHK(hipStreamEndCapture(c->stream, &c->graph));
   HIPCHK(hipGraphInstantiate(&c->exec, c->graph, nullptr, nullptr, 0));
-  c->graph_fused = fused;
-  c->graph_wmode = c->d.wmode;
-  c->graph_npl = c->d.npl;
+  c->gkey = key;
   return JKG_OK;
 }
 
@@ -379,7 +386,7 @@ static int run_loop(jkg_ctx *c, const jkg_opts *o, jkg_result *res) {
 }
 
 static int solve_common(jkg_ctx *c, int cplx, int n, int n2, const void *y, const int64_t *j, void *yu, void *zz,
-                        double *sy, const jkg_opts *opts, jkg_result *res, int wmode = 0) {
+                        double *sy, const jkg_opts *opts, jkg_result *res, int wmode = 0, double tol = 0.0) {
   if (!c) return -1;
   if (n < 3) return -2;
   if (!y) return -4;
@@ -403,6 +410,7 @@ static int solve_common(jkg_ctx *c, int cplx, int n, int n2, const void *y, cons
   int rc = ensure_buffers(c, n, cplx, cap);
   if (rc) return rc;
   c->d.wmode = wmode;
+  c->d.neg_tol = tol;
   if (wmode) {
     rc = ensure_mixed(c);
     if (rc) return rc;
@@ -440,6 +448,7 @@ extern "C" void jkg_mixed_opts_default(jkg_mixed_opts *m) {
   m->weights = JKG_W_GRAM;
   m->switch_tol = 1e-5;
   m->key_bits = 64;
+  m->tol = 0x1p-52;  // double epsilon
 }
 
 extern "C" int jkg_dsolve_mixed(jkg_ctx *c, int n, int n2, const double *y, const int64_t *j, double *yu,
@@ -449,8 +458,10 @@ extern "C" int jkg_dsolve_mixed(jkg_ctx *c, int n, int n2, const double *y, cons
   if (mopts) m = *mopts;
   else jkg_mixed_opts_default(&m);
   if (m.weights != JKG_W_PAIR && m.weights != JKG_W_GRAM) return -10;
+  if (!(m.tol >= 0.0)) return -10;
   if (m.fp32_warm != 0 || m.key_bits != 64 || m.controller != 0 || m.threshold != 0.0) return JKG_ENOTSUP;
-  return solve_common(c, 0, n, n2, y, j, yu, zz, sy, opts, res, m.weights == JKG_W_GRAM ? 1 : 0);
+  if (m.weights == JKG_W_PAIR && m.tol != 0.0) return JKG_ENOTSUP;
+  return solve_common(c, 0, n, n2, y, j, yu, zz, sy, opts, res, m.weights == JKG_W_GRAM ? 1 : 0, m.tol);
 }
 
 extern "C" int jkg_zsolve(jkg_ctx *c, int n, int n2, const double *y, const int64_t *j, double *yu, double *zz,
@@ -557,6 +568,7 @@ static int prepare_single(jkg_ctx *c, int n, int n2, const void *a, const int64_
   if (rc) return rc;
   c->d.fused = fused;
   c->d.wmode = 0;
+  c->d.neg_tol = 0.0;
   int npos, nneg;
   return upload_problem(c, n, jkg_jstep_len(n, n2), a, j, &npos, &nneg, -1, c->trace_cap);
 }
@@ -809,7 +821,7 @@ extern "C" int jkg_ncp(jkg_ctx *c, int n, int nn, int nm, jkg_aw *dz, int n2, in
 
 // ---------------------------------------------------------------- profiling
 static int profile_common(jkg_ctx *c, int n, int n2, const double *y, const int64_t *j, int fused, int steps,
-                          double *stage_ms, jkg_result *res, int wmode) {
+                          double *stage_ms, jkg_result *res, int wmode, double tol = 0.0) {
   if (!c) return -1;
   if (n < 3) return -2;
   if (steps < 1) return -7;
@@ -819,6 +831,7 @@ static int profile_common(jkg_ctx *c, int n, int n2, const double *y, const int6
   int rc = ensure_buffers(c, n, 0, steps + 1);
   if (rc) return rc;
   c->d.wmode = wmode;
+  c->d.neg_tol = tol;
   if (wmode) {
     rc = ensure_mixed(c);
     if (rc) return rc;
@@ -886,6 +899,8 @@ extern "C" int jkg_dprofile_mixed(jkg_ctx *c, int n, int n2, const double *y, co
   if (mopts) m = *mopts;
   else jkg_mixed_opts_default(&m);
   if (m.weights != JKG_W_PAIR && m.weights != JKG_W_GRAM) return -8;
+  if (!(m.tol >= 0.0)) return -8;
   if (m.fp32_warm != 0 || m.key_bits != 64 || m.controller != 0 || m.threshold != 0.0) return JKG_ENOTSUP;
-  return profile_common(c, n, n2, y, j, fused, steps, stage_ms, res, m.weights == JKG_W_GRAM ? 1 : 0);
+  if (m.weights == JKG_W_PAIR && m.tol != 0.0) return JKG_ENOTSUP;
+  return profile_common(c, n, n2, y, j, fused, steps, stage_ms, res, m.weights == JKG_W_GRAM ? 1 : 0, m.tol);
 }

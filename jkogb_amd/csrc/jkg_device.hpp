@@ -132,6 +132,16 @@ __host__ __device__ __forceinline__ uint32_t pack_pq(int p0, int q0) { return (u
 __host__ __device__ __forceinline__ int pq_p(uint32_t v) { return (int)(v & 0xFFFFu); }
 __host__ __device__ __forceinline__ int pq_q(uint32_t v) { return (int)(v >> 16); }
 
+// The mixed solver's convergence test for one pair: its off-diagonal entries
+// are negligible against the diagonal,
+//   max(|a_pq|, |a_qp|) <= tol * sqrt(|a_pp|) * sqrt(|a_qq|)
+// (the relative criterion of one-sided Jacobi methods; such entries move the
+// singular values by O(tol) relatively).
+__device__ __forceinline__ bool negligible(double aqp, double apq, double app, double aqq, double tol) {
+  const double m = f_max(__builtin_fabs(aqp), __builtin_fabs(apq));
+  return m <= tol * __builtin_sqrt(__builtin_fabs(app)) * __builtin_sqrt(__builtin_fabs(aqq));
+}
+
 // Complex FP64 value in the reference's COMPLEX(KIND=8) layout.
 struct dz {
   double re, im;

@@ -149,7 +149,7 @@ __global__ void __launch_bounds__(256) k_hyp_weights_gram(int n, const double *_
                                                           const double *__restrict__ G, int ldg,
                                                           const double *__restrict__ coln,
                                                           uint64_t *__restrict__ keys, uint32_t *__restrict__ vals,
-                                                          StepState *st) {
+                                                          StepState *st, double tol) {
   if (!st->active) return;
   const int j = blockIdx.x * 64 + (threadIdx.x & 63);
   const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -161,7 +161,9 @@ __global__ void __launch_bounds__(256) k_hyp_weights_gram(int n, const double *_
     const size_t P = p0, Q = q0;
     const double app = A[P * lda + P], aqp = A[P * lda + Q], apq = A[Q * lda + P], aqq = A[Q * lda + Q];
     uint64_t key = KEY_NAN;
-    if (aqp != 0.0 || apq != 0.0 || sign_neg(app) || sign_neg(aqq)) {
+    const bool order = sign_neg(app) || sign_neg(aqq);
+    const bool drop = tol > 0.0 && !order && negligible(aqp, apq, app, aqq, tol);
+    if ((aqp != 0.0 || apq != 0.0 || order) && !drop) {
       double a2[4] = {app, aqp, apq, aqq}, u2[4], z2[4];
       int j2[2] = {J[p0], J[q0]};
       const int info = dhsvd2<F>(a2, j2, u2, z2, TH1FIX);
@@ -196,10 +198,10 @@ void gram_hyp_weights_launch(DeviceStep &d, hipStream_t s) {
   dim3 g((d.nmi + 63) / 64, (d.npl + 3) / 4);
   if (d.fused)
     hipLaunchKernelGGL(k_hyp_weights_gram<true>, g, dim3(256), 0, s, d.n, A, d.lda, d.J, d.plus, d.npl, d.minus,
-                       d.nmi, d.gram, d.nmi, d.coln, d.sort.k0, d.sort.v0, d.st);
+                       d.nmi, d.gram, d.nmi, d.coln, d.sort.k0, d.sort.v0, d.st, d.neg_tol);
   else
     hipLaunchKernelGGL(k_hyp_weights_gram<false>, g, dim3(256), 0, s, d.n, A, d.lda, d.J, d.plus, d.npl, d.minus,
-                       d.nmi, d.gram, d.nmi, d.coln, d.sort.k0, d.sort.v0, d.st);
+                       d.nmi, d.gram, d.nmi, d.coln, d.sort.k0, d.sort.v0, d.st, d.neg_tol);
 }
 
 }  // namespace jkg

@@ -87,10 +87,14 @@ __global__ void k_step_begin(StepState *st, int nn, int n, int *colpiv) {
 
 // ------------------------------------------------------------------ (a) weights
 // DMAGF2T (src/D/dstep.F90:12-37) for the pairs with J_p == J_q.
+// tol > 0 (mixed solver only): a pair whose transform would be the identity
+// if its off-diagonal pair were zero and whose off-diagonal pair is negligible
+// (see negligible()) is dropped like the reference drops a zero pair.
 template <bool F>
 __global__ void __launch_bounds__(256) k_trig_weights(int n, const double *__restrict__ A, int lda,
                                                        const int *__restrict__ J, uint64_t *__restrict__ keys,
-                                                       uint32_t *__restrict__ vals, const StepState *st, int nn) {
+                                                       uint32_t *__restrict__ vals, const StepState *st, int nn,
+                                                       double tol) {
   if (!st->active) return;
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < (uint32_t)nn; k += gridDim.x * blockDim.x) {
     int p, q;
@@ -99,8 +103,9 @@ __global__ void __launch_bounds__(256) k_trig_weights(int n, const double *__res
     if (jp != jq) continue;  // hyperbolic: k_hyp_weights
     const size_t P = p - 1, Q = q - 1;
     double aqp = A[P * lda + Q], apq = A[Q * lda + P], app = A[P * lda + P], aqq = A[Q * lda + Q];
-    bool go = (aqp != 0.0) || (apq != 0.0) || sign_neg(app) || sign_neg(aqq) ||
-              ((jp + jq == -2) ? (aqq < app) : (app < aqq));
+    const bool order = sign_neg(app) || sign_neg(aqq) || ((jp + jq == -2) ? (aqq < app) : (app < aqq));
+    bool go = (aqp != 0.0) || (apq != 0.0) || order;
+    if (tol > 0.0 && go && !order && negligible(aqp, apq, app, aqq, tol)) go = false;
     keys[k] = go ? weight_key(dasum2<F>(aqp, apq)) : KEY_NAN;
     vals[k] = pack_pq(p - 1, q - 1);
   }
@@ -757,10 +762,10 @@ void weights_launch(DeviceStep &d, hipStream_t s) {
   const double *A = (const double *)d.A;
   if (d.fused) {
     hipLaunchKernelGGL(k_trig_weights<true>, dim3(grid_for(d.nn, 256, 65536)), dim3(256), 0, s, d.n, A, d.lda, d.J,
-                       d.sort.k0, d.sort.v0, d.st, d.nn);
+                       d.sort.k0, d.sort.v0, d.st, d.nn, d.wmode ? d.neg_tol : 0.0);
   } else {
     hipLaunchKernelGGL(k_trig_weights<false>, dim3(grid_for(d.nn, 256, 65536)), dim3(256), 0, s, d.n, A, d.lda, d.J,
-                       d.sort.k0, d.sort.v0, d.st, d.nn);
+                       d.sort.k0, d.sort.v0, d.st, d.nn, d.wmode ? d.neg_tol : 0.0);
   }
   if (d.wmode == 1) {
     gram_hyp_weights_launch(d, s);

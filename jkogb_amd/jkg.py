@@ -50,7 +50,7 @@ class MixedOpts(C.Structure):
     """jkg_mixed_opts: options of the mixed solver (no reference counterpart)."""
     _fields_ = [("weights", C.c_int32), ("fp32_warm", C.c_int32), ("switch_tol", C.c_double),
                 ("key_bits", C.c_int32), ("controller", C.c_int32), ("ctl", C.c_void_p),
-                ("threshold", C.c_double), ("reserved", C.c_int32 * 8)]
+                ("threshold", C.c_double), ("tol", C.c_double), ("reserved", C.c_int32 * 8)]
 
 
 W_PAIR, W_GRAM = 0, 1
