@@ -1,26 +1,34 @@
 #!/usr/bin/env python3
-"""Benchmark of the dynamic-ordering HSVD step at N = 4096 on one MI355X.
+"""Benchmark of the HSVD step at N = 4096 on one MI355X: the mixed solver
+against the plain FP64 solver on the same input.
 
-One "step" is one pass of the hot path (weights -> sort -> greedy select ->
-2x2 HSVD -> rotations) on the N = 4096 real input of the C5 configuration
-(Gaussian, column scales 10**U(-6,6), J half +1 / half -1).  The timed
-region starts with Y already in HBM and covers K consecutive steps from the
-start of the solve (device events around the graph launches).
+One "step" is one pass of the hot path (pair weights -> sort -> greedy
+select -> 2x2 HSVD -> rotations) on the N = 4096 real input of the
+learned-controller configuration (Gaussian Y, column scales log-uniform in
+[1e-6, 1e6], J half +1 / half -1, N_2 = 2048).  The timed regions start with
+Y already in HBM and cover K consecutive steps from the start of the solve
+(device events around the graph launches), once per solver:
+  plain  jkg_dsolve: the reference's arithmetic (bit-exact with djk.exe)
+  mixed  jkg_dsolve_mixed: Gram-form hyperbolic weights on the FP64 matrix
+         cores and the FP64 convergence test (tol = DBL_EPSILON)
 
 Printed as one JSON line:
-  value        algorithmic HBM GB/s of the step (bytes defined below / time)
-  ms_per_step  device time per step
-  roofline     the dominant kernel (the hyperbolic pair weights: FP64 VALU)
-  roofline_rotate   the rotation kernel against HBM
+  value        algorithmic HBM GB/s of the mixed solver's step (bytes below)
+  ms_per_step  the mixed solver's device time per step
+  plain        the same for the plain solver; step_speedup = plain / mixed
+  roofline     the dominant stage of the mixed step, against HBM peak
+  rooflines    the Gram product (FP64 MFMA) and the rotations (HBM)
+  full_solve   complete HSVDs at N = 1024 (C2 input) with both solvers: wall
+               time, steps, and the singular values' largest relative
+               difference in units of N eps
   cpu_baseline the reference (oracle/_ref/plain/djk.exe, built from the
                reference's own sources) on the same input, host cores
-  full_solve_c2     a complete HSVD at N = 1024 (C2), wall time on the device
 
 Algorithmic bytes of one step (N, NN = N(N-1)/2, IT finite weights, SL
 pivots, b2/b4 pivots with a row/column transform):
-  (a) weights  8 N^2 (read A) + 8 NN (one key per pair)
+  (a) weights  8 N^2 (read A) + 12 NN (one key and one pair value per pair)
   (b) sort     24 NN (read + write one 8-byte key and 4-byte value)
-  (c) select   12 IT
+  (c) select   12 IT (walk the sorted values once)
   (d) 2x2      112 SL
   (e) rotate   16 N^2 (A read + written once) + 32 N (b2 + b4) (U, Z columns)
 """
@@ -44,7 +52,7 @@ FP64_PEAK_TFLOPS = 78.6
 
 def step_bytes(n, it, sl, b2, b4):
     nn = n * (n - 1) // 2
-    a = 8.0 * n * n + 8.0 * nn
+    a = 8.0 * n * n + 12.0 * nn
     b = 24.0 * nn
     c = 12.0 * it
     d = 112.0 * sl
@@ -92,6 +100,21 @@ def cpu_reference_sample(y, j, n2, budget_s, threads):
     return steps
 
 
+def timed_steps(ctx, solve, y, j, n2, steps, warmup, **kw):
+    """K steps of one solver from the start (after W warmup steps that capture the step graph)."""
+    solve(y, j, n2, max_steps=warmup, **kw)
+    r = solve(y, j, n2, max_steps=steps, **kw)
+    k = max(r["steps"], 1)
+    ms = 1e3 * r["seconds"] / k
+    b, _ = step_bytes(y.shape[0], r["sum_it"] / k, r["sum_sl"] / k, r["sum_b2"] / k, r["sum_b4"] / k)
+    return dict(steps=k, ms_per_step=ms, gbs=b / (ms * 1e-3) / 1e9, bytes=b)
+
+
+def stage_profile(prof):
+    ps = max(prof["steps"], 1)
+    return {s: v / ps for s, v in prof["stage_ms"].items()}, ps
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,31 +137,39 @@ def main():
     y, j = inputs.make("c5", n=n, seed=0)
     n2 = n // 2
     ctx = jkg.Context(0)
-    # warmup: captures the step graph and touches every buffer
-    ctx.dsolve(y, j, n2, max_steps=args.warmup, fused=args.fused)
     t0 = time.time()
-    r = ctx.dsolve(y, j, n2, max_steps=args.steps, fused=args.fused)
+    mixed = timed_steps(ctx, ctx.dsolve_mixed, y, j, n2, args.steps, args.warmup, fused=args.fused)
+    plain = timed_steps(ctx, ctx.dsolve, y, j, n2, args.steps, args.warmup, fused=args.fused)
     wall = time.time() - t0
-    k = r["steps"] if r["info"] == args.steps else r["steps"]
-    ms_step = 1e3 * r["seconds"] / max(k, 1)
-    bytes_step, rot_bytes = step_bytes(n, r["sum_it"] / k, r["sum_sl"] / k, r["sum_b2"] / k, r["sum_b4"] / k)
-    gbs = bytes_step / (ms_step * 1e-3) / 1e9
 
-    # per-stage device times (each stage launched and timed on the library's stream)
-    prof = ctx.dprofile(y, j, n2, steps=args.profile_steps, fused=args.fused)
-    ps = prof["steps"]
-    st_ms = {s: v / ps for s, v in prof["stage_ms"].items()}
-    flops_w = 14.0 * (n - 2) * prof["sum_abod"] / ps
-    tflops_w = flops_w / (st_ms["weights"] * 1e-3) / 1e12
-    _, rot_b = step_bytes(n, 0, 0, prof["sum_b2"] / ps, prof["sum_b4"] / ps)
-    rot_gbs = rot_b / (st_ms["rotate"] * 1e-3) / 1e9
+    # per-stage device times (each stage launched and timed with HIP events on the library's stream)
+    pm, psm = stage_profile(ctx.dprofile_mixed(y, j, n2, steps=args.profile_steps, fused=args.fused))
+    pp, _ = stage_profile(ctx.dprofile(y, j, n2, steps=args.profile_steps, fused=args.fused))
+    prof_m = ctx.dprofile_mixed(y, j, n2, steps=1, fused=args.fused)
+    it1 = prof_m["sum_it"]
+    nn = n * (n - 1) // 2
+    npl = int(np.sum(np.asarray(j) > 0))
+    nmi = n - npl
+    # the dominant stage of the mixed step
+    dom = max(pm, key=lambda k: pm[k])
+    stage_bytes = {"weights": 8.0 * n * n + 12.0 * nn, "sort": 24.0 * nn, "select": 12.0 * it1,
+                   "rotate": step_bytes(n, 0, 0, prof_m["sum_b2"], prof_m["sum_b4"])[1]}
+    dom_gbs = stage_bytes.get(dom, 0.0) / (pm[dom] * 1e-3) / 1e9
+    # Gram product: 2 npl nmi N flops inside the weights stage (timed on its own in the rocprof summary)
+    gram_flops = 2.0 * npl * nmi * n
+    rot_gbs = stage_bytes["rotate"] / (pm["rotate"] * 1e-3) / 1e9
 
     full = None
     if not args.no_full:
         yc, jc = inputs.make("c2", n=1024, seed=0)
-        rc = ctx.dsolve(yc, jc, 512, fused=args.fused)
-        full = {"workload": "C2: N=1024 real, columns 10**U(-4,4), J balanced, N_2=512", "steps": rc["info"],
-                "seconds": round(rc["seconds"], 4), "ms_per_step": round(1e3 * rc["seconds"] / max(rc["steps"], 1), 4)}
+        rp = ctx.dsolve(yc, jc, 512, fused=args.fused)
+        rm = ctx.dsolve_mixed(yc, jc, 512, fused=args.fused)
+        err = float(np.max(np.abs(rm["SY"] - rp["SY"]) / np.abs(rp["SY"])) / (1024 * np.finfo(float).eps))
+        full = {"workload": "C2: N=1024 real, columns 10**U(-4,4), J balanced, N_2=512",
+                "plain": {"steps": rp["steps"], "seconds": round(rp["seconds"], 4)},
+                "mixed": {"steps": rm["steps"], "seconds": round(rm["seconds"], 4)},
+                "speedup": round(rp["seconds"] / rm["seconds"], 3),
+                "sy_max_rel_diff_N_eps": round(err, 4)}
     ctx.close()
 
     cpu = None
@@ -147,10 +178,11 @@ def main():
         steps = cpu_reference_sample(y, j, n2, args.cpu_budget, threads)
         if steps:
             per = float(np.mean([s[2] for s in steps]))
-            cpu = {"value": round(bytes_step / per / 1e9, 4), "unit": "GB/s", "cores": threads, "kind": "reference",
+            cpu = {"value": round(plain["bytes"] / per / 1e9, 4), "unit": "GB/s", "cores": threads,
+                   "kind": "reference",
                    "sample": "the reference djk.exe (flang build of its own sources) on the same N=4096 input, "
-                             "%d completed step(s) within %.0f s; %.3f s per step (its BUILDs+TRANSFs CSV)"
-                             % (len(steps), args.cpu_budget, per),
+                             "%d completed step(s) within %.0f s; %.3f s per step (its BUILDs+TRANSFs CSV), "
+                             "priced with the plain step's algorithmic bytes" % (len(steps), args.cpu_budget, per),
                    "s_per_step": round(per, 4)}
         else:
             cpu = {"value": None, "unit": "GB/s", "cores": threads, "kind": "reference",
@@ -158,31 +190,41 @@ def main():
 
     out = {
         "metric": "HSVD wall time + HBM GB/s at N=4096, mixed/learned vs plain FP64, same accuracy",
-        "value": round(gbs, 3),
+        "value": round(mixed["gbs"], 3),
         "unit": "GB/s",
         "n_gpus": 1,
-        "steps": k,
+        "steps": mixed["steps"],
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 4),
+        "ms_per_step": round(mixed["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "C5 input at N=4096: real Gaussian Y, column scales 10**U(-6,6), J 2048 (+1) / "
-                               "2048 (-1) permuted, N_2=2048; plain FP64 solver (faithful weights, %s FMAD)"
-                               % ("fused" if args.fused else "unfused"),
-                   "n": n, "n2": n2, "solver": "plain-fp64"},
-        "roofline": {"bound": "mfma", "kernel": "k_hyp_weights (FP64 VALU, ABODNDF2 per pair)",
-                     "achieved": round(tflops_w, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(tflops_w / FP64_PEAK_TFLOPS, 4), "traffic": None},
-        "roofline_rotate": {"bound": "hbm", "kernel": "k_rotate", "achieved": round(rot_gbs, 1),
-                            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(rot_gbs / HBM_PEAK_GBS, 4),
-                            "traffic": None, "bytes_per_launch": int(rot_b),
-                            "ms_per_launch": round(st_ms["rotate"], 4)},
-        "stage_ms_per_step": {s: round(v, 4) for s, v in st_ms.items()},
+        "config": {"workload": "N=4096 real Gaussian Y, column scales log-uniform in [1e-6,1e6], J 2048 (+1) / "
+                               "2048 (-1) permuted, N_2=2048; mixed solver (Gram-form FP64 weights, tol=eps, %s "
+                               "FMAD) vs the plain FP64 solver" % ("fused" if args.fused else "unfused"),
+                   "n": n, "n2": n2, "solver": "mixed-fp64"},
+        "plain": {"ms_per_step": round(plain["ms_per_step"], 4), "gbs": round(plain["gbs"], 3),
+                  "steps": plain["steps"]},
+        "step_speedup": round(plain["ms_per_step"] / mixed["ms_per_step"], 3),
+        "roofline": {"bound": "hbm", "kernel": "%s stage of the mixed step" % dom,
+                     "achieved": round(dom_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(dom_gbs / HBM_PEAK_GBS, 5), "traffic": None,
+                     "bytes_per_launch": int(stage_bytes.get(dom, 0.0)), "ms_per_launch": round(pm[dom], 4)},
+        "rooflines": [
+            {"bound": "mfma", "kernel": "weights stage (k_colnorm_off + k_gemm_tn + k_hyp_weights_gram)",
+             "achieved": round(gram_flops / (pm["weights"] * 1e-3) / 1e12, 3), "peak": FP64_PEAK_TFLOPS,
+             "unit": "TFLOP/s", "frac": round(gram_flops / (pm["weights"] * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4),
+             "flops_per_launch": gram_flops, "ms_per_launch": round(pm["weights"], 4)},
+            {"bound": "hbm", "kernel": "k_rotate", "achieved": round(rot_gbs, 1), "peak": HBM_PEAK_GBS,
+             "unit": "GB/s", "frac": round(rot_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+             "bytes_per_launch": int(stage_bytes["rotate"]), "ms_per_launch": round(pm["rotate"], 4)},
+        ],
+        "stage_ms_per_step": {"mixed": {s: round(v, 4) for s, v in pm.items()},
+                              "plain": {s: round(v, 4) for s, v in pp.items()}},
         "cpu_baseline": cpu,
-        "full_solve_c2": full,
+        "full_solve": full,
         "host_wall_s": round(wall, 3),
     }
     print(json.dumps(out))

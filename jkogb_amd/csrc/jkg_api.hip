@@ -163,7 +163,7 @@ static int ensure_mixed(jkg_ctx *c) {
   const size_t n = (size_t)d.n;
   void *g = nullptr, *cn = nullptr;
   if (hipMalloc(&g, sizeof(double) * (n / 2) * (n - n / 2)) != hipSuccess ||
-      hipMalloc(&cn, sizeof(double) * n) != hipSuccess) {
+      hipMalloc(&cn, sizeof(double) * 2 * n) != hipSuccess) {
     (void)hipGetLastError();
     if (g) (void)hipFree(g);
     return JKG_ENOMEM;
@@ -353,11 +353,20 @@ static int run_loop(jkg_ctx *c, const jkg_opts *o, jkg_result *res) {
   // The reference needs about 10 N steps; a run that has not ended after
   // 1000 N + 1000 steps is stopped and reported (a guard, never reached by a
   // converging run).
-  const long cap = (o->max_steps >= 0) ? (long)o->max_steps + 2 : 1000L * d.n + 1000;
-  long launched = 0;
+  // With a step limit K, exactly K step graphs are launched and then the
+  // step-begin kernel alone, which records the end (no idle graphs).
+  const long cap = 1000L * d.n + 1000;
+  long launched = 0, remaining = (o->max_steps >= 0) ? (long)o->max_steps : -1;
   for (;;) {
-    for (int k = 0; k < batch; ++k) HIPCHK(hipGraphLaunch(c->exec, c->stream));
-    launched += batch;
+    const long nb = (remaining >= 0 && remaining < batch) ? remaining : batch;
+    for (long k = 0; k < nb; ++k) HIPCHK(hipGraphLaunch(c->exec, c->stream));
+    launched += nb;
+    if (remaining >= 0) remaining -= nb;
+    if (remaining == 0) {
+      step_begin_launch(d, c->stream);
+      HIPCHK(hipGetLastError());
+      break;
+    }
     int done = 0;
     HIPCHK(hipMemcpyAsync(&done, &d.st->done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));

@@ -121,17 +121,19 @@ void gemm_tn_launch(int k, const double *X, int ldx, const int *xcols, int m, co
 }
 
 // ------------------------------------------------------------------ column norms
-// c[col] = sum_{r != col} A[r, col]^2, one wave per column.
+// c[col] = sum_{r != col} A[r, col]^2 and dg[col] = A[col, col], one wave per column.
 __global__ void __launch_bounds__(256) k_colnorm_off(int n, const double *__restrict__ A, int lda,
-                                                     double *__restrict__ c, const StepState *st) {
+                                                     double *__restrict__ c, double *__restrict__ dg,
+                                                     const StepState *st) {
   if (!st->active) return;
   const int col = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (col >= n) return;
   const double *a = A + (size_t)col * lda;
   double s = 0.0;
   for (int r = lane; r < n; r += 64) {
-    const double v = (r == col) ? 0.0 : a[r];
-    s = __builtin_fma(v, v, s);
+    const double v = a[r];
+    if (r == col) dg[col] = v;
+    else s = __builtin_fma(v, v, s);
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
@@ -139,27 +141,47 @@ __global__ void __launch_bounds__(256) k_colnorm_off(int n, const double *__rest
 }
 
 // ------------------------------------------------------------------ weights
-// One lane per hyperbolic pair (plus index i, minus index j).  The branch
-// structure is DMAGF2H's; only the ABODNDF2 sum is replaced by the quadratic
-// form above.
+// One workgroup per 64 (J = +1) x 64 (J = -1) tile of hyperbolic pairs.  The
+// two off-diagonal blocks A(minus, plus) and A(plus, minus) of the tile are
+// gathered through LDS so that every global read runs down a column; the
+// diagonal comes from dg.  The branch structure is DMAGF2H's; only the
+// ABODNDF2 sum is replaced by the quadratic form above.
+constexpr int WT = 64;
+
 template <bool F>
 __global__ void __launch_bounds__(256) k_hyp_weights_gram(int n, const double *__restrict__ A, int lda,
                                                           const int *__restrict__ J, const int *__restrict__ plus,
                                                           int npl, const int *__restrict__ minus, int nmi,
                                                           const double *__restrict__ G, int ldg,
                                                           const double *__restrict__ coln,
+                                                          const double *__restrict__ dg,
                                                           uint64_t *__restrict__ keys, uint32_t *__restrict__ vals,
                                                           StepState *st, double tol) {
   if (!st->active) return;
-  const int j = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
-  const bool valid = i < npl && j < nmi;
-  bool full = false;
-  if (valid) {
-    const int cp = plus[i], cm = minus[j];
+  __shared__ double E1[WT][WT + 1];  // E1[a][b] = A(minus b, plus a)
+  __shared__ double E2[WT][WT + 1];  // E2[b][a] = A(plus a, minus b)
+  __shared__ int pc[WT], mc[WT];
+  const int i0 = blockIdx.y * WT, j0 = blockIdx.x * WT, tid = threadIdx.x;
+  if (tid < WT) pc[tid] = (i0 + tid < npl) ? plus[i0 + tid] : -1;
+  else if (tid < 2 * WT) mc[tid - WT] = (j0 + tid - WT < nmi) ? minus[j0 + tid - WT] : -1;
+  __syncthreads();
+  for (int e = tid; e < WT * WT; e += 256) {
+    const int hi = e / WT, lo = e % WT;
+    const int c1 = pc[hi], r1 = mc[lo];  // column plus hi, row minus lo
+    E1[hi][lo] = (c1 >= 0 && r1 >= 0) ? A[(size_t)c1 * lda + r1] : 0.0;
+    const int c2 = mc[hi], r2 = pc[lo];  // column minus hi, row plus lo
+    E2[hi][lo] = (c2 >= 0 && r2 >= 0) ? A[(size_t)c2 * lda + r2] : 0.0;
+  }
+  __syncthreads();
+  unsigned nfull = 0;
+  for (int e = tid; e < WT * WT; e += 256) {
+    const int a = e / WT, b = e % WT;
+    const int cp = pc[a], cm = mc[b];
+    if (cp < 0 || cm < 0) continue;
     const int p0 = cp < cm ? cp : cm, q0 = cp < cm ? cm : cp;
-    const size_t P = p0, Q = q0;
-    const double app = A[P * lda + P], aqp = A[P * lda + Q], apq = A[Q * lda + P], aqq = A[Q * lda + Q];
+    const double e1 = E1[a][b], e2 = E2[b][a];  // A(cm, cp), A(cp, cm)
+    const double aqp = cp < cm ? e1 : e2, apq = cp < cm ? e2 : e1;
+    const double app = dg[p0], aqq = dg[q0];
     uint64_t key = KEY_NAN;
     const bool order = sign_neg(app) || sign_neg(aqq);
     const bool drop = tol > 0.0 && !order && negligible(aqp, apq, app, aqq, tol);
@@ -171,11 +193,12 @@ __global__ void __launch_bounds__(256) k_hyp_weights_gram(int n, const double *_
       else if ((info & 1) == 0) key = weight_key(0.0);
       else if ((info & 4) == 0) key = weight_key(dasum2<F>(aqp, apq));
       else {
-        full = true;
+        ++nfull;
         const double aq = (info & 8) ? a2[1] : 0.0, ap = (info & 8) ? a2[2] : 0.0;
         double d = fmad<F>(aqp - aq, aqp + aq, 0.0);
         d = fmad<F>(apq - ap, apq + ap, d);
-        const double gxx = coln[p0] - aqp * aqp, gyy = coln[q0] - apq * apq, gxy = G[(size_t)i * ldg + j];
+        const double gxx = coln[p0] - aqp * aqp, gyy = coln[q0] - apq * apq;
+        const double gxy = G[(size_t)(i0 + a) * ldg + j0 + b];
         const double b11 = z2[0], b21 = z2[1], b12 = z2[2], b22 = z2[3];
         const double w = d + gxx * (1.0 - b11 * b11 - b12 * b12) + gyy * (1.0 - b21 * b21 - b22 * b22) -
                          2.0 * gxy * (b11 * b21 + b12 * b22);
@@ -186,22 +209,24 @@ __global__ void __launch_bounds__(256) k_hyp_weights_gram(int n, const double *_
     keys[k] = key;
     vals[k] = pack_pq(p0, q0);
   }
-  const uint64_t b = __ballot(full);
-  if ((threadIdx.x & 63) == 0 && b) atomicAdd((unsigned long long *)&st->acc_abod, (unsigned long long)__popcll(b));
+  // work counter: pairs whose weight needed the column sums
+  for (int o = 32; o > 0; o >>= 1) nfull += __shfl_xor(nfull, o);
+  if ((tid & 63) == 0 && nfull) atomicAdd((unsigned long long *)&st->acc_abod, (unsigned long long)nfull);
 }
 
 void gram_hyp_weights_launch(DeviceStep &d, hipStream_t s) {
   if (d.npl <= 0 || d.nmi <= 0) return;
   const double *A = (const double *)d.A;
-  hipLaunchKernelGGL(k_colnorm_off, dim3((d.n + 3) / 4), dim3(256), 0, s, d.n, A, d.lda, d.coln, d.st);
+  hipLaunchKernelGGL(k_colnorm_off, dim3((d.n + 3) / 4), dim3(256), 0, s, d.n, A, d.lda, d.coln, d.coln + d.n,
+                     d.st);
   gemm_tn_launch(d.n, A, d.lda, d.plus, d.npl, A, d.lda, d.minus, d.nmi, d.gram, (long)d.nmi, 1L, true, d.st, s);
-  dim3 g((d.nmi + 63) / 64, (d.npl + 3) / 4);
+  dim3 g((d.nmi + WT - 1) / WT, (d.npl + WT - 1) / WT);
   if (d.fused)
     hipLaunchKernelGGL(k_hyp_weights_gram<true>, g, dim3(256), 0, s, d.n, A, d.lda, d.J, d.plus, d.npl, d.minus,
-                       d.nmi, d.gram, d.nmi, d.coln, d.sort.k0, d.sort.v0, d.st, d.neg_tol);
+                       d.nmi, d.gram, d.nmi, d.coln, d.coln + d.n, d.sort.k0, d.sort.v0, d.st, d.neg_tol);
   else
     hipLaunchKernelGGL(k_hyp_weights_gram<false>, g, dim3(256), 0, s, d.n, A, d.lda, d.J, d.plus, d.npl, d.minus,
-                       d.nmi, d.gram, d.nmi, d.coln, d.sort.k0, d.sort.v0, d.st, d.neg_tol);
+                       d.nmi, d.gram, d.nmi, d.coln, d.coln + d.n, d.sort.k0, d.sort.v0, d.st, d.neg_tol);
 }
 
 }  // namespace jkg

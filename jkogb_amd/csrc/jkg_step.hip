@@ -682,14 +682,30 @@ __global__ void __launch_bounds__(256) k_step_end(const int *__restrict__ J, con
     atomicAdd(&cnt[1], nt);
     atomicAdd(&cnt[2], nh);
   }
+  // TW: the weights of the pivots with a transform, summed in pivot order from
+  // -0 (one lane, as the reference's loop), staged through LDS in chunks; a
+  // NaN marks a pivot without a transform (pivot weights are finite).
+  __shared__ double wbuf[1024];
+  double tw = -0.0;
+  if (err == 0)
+    for (int c0 = 0; c0 < sl; c0 += 1024) {
+      __syncthreads();
+      for (int i = threadIdx.x; i < 1024 && c0 + i < sl; i += blockDim.x) {
+        const int l = piv_info[c0 + i];
+        wbuf[i] = ((l & 2) || (l & 4)) ? piv_w[c0 + i] : __builtin_nan("");
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        const int m = (sl - c0 < 1024) ? sl - c0 : 1024;
+#pragma unroll 8
+        for (int i = 0; i < m; ++i) {
+          const double w = wbuf[i];
+          tw = (w == w) ? tw + w : tw;
+        }
+      }
+    }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double tw = -0.0;
-    if (err == 0)
-      for (int i = 0; i < sl; ++i) {
-        const int l = piv_info[i];
-        if ((l & 2) || (l & 4)) tw = tw + piv_w[i];
-      }
     const int nl = (err == 0) ? cnt[0] : err;
     rec.newlen = nl;
     rec.tw = tw;

@@ -86,7 +86,7 @@ struct DeviceStep {
   // mixed solver
   int wmode;              // hyperbolic weights: 0 per pair (ABODNDF2), 1 Gram form
   double *gram;           // npl x nmi: (O^T O)(plus, minus), O = A with a zero diagonal
-  double *coln;           // N: off-diagonal column norms^2
+  double *coln;           // 2N: off-diagonal column norms^2, then the diagonal of A
   double neg_tol;         // > 0: negligible pairs are dropped (mixed solver's convergence test)
 };
 
