@@ -295,7 +295,7 @@ double jkg_key_weight(uint64_t key) {
 
 // ---------------------------------------------------------------- problem setup
 static int upload_problem(jkg_ctx *c, int n, int n2, const void *y, const int64_t *j, int *npos, int *nneg,
-                          int max_steps, int trace_cap) {
+                          int max_steps, int trace_cap, double tau = 0.0) {
   // (the element size follows the buffers: c->d.cplx)
   DeviceStep &d = c->d;
   std::vector<int> J32(n), pl, mi;
@@ -317,6 +317,7 @@ static int upload_problem(jkg_ctx *c, int n, int n2, const void *y, const int64_
   s.max_steps = max_steps;
   s.n2 = n2;
   s.trace_cap = trace_cap;
+  s.tau = tau;
   HIPCHK(hipMemcpyAsync(d.st, &s, sizeof(s), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));  // the host vectors go out of scope
   return JKG_OK;
@@ -395,7 +396,8 @@ static int run_loop(jkg_ctx *c, const jkg_opts *o, jkg_result *res) {
 }
 
 static int solve_common(jkg_ctx *c, int cplx, int n, int n2, const void *y, const int64_t *j, void *yu, void *zz,
-                        double *sy, const jkg_opts *opts, jkg_result *res, int wmode = 0, double tol = 0.0) {
+                        double *sy, const jkg_opts *opts, jkg_result *res, int wmode = 0, double tol = 0.0,
+                        double tau = 0.0) {
   if (!c) return -1;
   if (n < 3) return -2;
   if (!y) return -4;
@@ -426,7 +428,7 @@ static int solve_common(jkg_ctx *c, int cplx, int n, int n2, const void *y, cons
   }
   // TT: the number of trigonometric pairs (the reference's R%TRU result)
   int npos = 0, nneg = 0;
-  rc = upload_problem(c, n, n2, y, j, &npos, &nneg, o.max_steps, c->trace_cap);
+  rc = upload_problem(c, n, n2, y, j, &npos, &nneg, o.max_steps, c->trace_cap, tau);
   if (rc) return rc;
   res->npos = npos;
   res->nneg = nneg;
@@ -467,10 +469,11 @@ extern "C" int jkg_dsolve_mixed(jkg_ctx *c, int n, int n2, const double *y, cons
   if (mopts) m = *mopts;
   else jkg_mixed_opts_default(&m);
   if (m.weights != JKG_W_PAIR && m.weights != JKG_W_GRAM) return -10;
-  if (!(m.tol >= 0.0)) return -10;
-  if (m.fp32_warm != 0 || m.key_bits != 64 || m.controller != 0 || m.threshold != 0.0) return JKG_ENOTSUP;
+  if (!(m.tol >= 0.0) || !(m.threshold >= 0.0 && m.threshold < 1.0)) return -10;
+  if (m.fp32_warm != 0 || m.key_bits != 64 || m.controller != 0) return JKG_ENOTSUP;
   if (m.weights == JKG_W_PAIR && m.tol != 0.0) return JKG_ENOTSUP;
-  return solve_common(c, 0, n, n2, y, j, yu, zz, sy, opts, res, m.weights == JKG_W_GRAM ? 1 : 0, m.tol);
+  return solve_common(c, 0, n, n2, y, j, yu, zz, sy, opts, res, m.weights == JKG_W_GRAM ? 1 : 0, m.tol,
+                      m.threshold);
 }
 
 extern "C" int jkg_zsolve(jkg_ctx *c, int n, int n2, const double *y, const int64_t *j, double *yu, double *zz,

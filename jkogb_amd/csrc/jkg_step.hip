@@ -268,8 +268,23 @@ __global__ void __launch_bounds__(SEL_THREADS_RES) k_sel_resolve(int n, const ui
         if (keys[mid] == KEY_NAN) hi = mid;
         else lo = mid + 1;
       }
-      itv = lo;
       st->it = lo;
+      // the mixed solver's threshold: walk only the weights >= tau * (largest weight)
+      const double tau = st->tau;
+      if (tau > 0.0 && lo > 0) {
+        const double wmax = key_weight(keys[0]);
+        if (wmax > 0.0) {
+          const uint64_t kt = weight_key(tau * wmax);  // keys above kt stand for smaller weights
+          int a = 0, b = lo;
+          while (a < b) {
+            const int mid = (a + b) >> 1;
+            if (keys[mid] > kt) b = mid;
+            else a = mid + 1;
+          }
+          lo = a;
+        }
+      }
+      itv = lo;
       st->list_len = lo;
     }
     for (int i = tid; i <= n; i += SEL_THREADS_RES) used[i] = (i == n);

@@ -54,6 +54,7 @@ struct StepState {
   int nrec;       // trace records written (last step index + 1)
   int ntrig, nhyp;
   double tw;
+  double tau;     // > 0: the walk stops at the first weight below tau * (largest weight) (mixed solver)
   // work counters summed over the steps (bytes and flops of the run)
   unsigned long long acc_sl, acc_it, acc_b2, acc_b4, acc_abod;
 };

@@ -151,3 +151,30 @@ def test_mixed_graph_follows_options(ctx):
     c = ctx.dsolve_mixed(Y, J, 48, tol=0.0)
     assert b["steps"] < a["steps"]
     assert a["steps"] == c["steps"] and np.array_equal(a["SY"], c["SY"])
+
+
+@pytest.mark.parametrize("cfg,n", [("c1", 128), ("c2", 160)])
+def test_mixed_threshold(ctx, cfg, n):
+    # a walk cut at tau * (largest weight) changes the trajectory; the result stays within the spread
+    # between two valid builds of the reference (unfused and fused FMAD) plus 64 N eps
+    Y, J = inputs(cfg, n, 6)
+    plain = ctx.dsolve(Y, J, n // 2)
+    fused = ctx.dsolve(Y, J, n // 2, fused=True)
+    base = ctx.dsolve_mixed(Y, J, n // 2)
+    thr = ctx.dsolve_mixed(Y, J, n // 2, threshold=1e-6)
+    assert thr["info"] >= 0
+    assert thr["sum_it"] != base["sum_it"] or thr["steps"] != base["steps"]
+    spread = np.max(np.abs(fused["SY"] - plain["SY"]) / np.abs(plain["SY"]))
+    rel = np.max(np.abs(thr["SY"] - plain["SY"]) / np.abs(plain["SY"]))
+    assert rel <= 4 * spread + 64 * n * EPS, (rel, spread)
+    rec, orth, jorth = residuals(Y, J, thr)
+    prec, porth, pjorth = residuals(Y, J, plain)
+    assert rec <= 4 * prec + 2 * n * EPS and orth <= 4 * porth + 2 * n * EPS, (rec, orth)
+
+
+def test_mixed_threshold_rejects_out_of_range(ctx):
+    from jkogb_amd import jkg
+    Y, J = inputs("c1", 16, 1)
+    for t in (-1.0, 1.0, float("nan")):
+        with pytest.raises(jkg.JKGError):
+            ctx.dsolve_mixed(Y, J, 8, threshold=t)
