@@ -153,14 +153,45 @@ typedef struct {
   int32_t fp32_warm;  /* 1: FP32 steps until the off-norm stalls or falls below switch_tol, then FP64 */
   double switch_tol;  /* relative off-norm that ends the FP32 phase (default 1e-5) */
   int32_t key_bits;   /* 64: full sort keys (default); 32: the top 32 bits, ties by pair index */
-  int32_t controller; /* 0: N_2 and no threshold every step; 1: learned controller (ctl) */
+  int32_t controller; /* step controller: 0 none (N_2 and `threshold` every step); 1 the MLP, ctl = its
+                         JKG_CTL_NPARAM parameters; 2 a phase table, ctl = 8 floats (see below) */
   const float *ctl;   /* controller parameters, or NULL */
   double threshold;   /* fixed threshold: weights below threshold * (largest weight) dropped; 0: none */
   double tol;         /* convergence test: a pair with max(|a_pq|,|a_qp|) <= tol * sqrt(|a_pp| |a_qq|) whose
                          transform would otherwise be the identity is dropped (default: DBL_EPSILON);
                          0: the reference's test only */
-  int32_t reserved[8];
+  float *ctl_trace;   /* per-step controller records (JKG_CTL_REC floats each), or NULL */
+  int32_t ctl_trace_cap;
+  int32_t reserved[7];
 } jkg_mixed_opts;
+
+/* The step controller.  At the start of each step it sets that step's N_2 (a
+ * fraction in [0.25, 1] of the problem's N_2) and threshold tau (log10 tau in
+ * [-16, -4]; -16 means none) from 16 features of the previous step:
+ *   0 step / 16N, 1 log10(TW/TW_0)/20, 2 log10(TW/TW_prev)/2, 3 SL/N_2, 4 NL/SL,
+ *   5 IT/NN, 6 last chosen position / IT, 7 trigonometric share, 8 log10(w_max/|Y|^2)/32,
+ *   9 log2(N)/16, 10 N_2 fraction used, 11 (log10 tau used + 16)/12 (0: none),
+ *   12 log10(TW/|Y|^2)/32, 13 smoothed feature 2, 14 SL reached N_2, 15 1.
+ * MLP (16-64-64-2, ReLU): W1[64][16], b1[64], W2[64][64], b2[64], W3[2][64], b3[2] as
+ * FP32; the device rounds the weights to BF16 and accumulates in FP32.  Outputs
+ * y clamped to [0, 1]: N_2 fraction = 0.25 + 0.75 y0, log10 tau = -16 + 12 y1.
+ * Phase table t[8]: phase 0 while log10(TW/TW_0) > t[0], phase 1 while > t[1],
+ * then phase 2; actions (N_2 fraction, log10 tau) in t[2..7].
+ * Fallback (both): the plain full ordering on the first step and once TW has
+ * fallen below 1e-10 TW_0.  A trace record holds the 16 features after the
+ * step, the step's N_2 fraction and log10 tau, its device time (us) and TW. */
+#define JKG_CTL_NPARAM 5378
+#define JKG_CTL_REC 20
+/* jkg_ctl_grad: the controller MLP's loss, sum_s w_s (y_s - t_s)^2 / (2 sum w), over ns
+ * samples (x: ns x 16, t: ns x 2, w: ns or NULL) and its gradient, with the library's
+ * forward and backward kernels; y (ns x 2) gets the outputs if not NULL. */
+int jkg_ctl_grad(jkg_ctx *ctx, int ns, const float *x, const float *t, const float *w, const float *params,
+                 float *grad, float *loss, float *y);
+/* jkg_ctl_fit: trains params (in: initial, out: trained) with minibatch Adam on the
+ * GPU; batches follow a permutation drawn from seed each epoch; loss[e] is the
+ * full-set loss after epoch e. */
+int jkg_ctl_fit(jkg_ctx *ctx, int ns, const float *x, const float *t, const float *w, int epochs, int batch,
+                float lr, float weight_decay, uint64_t seed, float *params, float *loss);
 void jkg_mixed_opts_default(jkg_mixed_opts *m);
 int jkg_dsolve_mixed(jkg_ctx *ctx, int n, int n2, const double *y, const int64_t *j, double *yu, double *zz,
                      double *sy, const jkg_opts *opts, const jkg_mixed_opts *m, jkg_result *res);

@@ -13,6 +13,7 @@
 #include "../../include/jkg.h"
 #include "jkg_hsvd2.hpp"
 #include "jkg_step.hpp"
+#include "jkg_mlp.hpp"
 
 using namespace jkg;
 
@@ -42,15 +43,22 @@ struct jkg_ctx {
   hipGraphExec_t exec = nullptr;
   // host values the captured graph holds as launch shapes or kernel arguments
   struct GraphKey {
-    int fused = -1, wmode = -1, npl = -1;
+    int fused = -1, wmode = -1, npl = -1, ctl_mode = -1, ctl_cap = -1;
     double neg_tol = -1.0;
     bool operator==(const GraphKey &o) const {
-      return fused == o.fused && wmode == o.wmode && npl == o.npl && neg_tol == o.neg_tol;
+      return fused == o.fused && wmode == o.wmode && npl == o.npl && neg_tol == o.neg_tol &&
+             ctl_mode == o.ctl_mode && ctl_cap == o.ctl_cap;
     }
   } gkey;
   // mixed solver buffers (allocated on first use)
   std::vector<void *> mallocs;
   int mixed_n = 0;
+  // step controller and its training (allocated on first use, kept across problems)
+  float *ctl_master = nullptr, *ctl_table = nullptr, *ctl_trace = nullptr;
+  uint16_t *ctl_q = nullptr;
+  int ctl_trace_cap = 0;
+  float *tr_buf = nullptr;  // training scratch
+  size_t tr_floats = 0;
   // misc test buffers
   void *tmp = nullptr;
   size_t tmp_bytes = 0;
@@ -176,6 +184,49 @@ static int ensure_mixed(jkg_ctx *c) {
   return JKG_OK;
 }
 
+// Controller parameters (FP32 master + BF16 copy), phase table and per-step trace.
+static int ensure_ctl(jkg_ctx *c, int trace_cap) {
+  auto grab = [&](void **p, size_t bytes) -> int {
+    if (*p) return JKG_OK;
+    if (hipMalloc(p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      *p = nullptr;
+      return JKG_ENOMEM;
+    }
+    return JKG_OK;
+  };
+  int rc = grab((void **)&c->ctl_master, sizeof(float) * MLP_NPARAM);
+  if (!rc) rc = grab((void **)&c->ctl_q, sizeof(uint16_t) * MLP_NPARAM);
+  if (!rc) rc = grab((void **)&c->ctl_table, sizeof(float) * 8);
+  if (rc) return rc;
+  if (trace_cap > c->ctl_trace_cap) {
+    if (c->ctl_trace) (void)hipFree(c->ctl_trace);
+    c->ctl_trace = nullptr;
+    c->ctl_trace_cap = 0;
+    if (hipMalloc(&c->ctl_trace, sizeof(float) * CTL_REC * (size_t)trace_cap) != hipSuccess) {
+      (void)hipGetLastError();
+      c->ctl_trace = nullptr;
+      return JKG_ENOMEM;
+    }
+    c->ctl_trace_cap = trace_cap;
+  }
+  return JKG_OK;
+}
+
+// Training scratch of at least `floats` floats.
+static int ensure_train(jkg_ctx *c, size_t floats) {
+  if (c->tr_floats >= floats) return JKG_OK;
+  if (c->tr_buf) (void)hipFree(c->tr_buf);
+  c->tr_buf = nullptr;
+  c->tr_floats = 0;
+  if (hipMalloc(&c->tr_buf, sizeof(float) * floats) != hipSuccess) {
+    (void)hipGetLastError();
+    return JKG_ENOMEM;
+  }
+  c->tr_floats = floats;
+  return JKG_OK;
+}
+
 static int tmp_bytes(jkg_ctx *c, size_t bytes) {
   if (c->tmp_bytes >= bytes) return JKG_OK;
   if (c->tmp) (void)hipFree(c->tmp);
@@ -235,6 +286,9 @@ void jkg_destroy(jkg_ctx *c) {
   (void)hipStreamSynchronize(c->stream);
   free_all(c);
   if (c->tmp) (void)hipFree(c->tmp);
+  for (void *p : {(void *)c->ctl_master, (void *)c->ctl_q, (void *)c->ctl_table, (void *)c->ctl_trace,
+                  (void *)c->tr_buf})
+    if (p) (void)hipFree(p);
   (void)hipEventDestroy(c->ev0);
   (void)hipEventDestroy(c->ev1);
   (void)hipStreamDestroy(c->stream);
@@ -318,6 +372,16 @@ static int upload_problem(jkg_ctx *c, int n, int n2, const void *y, const int64_
   s.n2 = n2;
   s.trace_cap = trace_cap;
   s.tau = tau;
+  s.n2max = n2;
+  s.n2f = 1.f;
+  s.ltau = (tau > 0.0) ? (float)std::log10(tau) : -16.f;
+  {
+    double y2 = 0.0;  // ||Y||_F^2 (the controller's scale)
+    const double *yd = (const double *)y;
+    const size_t cnt = (size_t)n * n * (d.cplx ? 2 : 1);
+    for (size_t e = 0; e < cnt; ++e) y2 += yd[e] * yd[e];
+    s.ynorm2 = y2 > 0.0 ? y2 : 1.0;
+  }
   HIPCHK(hipMemcpyAsync(d.st, &s, sizeof(s), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));  // the host vectors go out of scope
   return JKG_OK;
@@ -329,6 +393,8 @@ static int capture_step(jkg_ctx *c, int fused) {
   key.wmode = c->d.wmode;
   key.npl = c->d.npl;  // grids and arguments follow this problem's J split (npl, nmi)
   key.neg_tol = c->d.neg_tol;
+  key.ctl_mode = c->d.ctl_mode;
+  key.ctl_cap = c->d.ctl_trace_cap;
   if (c->exec && c->gkey == key) return JKG_OK;
   if (c->exec) (void)hipGraphExecDestroy(c->exec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
@@ -395,9 +461,20 @@ static int run_loop(jkg_ctx *c, const jkg_opts *o, jkg_result *res) {
   return JKG_OK;
 }
 
+// What the mixed solver adds to a solve (all zero: the plain solver).
+struct MixedCfg {
+  int wmode = 0;
+  double tol = 0.0, tau = 0.0;
+  int ctl_mode = 0;
+  const float *ctl = nullptr;
+  float *ctl_trace = nullptr;
+  int ctl_trace_cap = 0;
+};
+
 static int solve_common(jkg_ctx *c, int cplx, int n, int n2, const void *y, const int64_t *j, void *yu, void *zz,
-                        double *sy, const jkg_opts *opts, jkg_result *res, int wmode = 0, double tol = 0.0,
-                        double tau = 0.0) {
+                        double *sy, const jkg_opts *opts, jkg_result *res, const MixedCfg &mc = MixedCfg{}) {
+  const int wmode = mc.wmode;
+  const double tol = mc.tol, tau = mc.tau;
   if (!c) return -1;
   if (n < 3) return -2;
   if (!y) return -4;
@@ -426,6 +503,28 @@ static int solve_common(jkg_ctx *c, int cplx, int n, int n2, const void *y, cons
     rc = ensure_mixed(c);
     if (rc) return rc;
   }
+  c->d.ctl_mode = mc.ctl_mode;
+  c->d.ctl_trace = nullptr;
+  c->d.ctl_trace_cap = 0;
+  if (mc.ctl_mode) {
+    rc = ensure_ctl(c, std::max(mc.ctl_trace_cap, 1));
+    if (rc) return rc;
+    c->d.ctl_master = c->ctl_master;
+    c->d.ctl_q = c->ctl_q;
+    c->d.ctl_table = c->ctl_table;
+    if (mc.ctl_trace) {
+      c->d.ctl_trace = c->ctl_trace;
+      c->d.ctl_trace_cap = mc.ctl_trace_cap;
+    }
+    if (mc.ctl_mode == 1) {
+      HIPCHK(hipMemcpyAsync(c->ctl_master, mc.ctl, sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice, c->stream));
+      mlp_quantize_launch(c->ctl_master, c->ctl_q, c->stream);
+      HIPCHK(hipGetLastError());
+    } else {
+      HIPCHK(hipMemcpyAsync(c->ctl_table, mc.ctl, sizeof(float) * 8, hipMemcpyHostToDevice, c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
   // TT: the number of trigonometric pairs (the reference's R%TRU result)
   int npos = 0, nneg = 0;
   rc = upload_problem(c, n, n2, y, j, &npos, &nneg, o.max_steps, c->trace_cap, tau);
@@ -439,6 +538,11 @@ static int solve_common(jkg_ctx *c, int cplx, int n, int n2, const void *y, cons
   if (o.trace && o.trace_cap > 0) {
     int cnt = std::min(res->steps, o.trace_cap);
     if (cnt > 0) HIPCHK(hipMemcpy(o.trace, d.trace, sizeof(jkg_steprec) * cnt, hipMemcpyDeviceToHost));
+  }
+  if (mc.ctl_mode && mc.ctl_trace && mc.ctl_trace_cap > 0) {
+    int cnt = std::min(res->steps, mc.ctl_trace_cap);
+    if (cnt > 0)
+      HIPCHK(hipMemcpy(mc.ctl_trace, d.ctl_trace, sizeof(float) * CTL_REC * cnt, hipMemcpyDeviceToHost));
   }
   if (res->info >= 0) {
     const size_t esz = cplx ? 16 : 8;
@@ -470,10 +574,18 @@ extern "C" int jkg_dsolve_mixed(jkg_ctx *c, int n, int n2, const double *y, cons
   else jkg_mixed_opts_default(&m);
   if (m.weights != JKG_W_PAIR && m.weights != JKG_W_GRAM) return -10;
   if (!(m.tol >= 0.0) || !(m.threshold >= 0.0 && m.threshold < 1.0)) return -10;
-  if (m.fp32_warm != 0 || m.key_bits != 64 || m.controller != 0) return JKG_ENOTSUP;
-  if (m.weights == JKG_W_PAIR && m.tol != 0.0) return JKG_ENOTSUP;
-  return solve_common(c, 0, n, n2, y, j, yu, zz, sy, opts, res, m.weights == JKG_W_GRAM ? 1 : 0, m.tol,
-                      m.threshold);
+  if (m.controller < 0 || m.controller > 2 || (m.controller && !m.ctl) || m.ctl_trace_cap < 0) return -10;
+  if (m.fp32_warm != 0 || m.key_bits != 64) return JKG_ENOTSUP;
+  if (m.weights == JKG_W_PAIR && (m.tol != 0.0 || m.controller)) return JKG_ENOTSUP;
+  MixedCfg mc;
+  mc.wmode = m.weights == JKG_W_GRAM ? 1 : 0;
+  mc.tol = m.tol;
+  mc.tau = m.controller ? 0.0 : m.threshold;
+  mc.ctl_mode = m.controller;
+  mc.ctl = m.ctl;
+  mc.ctl_trace = m.ctl_trace;
+  mc.ctl_trace_cap = m.ctl_trace ? m.ctl_trace_cap : 0;
+  return solve_common(c, 0, n, n2, y, j, yu, zz, sy, opts, res, mc);
 }
 
 extern "C" int jkg_zsolve(jkg_ctx *c, int n, int n2, const double *y, const int64_t *j, double *yu, double *zz,
@@ -581,6 +693,7 @@ static int prepare_single(jkg_ctx *c, int n, int n2, const void *a, const int64_
   c->d.fused = fused;
   c->d.wmode = 0;
   c->d.neg_tol = 0.0;
+  c->d.ctl_mode = 0;
   int npos, nneg;
   return upload_problem(c, n, jkg_jstep_len(n, n2), a, j, &npos, &nneg, -1, c->trace_cap);
 }
@@ -844,6 +957,7 @@ static int profile_common(jkg_ctx *c, int n, int n2, const double *y, const int6
   if (rc) return rc;
   c->d.wmode = wmode;
   c->d.neg_tol = tol;
+  c->d.ctl_mode = 0;
   if (wmode) {
     rc = ensure_mixed(c);
     if (rc) return rc;
@@ -915,4 +1029,149 @@ extern "C" int jkg_dprofile_mixed(jkg_ctx *c, int n, int n2, const double *y, co
   if (m.fp32_warm != 0 || m.key_bits != 64 || m.controller != 0 || m.threshold != 0.0) return JKG_ENOTSUP;
   if (m.weights == JKG_W_PAIR && m.tol != 0.0) return JKG_ENOTSUP;
   return profile_common(c, n, n2, y, j, fused, steps, stage_ms, res, m.weights == JKG_W_GRAM ? 1 : 0, m.tol);
+}
+
+// ---------------------------------------------------------------- controller training
+// Layout of the training scratch (floats): master | Adam m1 | Adam m2 | grad |
+// q (BF16, NPARAM/2 + 1 floats) | loss | X | T | W | Y | block partials | block losses.
+namespace {
+struct TrainBufs {
+  float *master, *m1, *m2, *grad, *loss, *X, *T, *W, *Y, *part, *lpart;
+  uint16_t *q;
+};
+size_t train_floats(int ns) {
+  const size_t np = MLP_NPARAM, nb = (size_t)mlp_blocks(ns);
+  return 4 * np + (np / 2 + 2) + 1 + (size_t)ns * (MLP_IN + 2 * MLP_OUT + 1) + nb * np + nb + 64;
+}
+TrainBufs train_bufs(float *b, int ns) {
+  const size_t np = MLP_NPARAM, nb = (size_t)mlp_blocks(ns);
+  TrainBufs t;
+  t.master = b;
+  t.m1 = b + np;
+  t.m2 = b + 2 * np;
+  t.grad = b + 3 * np;
+  t.q = (uint16_t *)(b + 4 * np);
+  float *p = b + 4 * np + (np / 2 + 2);
+  t.loss = p++;
+  t.X = p;
+  p += (size_t)ns * MLP_IN;
+  t.T = p;
+  p += (size_t)ns * MLP_OUT;
+  t.W = p;
+  p += ns;
+  t.Y = p;
+  p += (size_t)ns * MLP_OUT;
+  t.part = p;
+  p += nb * np;
+  t.lpart = p;
+  return t;
+}
+}  // namespace
+
+extern "C" int jkg_ctl_grad(jkg_ctx *c, int ns, const float *x, const float *t, const float *w,
+                            const float *params, float *grad, float *loss, float *y) {
+  if (!c) return -1;
+  if (ns < 1) return -2;
+  if (!x) return -3;
+  if (!t) return -4;
+  if (!params) return -6;
+  HIPCHK(hipSetDevice(c->device));
+  int rc = ensure_train(c, train_floats(ns));
+  if (rc) return rc;
+  TrainBufs b = train_bufs(c->tr_buf, ns);
+  double wsum = ns;
+  if (w) {
+    wsum = 0.0;
+    for (int i = 0; i < ns; ++i) wsum += w[i];
+    if (!(wsum > 0.0)) return -5;
+  }
+  HIPCHK(hipMemcpyAsync(b.master, params, sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(b.X, x, sizeof(float) * ns * MLP_IN, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(b.T, t, sizeof(float) * ns * MLP_OUT, hipMemcpyHostToDevice, c->stream));
+  if (w) HIPCHK(hipMemcpyAsync(b.W, w, sizeof(float) * ns, hipMemcpyHostToDevice, c->stream));
+  mlp_quantize_launch(b.master, b.q, c->stream);
+  mlp_grad_launch(b.q, b.master, b.X, b.T, w ? b.W : nullptr, ns, (float)(1.0 / wsum), b.part, b.lpart, b.grad,
+                  b.loss, y ? b.Y : nullptr, c->stream);
+  HIPCHK(hipGetLastError());
+  if (grad) HIPCHK(hipMemcpyAsync(grad, b.grad, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost, c->stream));
+  if (loss) HIPCHK(hipMemcpyAsync(loss, b.loss, sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  if (y) HIPCHK(hipMemcpyAsync(y, b.Y, sizeof(float) * ns * MLP_OUT, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return JKG_OK;
+}
+
+extern "C" int jkg_ctl_fit(jkg_ctx *c, int ns, const float *x, const float *t, const float *w, int epochs,
+                           int batch, float lr, float weight_decay, uint64_t seed, float *params, float *loss) {
+  if (!c) return -1;
+  if (ns < 1) return -2;
+  if (!x) return -3;
+  if (!t) return -4;
+  if (epochs < 1) return -6;
+  if (batch < 1) return -7;
+  if (!(lr > 0.f)) return -8;
+  if (!params) return -11;
+  HIPCHK(hipSetDevice(c->device));
+  batch = std::min(batch, ns);
+  // scratch sized for the full set (the per-epoch loss) and for one batch
+  int rc = ensure_train(c, train_floats(ns) + train_floats(batch));
+  if (rc) return rc;
+  TrainBufs full = train_bufs(c->tr_buf, ns);
+  TrainBufs bt = train_bufs(c->tr_buf + train_floats(ns), batch);
+  std::vector<float> zeros(MLP_NPARAM, 0.f);
+  HIPCHK(hipMemcpyAsync(full.master, params, sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(full.m1, zeros.data(), sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(full.m2, zeros.data(), sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(full.X, x, sizeof(float) * ns * MLP_IN, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(full.T, t, sizeof(float) * ns * MLP_OUT, hipMemcpyHostToDevice, c->stream));
+  std::vector<float> wv(ns, 1.f);
+  if (w) for (int i = 0; i < ns; ++i) wv[i] = w[i];
+  double wsum = 0.0;
+  for (int i = 0; i < ns; ++i) wsum += wv[i];
+  if (!(wsum > 0.0)) return -5;
+  HIPCHK(hipMemcpyAsync(full.W, wv.data(), sizeof(float) * ns, hipMemcpyHostToDevice, c->stream));
+  mlp_quantize_launch(full.master, full.q, c->stream);
+  // host-side permutation per epoch (splitmix64 + Fisher-Yates), batches copied in order
+  std::vector<int> perm(ns);
+  std::vector<float> bx((size_t)batch * MLP_IN), btg((size_t)batch * MLP_OUT), bw(batch);
+  uint64_t st = seed;
+  auto rnd = [&]() {
+    st += 0x9E3779B97F4A7C15ull;
+    uint64_t z = st;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  };
+  int stepno = 0;
+  for (int e = 0; e < epochs; ++e) {
+    for (int i = 0; i < ns; ++i) perm[i] = i;
+    for (int i = ns - 1; i > 0; --i) std::swap(perm[i], perm[rnd() % (uint64_t)(i + 1)]);
+    for (int b0 = 0; b0 + batch <= ns; b0 += batch) {
+      double bws = 0.0;
+      for (int k = 0; k < batch; ++k) {
+        const int i = perm[b0 + k];
+        for (int f = 0; f < MLP_IN; ++f) bx[(size_t)k * MLP_IN + f] = x[(size_t)i * MLP_IN + f];
+        for (int o = 0; o < MLP_OUT; ++o) btg[(size_t)k * MLP_OUT + o] = t[(size_t)i * MLP_OUT + o];
+        bw[k] = wv[i];
+        bws += wv[i];
+      }
+      if (!(bws > 0.0)) continue;
+      HIPCHK(hipMemcpyAsync(bt.X, bx.data(), sizeof(float) * batch * MLP_IN, hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(bt.T, btg.data(), sizeof(float) * batch * MLP_OUT, hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(bt.W, bw.data(), sizeof(float) * batch, hipMemcpyHostToDevice, c->stream));
+      mlp_grad_launch(full.q, full.master, bt.X, bt.T, bt.W, batch, (float)(1.0 / bws), bt.part, bt.lpart, bt.grad,
+                      nullptr, nullptr, c->stream);
+      mlp_adam_launch(full.master, full.q, bt.grad, full.m1, full.m2, lr, ++stepno, weight_decay, c->stream);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipStreamSynchronize(c->stream));  // the host batch buffers are reused
+    }
+    if (loss) {
+      mlp_grad_launch(full.q, full.master, full.X, full.T, full.W, ns, (float)(1.0 / wsum), full.part, full.lpart,
+                      full.grad, full.loss, nullptr, c->stream);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(&loss[e], full.loss, sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    }
+  }
+  HIPCHK(hipMemcpyAsync(params, full.master, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return JKG_OK;
 }

@@ -62,6 +62,7 @@ __device__ __forceinline__ Rot rot_ab(const double B[4]) {
 __global__ void k_step_begin(StepState *st, int nn, int n, int *colpiv) {
   __shared__ int act;
   if (threadIdx.x == 0) {
+    st->t_begin = wall_clock64();
     int a = !st->done;
     if (a && st->max_steps >= 0 && st->step >= st->max_steps) {
       st->done = 1;
@@ -269,6 +270,7 @@ __global__ void __launch_bounds__(SEL_THREADS_RES) k_sel_resolve(int n, const ui
         else lo = mid + 1;
       }
       st->it = lo;
+      st->wmax = (lo > 0) ? key_weight(keys[0]) : 0.0;
       // the mixed solver's threshold: walk only the weights >= tau * (largest weight)
       const double tau = st->tau;
       if (tau > 0.0 && lo > 0) {
@@ -644,9 +646,62 @@ __global__ void __launch_bounds__(512) k_rotate(int n, double *__restrict__ A, i
 // ------------------------------------------------------------------ end
 // NEWLEN, NORMF2D (summed in pivot order from -0, as one thread of the
 // reference does), TRIG, HYP; then SL = MIN(SL, NL) and the loop test.
+// Features of the mixed solver's step controller, written after each step
+// (the controller reads them at the start of the next one):
+//   0 step / 16N          1 log10(TW / TW_0) / 20   2 log10(TW / TW_prev) / 2
+//   3 SL / N_2max         4 NL / SL                 5 IT / NN
+//   6 walk depth (last chosen position / IT)        7 trigonometric share of the transforms
+//   8 log10(w_max / |Y|^2) / 32                     9 log2(N) / 16
+//  10 N_2 fraction used  11 normalised log10 tau used  12 log10(TW / |Y|^2) / 32
+//  13 smoothed feature 2 14 SL reached N_2           15 1
+__device__ __forceinline__ float clampf(float x, float lo, float hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+__device__ void controller_features(StepState *st, int s, int n, int nn, int sl, int nl, int ntr, int nhy, double tw,
+                                    const int *piv_pos, float *ctl_trace, int ctl_cap) {
+  const double tiny = 1e-300;
+  const double twv = tw > tiny ? tw : tiny;
+  if (s == 0) {
+    st->tw0 = twv;
+    st->twprev = twv;
+    st->ema = 0.f;
+  }
+  float f[16];
+  f[0] = fminf((float)s / (16.f * (float)n), 1.f);
+  f[1] = clampf((float)log10(twv / st->tw0), -20.f, 0.f) / 20.f;
+  const float rate = clampf((float)log10(twv / st->twprev), -2.f, 2.f) / 2.f;
+  f[2] = rate;
+  f[3] = (float)sl / (float)(st->n2max > 0 ? st->n2max : 1);
+  f[4] = (float)(nl > 0 ? nl : 0) / (float)(sl > 0 ? sl : 1);
+  f[5] = (float)st->it / (float)(nn > 0 ? nn : 1);
+  f[6] = (sl > 0 && st->it > 0) ? fminf((float)piv_pos[sl - 1] / (float)st->it, 1.f) : 0.f;
+  f[7] = (float)ntr / (float)(ntr + nhy > 0 ? ntr + nhy : 1);
+  const double wm = st->wmax > tiny ? st->wmax : tiny;
+  f[8] = clampf((float)log10(wm / st->ynorm2), -32.f, 0.f) / 32.f;
+  f[9] = log2f((float)n) / 16.f;
+  f[10] = st->n2f;
+  f[11] = (st->ltau <= -16.f) ? 0.f : (st->ltau + 16.f) / 12.f;
+  f[12] = clampf((float)log10(twv / st->ynorm2), -32.f, 0.f) / 32.f;
+  st->ema = 0.9f * st->ema + 0.1f * rate;
+  f[13] = st->ema;
+  f[14] = (sl >= st->n2) ? 1.f : 0.f;
+  f[15] = 1.f;
+  for (int i = 0; i < 16; ++i) st->feat[i] = f[i];
+  st->twprev = twv;
+  if (ctl_trace && s < ctl_cap) {
+    float *r = ctl_trace + (size_t)s * CTL_REC;
+    for (int i = 0; i < 16; ++i) r[i] = f[i];
+    r[16] = st->n2f;
+    r[17] = st->ltau;
+    r[18] = (float)(wall_clock64() - st->t_begin) * 0.01f;  // 100 MHz ticks -> microseconds
+    r[19] = (float)tw;
+  }
+}
+
 __global__ void __launch_bounds__(256) k_step_end(const int *__restrict__ J, const int *__restrict__ piv_p,
                                                    const int *__restrict__ piv_q, const int *__restrict__ piv_info,
-                                                   const double *__restrict__ piv_w, StepState *st, StepRec *trace) {
+                                                   const double *__restrict__ piv_w, StepState *st, StepRec *trace,
+                                                   int n, int nn, const int *__restrict__ piv_pos, int ctl_on,
+                                                   float *ctl_trace, int ctl_cap) {
   __shared__ int cnt[5];
   if (!st->active) return;
   const int sl = st->sl, s = st->step;
@@ -667,6 +722,7 @@ __global__ void __launch_bounds__(256) k_step_end(const int *__restrict__ J, con
       st->nrec = s + 1;
       st->done = 1;
       st->info = s;
+      if (ctl_on) controller_features(st, s, n, nn, 0, 0, 0, 0, -0.0, piv_pos, ctl_trace, ctl_cap);
     }
     return;
   }
@@ -731,6 +787,7 @@ __global__ void __launch_bounds__(256) k_step_end(const int *__restrict__ J, con
     st->acc_b2 += (unsigned long long)cnt[3];
     st->acc_b4 += (unsigned long long)cnt[4];
     st->nl = nl;
+    if (ctl_on) controller_features(st, s, n, nn, sl, nl, rec.trig, rec.hyp, tw, piv_pos, ctl_trace, ctl_cap);
     const int sl2 = (nl < sl) ? nl : sl;
     if (sl2 <= 0) {
       st->done = 1;
@@ -872,7 +929,7 @@ void rotate_launch(DeviceStep &d, hipStream_t s) {
 
 void step_end_launch(DeviceStep &d, hipStream_t s) {
   hipLaunchKernelGGL(k_step_end, dim3(1), dim3(256), 0, s, d.J, d.piv_p, d.piv_q, d.piv_info, d.piv_w, d.st,
-                     d.trace);
+                     d.trace, d.n, d.nn, d.piv_pos, d.ctl_mode != 0 ? 1 : 0, d.ctl_trace, d.ctl_trace_cap);
 }
 
 int step_prepare(DeviceStep &d) {
@@ -889,6 +946,7 @@ int step_prepare(DeviceStep &d) {
 
 void step_launch(DeviceStep &d, hipStream_t s) {
   step_begin_launch(d, s);
+  if (d.ctl_mode) controller_launch(d, s);
   weights_launch(d, s);
   sort_pairs_launch(d.sort, &d.st->nn_sort, d.nn, s, 1);
   select_launch(d, s);

@@ -55,6 +55,14 @@ struct StepState {
   int ntrig, nhyp;
   double tw;
   double tau;     // > 0: the walk stops at the first weight below tau * (largest weight) (mixed solver)
+  // step controller of the mixed solver (features after each step, last action)
+  int n2max;      // N_2 of the problem: the largest N_2 a controller may choose
+  int ctl_phase;  // phase of the last step under a phase-table policy
+  double wmax;    // largest weight of this step
+  double tw0, twprev, ynorm2;  // TW of the first and of the previous step; ||Y||_F^2
+  unsigned long long t_begin;  // wall clock (100 MHz) at the start of this step
+  float ema, n2f, ltau;        // smoothed TW rate; the action of this step (N_2 / n2max, log10 tau)
+  float feat[16];              // controller features after the last step
   // work counters summed over the steps (bytes and flops of the run)
   unsigned long long acc_sl, acc_it, acc_b2, acc_b4, acc_abod;
 };
@@ -89,7 +97,16 @@ struct DeviceStep {
   double *gram;           // npl x nmi: (O^T O)(plus, minus), O = A with a zero diagonal
   double *coln;           // 2N: off-diagonal column norms^2, then the diagonal of A
   double neg_tol;         // > 0: negligible pairs are dropped (mixed solver's convergence test)
+  // step controller: 0 none, 1 MLP (ctl_master/ctl_q), 2 phase table (ctl_table)
+  int ctl_mode;
+  const float *ctl_master;
+  const uint16_t *ctl_q;
+  const float *ctl_table;
+  float *ctl_trace;       // per step: 16 features, N_2 fraction, log10 tau, step time (us), TW
+  int ctl_trace_cap;
 };
+
+constexpr int CTL_REC = 20;  // floats per controller trace record
 
 // sets kernel attributes (dynamic LDS) for the problem size; before capture
 int step_prepare(DeviceStep &d);
@@ -117,6 +134,8 @@ void step_end_launch(DeviceStep &d, hipStream_t s);
 //   C[i * csi + j * csj] = sum_{r < k} X[r, xc(i)] * Y[r, yc(j)]
 // (xc(i) = xcols ? xcols[i] : i; zero_diag reads X[c, c] and Y[c, c] as 0).
 void gram_hyp_weights_launch(DeviceStep &d, hipStream_t s);
+// the step controller (jkg_mlp.hip): sets StepState::n2 and ::tau for this step
+void controller_launch(DeviceStep &d, hipStream_t s);
 void gemm_tn_launch(int k, const double *X, int ldx, const int *xcols, int m, const double *Y, int ldy,
                     const int *ycols, int nc, double *C, long csi, long csj, bool zero_diag, const StepState *st,
                     hipStream_t s);

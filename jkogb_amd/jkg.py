@@ -50,7 +50,15 @@ class MixedOpts(C.Structure):
     """jkg_mixed_opts: options of the mixed solver (no reference counterpart)."""
     _fields_ = [("weights", C.c_int32), ("fp32_warm", C.c_int32), ("switch_tol", C.c_double),
                 ("key_bits", C.c_int32), ("controller", C.c_int32), ("ctl", C.c_void_p),
-                ("threshold", C.c_double), ("tol", C.c_double), ("reserved", C.c_int32 * 8)]
+                ("threshold", C.c_double), ("tol", C.c_double), ("ctl_trace", C.c_void_p),
+                ("ctl_trace_cap", C.c_int32), ("reserved", C.c_int32 * 7)]
+
+
+CTL_NPARAM = 5378
+CTL_REC = 20
+CTL_FEATURES = ["step/16N", "log10(TW/TW0)/20", "log10(TW/TWprev)/2", "SL/N2", "NL/SL", "IT/NN", "walk depth",
+                "trig share", "log10(wmax/|Y|^2)/32", "log2(N)/16", "N2 fraction", "log10 tau (norm)",
+                "log10(TW/|Y|^2)/32", "smoothed rate", "SL reached N2", "1"]
 
 
 W_PAIR, W_GRAM = 0, 1
@@ -93,6 +101,8 @@ _dsolve_mixed = _sig("jkg_dsolve_mixed", _i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _
 _dweights_mixed = _sig("jkg_dweights_mixed", _i, [_vp, _i, _vp, _vp, _i, _vp])
 _dprofile_mixed = _sig("jkg_dprofile_mixed", _i, [_vp, _i, _i, _vp, _vp, _i, _i, C.POINTER(MixedOpts), _vp,
                                                   C.POINTER(Result)])
+_ctl_grad = _sig("jkg_ctl_grad", _i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp])
+_ctl_fit = _sig("jkg_ctl_fit", _i, [_vp, _i, _vp, _vp, _vp, _i, _i, C.c_float, C.c_float, C.c_uint64, _vp, _vp])
 _pair_index = _sig("jkg_pair_index", C.c_uint32, [_i, _i, _i])
 _key_weight = _sig("jkg_key_weight", _d, [C.c_uint64])
 
@@ -100,7 +110,7 @@ EXPORTS = ["jkg_create", "jkg_destroy", "jkg_strerror", "jkg_last_hip_error", "j
            "jkg_zsolve", "jkg_srt", "jkg_ncp", "jkg_tru", "jkg_jstep_len", "jkg_dhsvd2", "jkg_zhsvd2",
            "jkg_dweights", "jkg_zweights", "jkg_dbuild", "jkg_dstep", "jkg_dprofile", "jkg_pair_index",
            "jkg_key_weight", "jkg_mixed_opts_default", "jkg_dsolve_mixed", "jkg_dweights_mixed",
-           "jkg_dprofile_mixed"]
+           "jkg_dprofile_mixed", "jkg_ctl_grad", "jkg_ctl_fit"]
 
 
 def mixed_opts(**kw):
@@ -216,8 +226,11 @@ class Context:
         return dict(YU=YU, ZZ=ZZ, SY=SY, info=r.info, trace=trace[:min(r.steps, trace_cap)], tt=r.tt,
                     pn=(r.npos, r.nneg), n2=r.n2, seconds=r.seconds, steps=r.steps, **_counts(r))
 
-    def dsolve_mixed(self, Y, J, n2=0, fused=False, max_steps=-1, trace_cap=None, steps_per_sync=0, **mixed):
-        """The mixed solver (jkg_dsolve_mixed): same outputs as dsolve, agreeing to rounding."""
+    def dsolve_mixed(self, Y, J, n2=0, fused=False, max_steps=-1, trace_cap=None, steps_per_sync=0,
+                     ctl_params=None, ctl_table=None, ctl_trace=False, **mixed):
+        """The mixed solver (jkg_dsolve_mixed): same outputs as dsolve, agreeing to rounding.
+        ctl_params (JKG_CTL_NPARAM floats): the MLP controller; ctl_table (8 floats): a phase
+        table; ctl_trace: also return the per-step controller records as out["ctl"]."""
         Y = np.asfortranarray(Y, dtype=np.float64)
         n = Y.shape[0]
         if Y.shape != (n, n):
@@ -231,11 +244,57 @@ class Context:
         trace = np.zeros(max(trace_cap, 1), STEPREC_DTYPE)
         o = Opts(0, 0, 1 if fused else 0, max_steps, steps_per_sync, trace_cap, _p(trace))
         m = mixed_opts(**mixed)
+        keep = []
+        if ctl_params is not None:
+            cp = np.ascontiguousarray(ctl_params, dtype=np.float32)
+            if cp.size != CTL_NPARAM:
+                raise ValueError("ctl_params needs %d floats" % CTL_NPARAM)
+            keep.append(cp)
+            m.controller, m.ctl = 1, cp.ctypes.data
+        elif ctl_table is not None:
+            ct = np.ascontiguousarray(ctl_table, dtype=np.float32)
+            if ct.size != 8:
+                raise ValueError("ctl_table needs 8 floats")
+            keep.append(ct)
+            m.controller, m.ctl = 2, ct.ctypes.data
+        ctr = None
+        if ctl_trace:
+            ctr = np.zeros((max(trace_cap, 1), CTL_REC), np.float32)
+            m.ctl_trace, m.ctl_trace_cap = ctr.ctypes.data, ctr.shape[0]
         r = Result()
         _chk(_dsolve_mixed(self._h, n, n2, _p(Y), _p(J64), _p(YU), _p(ZZ), _p(SY), C.byref(o), C.byref(m),
                            C.byref(r)), "jkg_dsolve_mixed")
-        return dict(YU=YU, ZZ=ZZ, SY=SY, info=r.info, trace=trace[:min(r.steps, trace_cap)], tt=r.tt,
-                    pn=(r.npos, r.nneg), n2=r.n2, seconds=r.seconds, steps=r.steps, **_counts(r))
+        out = dict(YU=YU, ZZ=ZZ, SY=SY, info=r.info, trace=trace[:min(r.steps, trace_cap)], tt=r.tt,
+                   pn=(r.npos, r.nneg), n2=r.n2, seconds=r.seconds, steps=r.steps, **_counts(r))
+        if ctr is not None:
+            out["ctl"] = ctr[:min(r.steps, ctr.shape[0])].copy()
+        return out
+
+    # ---------------------------------------------------------------- controller training
+    def ctl_grad(self, X, T, params, W=None):
+        """Loss, gradient and outputs of the controller MLP over the samples (library kernels)."""
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        P = np.ascontiguousarray(params, dtype=np.float32)
+        ns = X.shape[0]
+        Wv = None if W is None else np.ascontiguousarray(W, dtype=np.float32)
+        g = np.zeros(CTL_NPARAM, np.float32)
+        y = np.zeros((ns, 2), np.float32)
+        loss = np.zeros(1, np.float32)
+        _chk(_ctl_grad(self._h, ns, _p(X), _p(T), None if Wv is None else _p(Wv), _p(P), _p(g), _p(loss), _p(y)),
+             "jkg_ctl_grad")
+        return float(loss[0]), g, y
+
+    def ctl_fit(self, X, T, params, W=None, epochs=50, batch=256, lr=1e-3, weight_decay=0.0, seed=1):
+        """Trains the controller MLP on the GPU; returns (params, per-epoch full-set loss)."""
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        P = np.array(params, dtype=np.float32, copy=True)
+        Wv = None if W is None else np.ascontiguousarray(W, dtype=np.float32)
+        loss = np.zeros(epochs, np.float32)
+        _chk(_ctl_fit(self._h, X.shape[0], _p(X), _p(T), None if Wv is None else _p(Wv), epochs, batch, lr,
+                      weight_decay, seed, _p(P), _p(loss)), "jkg_ctl_fit")
+        return P, loss
 
     def dprofile_mixed(self, Y, J, n2=0, steps=4, fused=False, **mixed):
         """As dprofile, for the steps of the mixed solver."""

@@ -1,0 +1,117 @@
+"""The mixed solver's step controller on the GPU, through the C ABI: the
+training kernels against the float64 CPU restatement (oracle/mlp_ref.py), and
+the in-graph controller's actions against the restatement's forward pass of
+the features it recorded.  Tolerances are written in each test."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+EPS = np.finfo(np.float64).eps
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from jkogb_amd import jkg
+    c = jkg.Context(0)
+    yield c
+    c.close()
+
+
+def inputs(cfg, n, seed):
+    from jkogb_amd import inputs as gen
+    return gen.make(cfg, n=n, seed=seed)
+
+
+@pytest.mark.parametrize("ns", [1, 63, 300])
+def test_ctl_grad_vs_oracle(ctx, ns):
+    from oracle import mlp_ref
+    rng = np.random.default_rng(ns)
+    p = mlp_ref.init(ns).astype(np.float32)
+    X = rng.standard_normal((ns, 16)).astype(np.float32)
+    T = rng.random((ns, 2)).astype(np.float32)
+    W = (rng.random(ns) + 0.25).astype(np.float32)
+    loss, g, y = ctx.ctl_grad(X, T, p, W)
+    rl, rg, ry = mlp_ref.loss_grad(p, X, T, W)
+    # FP32 accumulation against float64: 1e-4 of the largest magnitude per quantity
+    assert abs(loss - rl) <= 1e-4 * abs(rl) + 1e-7
+    assert np.max(np.abs(y - ry)) <= 1e-4 * max(1.0, np.max(np.abs(ry)))
+    assert np.max(np.abs(g - rg)) <= 1e-4 * max(1e-3, np.max(np.abs(rg)))
+
+
+def test_ctl_fit_learns_a_mapping(ctx):
+    from oracle import mlp_ref
+    rng = np.random.default_rng(2)
+    X = rng.random((2000, 16)).astype(np.float32)
+    X[:, 15] = 1.0
+    T = np.stack([0.5 + 0.4 * np.sin(3 * X[:, 0]), np.clip(X[:, 1] * X[:, 2] + 0.2, 0, 1)], 1).astype(np.float32)
+    p0 = mlp_ref.init(5)
+    l0, _, _ = ctx.ctl_grad(X, T, p0)
+    p, hist = ctx.ctl_fit(X, T, p0, epochs=40, batch=128, lr=3e-3, seed=7)
+    l1, _, _ = ctx.ctl_grad(X, T, p)
+    assert np.all(np.isfinite(hist)) and hist[-1] == pytest.approx(l1, rel=1e-5)
+    assert l1 < 0.05 * l0, (l0, l1)
+    # deterministic: the same seed gives the same parameters
+    p2, _ = ctx.ctl_fit(X, T, p0, epochs=40, batch=128, lr=3e-3, seed=7)
+    assert np.array_equal(p, p2)
+
+
+def test_in_graph_controller_follows_the_network(ctx):
+    from oracle import mlp_ref
+    rng = np.random.default_rng(3)
+    p = mlp_ref.init(11, scale=0.5)
+    d = mlp_ref.unpack(p)
+    d["b3"][:] = [0.6, 0.4]  # mid-range actions
+    p = mlp_ref.pack(d).astype(np.float32)
+    Y, J = inputs("c2", 128, 5)
+    out = ctx.dsolve_mixed(Y, J, 64, ctl_params=p, ctl_trace=True)
+    ctl = out["ctl"]
+    assert out["info"] >= 0 and len(ctl) == out["steps"]
+    assert ctl[0, 16] == 1.0 and ctl[0, 17] == -16.0  # first step: the plain ordering
+    checked = 0
+    for s in range(1, len(ctl)):
+        f = ctl[s - 1, :16].astype(np.float64)
+        if f[1] * 20 <= -10:  # fallback region
+            assert ctl[s, 16] == 1.0 and ctl[s, 17] == -16.0
+            continue
+        n2f, lt = mlp_ref.action(p, f)
+        assert abs(ctl[s, 16] - n2f) <= 1e-4 and abs(ctl[s, 17] - lt) <= 2e-3, (s, ctl[s, 16:18], n2f, lt)
+        checked += 1
+    assert checked > 10
+    assert np.all(ctl[:, 18] > 0)  # step times recorded
+
+
+def test_phase_table_controller(ctx):
+    Y, J = inputs("c2", 96, 6)
+    table = np.array([-3.0, -6.0, 0.75, -8.0, 1.0, -12.0, 1.0, -16.0], np.float32)
+    out = ctx.dsolve_mixed(Y, J, 48, ctl_table=table, ctl_trace=True)
+    ctl = out["ctl"]
+    for s in range(1, len(ctl)):
+        lrel = ctl[s - 1, 1] * 20
+        if lrel <= -10:
+            want = (1.0, -16.0)
+        elif lrel > -3:
+            want = (0.75, -8.0)
+        elif lrel > -6:
+            want = (1.0, -12.0)
+        else:
+            want = (1.0, -16.0)
+        assert (ctl[s, 16], ctl[s, 17]) == pytest.approx(want), s
+
+
+@pytest.mark.parametrize("cfg,n", [("c1", 128), ("c2", 160)])
+def test_controlled_solve_accuracy(ctx, cfg, n):
+    # a controlled run stays within the spread of the reference's two FMA builds (+64 N eps)
+    from oracle import mlp_ref
+    p = mlp_ref.init(13, scale=0.5)
+    d = mlp_ref.unpack(p)
+    d["b3"][:] = [0.5, 0.5]
+    p = mlp_ref.pack(d).astype(np.float32)
+    Y, J = inputs(cfg, n, 8)
+    plain = ctx.dsolve(Y, J, n // 2)
+    fused = ctx.dsolve(Y, J, n // 2, fused=True)
+    out = ctx.dsolve_mixed(Y, J, n // 2, ctl_params=p)
+    assert out["info"] >= 0
+    spread = np.max(np.abs(fused["SY"] - plain["SY"]) / np.abs(plain["SY"]))
+    rel = np.max(np.abs(out["SY"] - plain["SY"]) / np.abs(plain["SY"]))
+    assert rel <= 4 * spread + 64 * n * EPS, (rel, spread)
