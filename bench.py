@@ -18,9 +18,10 @@ Printed as one JSON line:
   plain        the same for the plain solver; step_speedup = plain / mixed
   roofline     the dominant stage of the mixed step, against HBM peak
   rooflines    the Gram product (FP64 MFMA) and the rotations (HBM)
-  full_solve   complete HSVDs at N = 1024 (C2 input) with both solvers: wall
-               time, steps, and the singular values' largest relative
-               difference in units of N eps
+  full_solve   complete HSVDs at N = 1024 (C2 input): plain, mixed, and mixed
+               with the trained step controller (jkogb_amd/ctl_params.npy):
+               device time, steps, and the singular values' largest
+               relative difference from the plain solver in units of N eps
   cpu_baseline the reference (oracle/_ref/plain/djk.exe, built from the
                reference's own sources) on the same input, host cores
 
@@ -163,13 +164,21 @@ def main():
     if not args.no_full:
         yc, jc = inputs.make("c2", n=1024, seed=0)
         rp = ctx.dsolve(yc, jc, 512, fused=args.fused)
+        rf = ctx.dsolve(yc, jc, 512, fused=not args.fused)
         rm = ctx.dsolve_mixed(yc, jc, 512, fused=args.fused)
-        err = float(np.max(np.abs(rm["SY"] - rp["SY"]) / np.abs(rp["SY"])) / (1024 * np.finfo(float).eps))
+        ctl = np.load(os.path.join(ROOT, "jkogb_amd", "ctl_params.npy"))
+        rl = ctx.dsolve_mixed(yc, jc, 512, fused=args.fused, ctl_params=ctl)
+        ne = 1024 * np.finfo(float).eps
+        err = lambda r: round(float(np.max(np.abs(r["SY"] - rp["SY"]) / np.abs(rp["SY"])) / ne), 4)
         full = {"workload": "C2: N=1024 real, columns 10**U(-4,4), J balanced, N_2=512",
                 "plain": {"steps": rp["steps"], "seconds": round(rp["seconds"], 4)},
-                "mixed": {"steps": rm["steps"], "seconds": round(rm["seconds"], 4)},
-                "speedup": round(rp["seconds"] / rm["seconds"], 3),
-                "sy_max_rel_diff_N_eps": round(err, 4)}
+                "mixed": {"steps": rm["steps"], "seconds": round(rm["seconds"], 4), "sy_err_N_eps": err(rm)},
+                "learned": {"steps": rl["steps"], "seconds": round(rl["seconds"], 4), "sy_err_N_eps": err(rl)},
+                "speedup_mixed": round(rp["seconds"] / rm["seconds"], 3),
+                "speedup_learned": round(rp["seconds"] / rl["seconds"], 3),
+                "reference_build_spread_N_eps": err(rf),
+                "note": "sy_err: largest relative SY difference against the plain (reference-exact) solver; "
+                        "reference_build_spread: the same between the reference's unfused and fused builds"}
     ctx.close()
 
     cpu = None

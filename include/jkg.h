@@ -166,15 +166,15 @@ typedef struct {
 } jkg_mixed_opts;
 
 /* The step controller.  At the start of each step it sets that step's N_2 (a
- * fraction in [0.25, 1] of the problem's N_2) and threshold tau (log10 tau in
- * [-16, -4]; -16 means none) from 16 features of the previous step:
+ * fraction of the problem's N_2) and threshold tau (-16 for log10 tau means
+ * none) from 16 features of the previous step:
  *   0 step / 16N, 1 log10(TW/TW_0)/20, 2 log10(TW/TW_prev)/2, 3 SL/N_2, 4 NL/SL,
  *   5 IT/NN, 6 last chosen position / IT, 7 trigonometric share, 8 log10(w_max/|Y|^2)/32,
  *   9 log2(N)/16, 10 N_2 fraction used, 11 (log10 tau used + 16)/12 (0: none),
  *   12 log10(TW/|Y|^2)/32, 13 smoothed feature 2, 14 SL reached N_2, 15 1.
  * MLP (16-64-64-2, ReLU): W1[64][16], b1[64], W2[64][64], b2[64], W3[2][64], b3[2] as
  * FP32; the device rounds the weights to BF16 and accumulates in FP32.  Outputs
- * y clamped to [0, 1]: N_2 fraction = 0.25 + 0.75 y0, log10 tau = -16 + 12 y1.
+ * y clamped to [0, 1]: N_2 fraction = 0.75 + 0.25 y0, log10 tau = -16 + 10 y1.
  * Phase table t[8]: phase 0 while log10(TW/TW_0) > t[0], phase 1 while > t[1],
  * then phase 2; actions (N_2 fraction, log10 tau) in t[2..7].
  * Fallback (both): the plain full ordering on the first step and once TW has

@@ -196,7 +196,7 @@ __global__ void k_mlp_adam(float *__restrict__ master, uint16_t *__restrict__ q,
 // N_2 (st->n2, a fraction of the problem's N_2) and threshold (st->tau) from
 // the features the previous step left (st->feat):
 //   mode 1: the MLP; outputs y clamped to [0, 1] map to
-//           N_2 fraction = 0.25 + 0.75 y0 and log10 tau = -16 + 12 y1;
+//           N_2 fraction = 0.75 + 0.25 y0 and log10 tau = -16 + 10 y1;
 //   mode 2: a phase table t[8]: phases by log10(TW / TW_0) against t[0] > t[1],
 //           actions (N_2 fraction, log10 tau) in t[2..7].
 // The plain full ordering (all of N_2, no threshold) is the fallback: on the
@@ -229,8 +229,8 @@ __global__ void __launch_bounds__(64) k_controller(StepState *st, int mode, cons
     if (y[0] == y[0] && y[1] == y[1] && lrel > -10.f) {
       const float y0 = y[0] < 0.f ? 0.f : (y[0] > 1.f ? 1.f : y[0]);
       const float y1 = y[1] < 0.f ? 0.f : (y[1] > 1.f ? 1.f : y[1]);
-      n2f = 0.25f + 0.75f * y0;
-      ltau = -16.f + 12.f * y1;
+      n2f = 0.75f + 0.25f * y0;
+      ltau = -16.f + 10.f * y1;
     }
   } else if (mode == 2) {
     const int ph = first ? 0 : (lrel > table[0] ? 0 : (lrel > table[1] ? 1 : 2));

@@ -79,4 +79,4 @@ def action(p, f):
     """The controller's action for one feature vector: (N_2 fraction, log10 tau)."""
     y, _ = forward(p, np.asarray(f, dtype=np.float64)[None, :])
     y = np.clip(y[0], 0.0, 1.0)
-    return 0.25 + 0.75 * y[0], -16.0 + 12.0 * y[1]
+    return 0.75 + 0.25 * y[0], -16.0 + 10.0 * y[1]

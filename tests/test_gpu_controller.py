@@ -83,7 +83,7 @@ def test_in_graph_controller_follows_the_network(ctx):
 
 def test_phase_table_controller(ctx):
     Y, J = inputs("c2", 96, 6)
-    table = np.array([-3.0, -6.0, 0.75, -8.0, 1.0, -12.0, 1.0, -16.0], np.float32)
+    table = np.array([-3.0, -6.0, 0.75, -8.0, 1.0, -12.0, 1.0, -16.0], np.float32)  # any actions
     out = ctx.dsolve_mixed(Y, J, 48, ctl_table=table, ctl_trace=True)
     ctl = out["ctl"]
     for s in range(1, len(ctl)):
@@ -115,3 +115,25 @@ def test_controlled_solve_accuracy(ctx, cfg, n):
     spread = np.max(np.abs(fused["SY"] - plain["SY"]) / np.abs(plain["SY"]))
     rel = np.max(np.abs(out["SY"] - plain["SY"]) / np.abs(plain["SY"]))
     assert rel <= 4 * spread + 64 * n * EPS, (rel, spread)
+
+
+def test_trained_controller_gain(ctx):
+    """What the trained network (jkogb_amd/ctl_params.npy) gains over the mixed solver without it, on an
+    input of the learned-controller family that training and validation did not see (seed 7, N=512);
+    the plain full-sort ordering judges: equal accuracy (within the spread of the reference's two FMA
+    builds + 64 N eps) and fewer steps in less device time."""
+    import os
+    from jkogb_amd import ctl_train
+    p = np.load(os.path.join(os.path.dirname(ctl_train.__file__), "ctl_params.npy"))
+    Y, J = inputs("c5", 512, 7)
+    plain = ctx.dsolve(Y, J, 256)
+    fused = ctx.dsolve(Y, J, 256, fused=True)
+    base = ctx.dsolve_mixed(Y, J, 256)
+    mlp = ctx.dsolve_mixed(Y, J, 256, ctl_params=p)
+    gain = base["seconds"] / mlp["seconds"]
+    print("trained controller: %.3f s (%d steps) vs %.3f s (%d steps) without it: x%.3f; plain %.3f s (%d steps)"
+          % (mlp["seconds"], mlp["steps"], base["seconds"], base["steps"], gain, plain["seconds"], plain["steps"]))
+    spread = np.max(np.abs(fused["SY"] - plain["SY"]) / np.abs(plain["SY"]))
+    rel = np.max(np.abs(mlp["SY"] - plain["SY"]) / np.abs(plain["SY"]))
+    assert rel <= 4 * spread + 64 * 512 * EPS, (rel, spread)
+    assert mlp["steps"] < base["steps"] and gain > 1.02, gain
