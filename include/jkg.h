@@ -201,6 +201,36 @@ int jkg_dweights_mixed(jkg_ctx *ctx, int n, const double *a, const int64_t *j, i
 int jkg_dprofile_mixed(jkg_ctx *ctx, int n, int n2, const double *y, const int64_t *j, int fused, int steps,
                        const jkg_mixed_opts *m, double *stage_ms, jkg_result *res);
 
+/* ---------------------------------------------------------------------------
+ * Batched solver: count independent real HSVDs of different sizes, each with
+ * djk's semantics and arithmetic (YU, ZZ, SY and INFO of instance i are what
+ * jkg_dsolve returns for it alone), advanced together by one persistent launch
+ * per step (a workgroup runs a whole step of one instance; the workgroups walk
+ * instance lists scheduled by decreasing cost).  ns[i]: N of instance i
+ * (3..1024); n2s: N_2 per instance (NULL or 0: N/2; clamped by JSTEP_LEN);
+ * y, yu, zz: the N_i x N_i matrices one after another (column-major); j: the
+ * N_i int64 signs one after another; sy: the N_i values one after another;
+ * info[i]: the instance's INFO.  The launch grid and the kernel variant (rows
+ * staged per weight tile, walk entries per lane) are chosen by an autotuner
+ * that times two steps of every candidate on this batch unless opts sets them.
+ * --------------------------------------------------------------------------- */
+typedef struct {
+  int32_t fused;     /* FMAD policy: 0 unfused (the reference's default build), 1 fma */
+  int32_t max_steps; /* < 0: to convergence; else at most this many steps per instance */
+  int32_t grid;      /* workgroups of the persistent launch; 0: tuned */
+  int32_t variant;   /* kernel variant 0..3; -1: tuned */
+  /* results */
+  int32_t steps;         /* batched steps launched */
+  int32_t tuned_grid;    /* grid used */
+  int32_t tuned_variant; /* variant used */
+  int32_t reserved;
+  double seconds;      /* device time of the step loop */
+  double tune_seconds; /* host time spent tuning */
+} jkg_batch_opts;
+void jkg_batch_opts_default(jkg_batch_opts *o);
+int jkg_dsolve_batched(jkg_ctx *ctx, int count, const int32_t *ns, const int32_t *n2s, const double *y,
+                       const int64_t *j, double *yu, double *zz, double *sy, int32_t *info, jkg_batch_opts *opts);
+
 /* index of pair (p,q), 1 <= p < q <= n, in the tie-break order of AW_CMP */
 uint32_t jkg_pair_index(int n, int p, int q);
 /* the weight a sort key stands for (NaN for the key of a dropped pair) */

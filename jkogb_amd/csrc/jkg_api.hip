@@ -14,6 +14,7 @@
 #include "jkg_hsvd2.hpp"
 #include "jkg_step.hpp"
 #include "jkg_mlp.hpp"
+#include "jkg_batch.hpp"
 
 using namespace jkg;
 
@@ -1173,5 +1174,267 @@ extern "C" int jkg_ctl_fit(jkg_ctx *c, int ns, const float *x, const float *t, c
   }
   HIPCHK(hipMemcpyAsync(params, full.master, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  return JKG_OK;
+}
+
+// ---------------------------------------------------------------- batched solver
+extern "C" void jkg_batch_opts_default(jkg_batch_opts *o) {
+  memset(o, 0, sizeof(*o));
+  o->max_steps = -1;
+  o->variant = -1;
+}
+
+namespace {
+// Device memory of one batch, freed by the destructor.
+struct BatchMem {
+  std::vector<void *> ptrs;
+  ~BatchMem() {
+    for (void *p : ptrs) (void)hipFree(p);
+  }
+  template <class T>
+  int get(T **p, size_t count) {
+    void *q = nullptr;
+    if (hipMalloc(&q, sizeof(T) * (count ? count : 1)) != hipSuccess) {
+      (void)hipGetLastError();
+      return JKG_ENOMEM;
+    }
+    ptrs.push_back(q);
+    *p = (T *)q;
+    return JKG_OK;
+  }
+};
+}  // namespace
+
+extern "C" int jkg_dsolve_batched(jkg_ctx *c, int count, const int32_t *ns, const int32_t *n2s, const double *y,
+                                  const int64_t *j, double *yu, double *zz, double *sy, int32_t *info,
+                                  jkg_batch_opts *opts) {
+  if (!c) return -1;
+  if (count < 1) return -2;
+  if (!ns) return -3;
+  if (!y) return -5;
+  if (!j) return -6;
+  if (!yu) return -7;
+  if (!zz) return -8;
+  if (!sy) return -9;
+  if (!info) return -10;
+  jkg_batch_opts o;
+  if (opts) o = *opts;
+  else jkg_batch_opts_default(&o);
+  if (o.grid < 0 || o.variant < -1 || o.variant >= batch_variants()) return -11;
+  HIPCHK(hipSetDevice(c->device));
+  // layout of the pools
+  std::vector<BInst> inst(count);
+  long long mat = 0, pairs = 0, ints = 0, dbl = 0, rot = 0, syo = 0;
+  std::vector<int> J32;
+  std::vector<int> lists;
+  for (int i = 0; i < count; ++i) {
+    const int n = ns[i];
+    if (n < 3 || n > 1024) return -3;
+    BInst &I = inst[i];
+    I.n = n;
+    I.n2 = jkg_jstep_len(n, n2s ? n2s[i] : 0);
+    if (I.n2 <= 0) return -4;
+    I.a = mat;
+    I.u = mat + (long long)n * n;
+    I.z = mat + 2LL * n * n;
+    mat += 3LL * n * n;
+    I.k = pairs;
+    pairs += (long long)n * (n - 1) / 2;
+    I.j = ints;
+    I.pm = ints + n;
+    I.cp = ints + 2 * n;
+    I.pv = ints + 3 * n;
+    ints += 3LL * n + 4LL * I.n2;
+    I.pw = dbl;
+    dbl += 3LL * I.n2;
+    I.pr = rot;
+    rot += 2LL * I.n2;
+    I.sy = syo;
+    syo += n;
+  }
+  // J, plus/minus lists (host) into the int pool image
+  std::vector<int> ipool((size_t)ints, 0);
+  {
+    long long jo = 0;
+    for (int i = 0; i < count; ++i) {
+      BInst &I = inst[i];
+      int np = 0, nm = 0;
+      std::vector<int> pl, mi;
+      for (int r = 0; r < I.n; ++r) {
+        const int64_t v = j[jo + r];
+        if (v != 1 && v != -1) return JKG_EJ;
+        ipool[I.j + r] = (int)v;
+        (v == 1 ? pl : mi).push_back(r);
+      }
+      np = (int)pl.size();
+      nm = (int)mi.size();
+      I.npl = np;
+      I.nmi = nm;
+      for (int r = 0; r < np; ++r) ipool[I.pm + r] = pl[r];
+      for (int r = 0; r < nm; ++r) ipool[I.pm + np + r] = mi[r];
+      jo += I.n;
+    }
+  }
+  // schedule: instances by decreasing cost (N^3) onto the least-loaded workgroup
+  std::vector<int> order(count);
+  for (int i = 0; i < count; ++i) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](int x, int y2) { return inst[x].n > inst[y2].n; });
+  auto make_sched = [&](int grid, std::vector<int> &sched, std::vector<int> &off) {
+    std::vector<std::vector<int>> lists2(grid);
+    std::vector<double> load(grid, 0.0);
+    for (int i : order) {
+      int b = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      lists2[b].push_back(i);
+      load[b] += (double)inst[i].n * inst[i].n * inst[i].n;
+    }
+    sched.clear();
+    off.assign(grid + 1, 0);
+    for (int b = 0; b < grid; ++b) {
+      off[b] = (int)sched.size();
+      for (int i : lists2[b]) sched.push_back(i);
+    }
+    off[grid] = (int)sched.size();
+  };
+  BatchMem m;
+  BatchArgs a{};
+  a.count = count;
+  a.pair_total = pairs;
+  BInst *dinst;
+  int rc = JKG_OK;
+  int *dsched, *doff;
+  const int maxgrid = 4096;
+  if ((rc = m.get(&dinst, count)) || (rc = m.get(&a.state, count)) || (rc = m.get(&a.mat, mat)) ||
+      (rc = m.get(&a.dbl, dbl)) || (rc = m.get(&a.ints, ints)) || (rc = m.get(&a.keys, 2 * pairs)) ||
+      (rc = m.get(&a.vals, 2 * pairs)) || (rc = m.get(&a.rot, rot)) || (rc = m.get(&dsched, count)) ||
+      (rc = m.get(&doff, maxgrid + 1)) || (rc = m.get(&a.ndone, 1)))
+    return rc;
+  double *dsy;
+  if ((rc = m.get(&dsy, syo))) return rc;
+  a.inst = dinst;
+  a.sched = dsched;
+  a.sched_off = doff;
+  HIPCHK(hipMemcpy(dinst, inst.data(), sizeof(BInst) * count, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(a.ints, ipool.data(), sizeof(int) * ints, hipMemcpyHostToDevice));
+  // (re)starts every instance from its Y
+  auto reset = [&](int max_steps) -> int {
+    long long yo = 0;
+    for (int i = 0; i < count; ++i) {
+      const long long nn2 = (long long)inst[i].n * inst[i].n;
+      HIPCHK(hipMemcpyAsync(a.mat + inst[i].a, y + yo, sizeof(double) * nn2, hipMemcpyHostToDevice, c->stream));
+      yo += nn2;
+    }
+    std::vector<BState> st0(count);
+    for (int i = 0; i < count; ++i) {
+      memset(&st0[i], 0, sizeof(BState));
+      st0[i].max_steps = max_steps;
+    }
+    HIPCHK(hipMemcpyAsync(a.state, st0.data(), sizeof(BState) * count, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemsetAsync(a.ndone, 0, sizeof(int), c->stream));
+    batch_identity_launch(a, c->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return JKG_OK;
+  };
+  std::vector<int> sched, off;
+  auto set_grid = [&](int grid) -> int {
+    make_sched(grid, sched, off);
+    HIPCHK(hipMemcpy(dsched, sched.data(), sizeof(int) * count, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(doff, off.data(), sizeof(int) * (grid + 1), hipMemcpyHostToDevice));
+    return JKG_OK;
+  };
+  // autotuner: two steps of every candidate (grid x variant) from the start
+  int grid = o.grid, variant = o.variant;
+  const double t_tune0 = (double)clock() / CLOCKS_PER_SEC;
+  if (grid == 0 || variant < 0) {
+    std::vector<int> grids;
+    for (int g : {256, 512, 1024, 2048})
+      if (o.grid == 0 ? (g <= std::max(count, 256)) : g == o.grid) grids.push_back(g);
+    if (o.grid != 0 && grids.empty()) grids.push_back(o.grid);
+    float best = 1e30f;
+    for (int g : grids) {
+      if ((rc = set_grid(g))) return rc;
+      for (int v = 0; v < batch_variants(); ++v) {
+        if (o.variant >= 0 && v != o.variant) continue;
+        if ((rc = reset(-1))) return rc;
+        HIPCHK(hipEventRecord(c->ev0, c->stream));
+        for (int k = 0; k < 2; ++k) batch_step_launch(a, g, v, o.fused, c->stream);
+        HIPCHK(hipEventRecord(c->ev1, c->stream));
+        HIPCHK(hipEventSynchronize(c->ev1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        if (ms < best) {
+          best = ms;
+          grid = g;
+          variant = v;
+        }
+      }
+    }
+  }
+  o.tune_seconds = (double)clock() / CLOCKS_PER_SEC - t_tune0;
+  if ((rc = set_grid(grid))) return rc;
+  if ((rc = reset(o.max_steps))) return rc;
+  // the step loop: a graph of 8 batched steps, replayed until every instance is done
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const int per_graph = 8;
+  HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  for (int k = 0; k < per_graph; ++k) batch_step_launch(a, grid, variant, o.fused, c->stream);
+  HIPCHK(hipStreamEndCapture(c->stream, &graph));
+  if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    return JKG_EHIP;
+  }
+  int maxn = 0;
+  for (int i = 0; i < count; ++i) maxn = std::max(maxn, inst[i].n);
+  const long cap = (o.max_steps >= 0) ? (long)o.max_steps + per_graph : 1000L * maxn + 1000;
+  long launched = 0;
+  int ndone = 0;
+  rc = JKG_OK;
+  (void)hipEventRecord(c->ev0, c->stream);
+  for (;;) {
+    if (hipGraphLaunch(exec, c->stream) != hipSuccess) {
+      rc = JKG_EHIP;
+      break;
+    }
+    launched += per_graph;
+    if (hipMemcpyAsync(&ndone, a.ndone, sizeof(int), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) {
+      rc = JKG_EHIP;
+      break;
+    }
+    if (ndone >= count) break;
+    if (launched > cap) {
+      rc = JKG_ENOCONV;
+      break;
+    }
+  }
+  (void)hipEventRecord(c->ev1, c->stream);
+  (void)hipGraphExecDestroy(exec);
+  (void)hipGraphDestroy(graph);
+  if (rc) return rc;
+  HIPCHK(hipEventSynchronize(c->ev1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  batch_finalize_launch(a, dsy, c->stream);
+  HIPCHK(hipGetLastError());
+  std::vector<BState> st(count);
+  HIPCHK(hipMemcpyAsync(st.data(), a.state, sizeof(BState) * count, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(sy, dsy, sizeof(double) * syo, hipMemcpyDeviceToHost, c->stream));
+  {
+    long long yo = 0;
+    for (int i = 0; i < count; ++i) {
+      const long long nn2 = (long long)inst[i].n * inst[i].n;
+      HIPCHK(hipMemcpyAsync(yu + yo, a.mat + inst[i].u, sizeof(double) * nn2, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipMemcpyAsync(zz + yo, a.mat + inst[i].z, sizeof(double) * nn2, hipMemcpyDeviceToHost, c->stream));
+      yo += nn2;
+    }
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < count; ++i) info[i] = st[i].info;
+  o.steps = (int)launched;
+  o.tuned_grid = grid;
+  o.tuned_variant = variant;
+  o.seconds = ms * 1e-3;
+  if (opts) *opts = o;
   return JKG_OK;
 }

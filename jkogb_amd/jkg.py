@@ -54,6 +54,13 @@ class MixedOpts(C.Structure):
                 ("ctl_trace_cap", C.c_int32), ("reserved", C.c_int32 * 7)]
 
 
+class BatchOpts(C.Structure):
+    """jkg_batch_opts."""
+    _fields_ = [("fused", C.c_int32), ("max_steps", C.c_int32), ("grid", C.c_int32), ("variant", C.c_int32),
+                ("steps", C.c_int32), ("tuned_grid", C.c_int32), ("tuned_variant", C.c_int32),
+                ("reserved", C.c_int32), ("seconds", C.c_double), ("tune_seconds", C.c_double)]
+
+
 CTL_NPARAM = 5378
 CTL_REC = 20
 CTL_FEATURES = ["step/16N", "log10(TW/TW0)/20", "log10(TW/TWprev)/2", "SL/N2", "NL/SL", "IT/NN", "walk depth",
@@ -101,6 +108,9 @@ _dsolve_mixed = _sig("jkg_dsolve_mixed", _i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _
 _dweights_mixed = _sig("jkg_dweights_mixed", _i, [_vp, _i, _vp, _vp, _i, _vp])
 _dprofile_mixed = _sig("jkg_dprofile_mixed", _i, [_vp, _i, _i, _vp, _vp, _i, _i, C.POINTER(MixedOpts), _vp,
                                                   C.POINTER(Result)])
+_batch_default = _sig("jkg_batch_opts_default", None, [C.POINTER(BatchOpts)])
+_dsolve_batched = _sig("jkg_dsolve_batched", _i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                                  C.POINTER(BatchOpts)])
 _ctl_grad = _sig("jkg_ctl_grad", _i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp])
 _ctl_fit = _sig("jkg_ctl_fit", _i, [_vp, _i, _vp, _vp, _vp, _i, _i, C.c_float, C.c_float, C.c_uint64, _vp, _vp])
 _pair_index = _sig("jkg_pair_index", C.c_uint32, [_i, _i, _i])
@@ -110,7 +120,7 @@ EXPORTS = ["jkg_create", "jkg_destroy", "jkg_strerror", "jkg_last_hip_error", "j
            "jkg_zsolve", "jkg_srt", "jkg_ncp", "jkg_tru", "jkg_jstep_len", "jkg_dhsvd2", "jkg_zhsvd2",
            "jkg_dweights", "jkg_zweights", "jkg_dbuild", "jkg_dstep", "jkg_dprofile", "jkg_pair_index",
            "jkg_key_weight", "jkg_mixed_opts_default", "jkg_dsolve_mixed", "jkg_dweights_mixed",
-           "jkg_dprofile_mixed", "jkg_ctl_grad", "jkg_ctl_fit"]
+           "jkg_dprofile_mixed", "jkg_ctl_grad", "jkg_ctl_fit", "jkg_batch_opts_default", "jkg_dsolve_batched"]
 
 
 def mixed_opts(**kw):
@@ -269,6 +279,38 @@ class Context:
         if ctr is not None:
             out["ctl"] = ctr[:min(r.steps, ctr.shape[0])].copy()
         return out
+
+    # ---------------------------------------------------------------- batched solver
+    def dsolve_batched(self, Ys, Js, n2s=None, fused=False, max_steps=-1, grid=0, variant=-1):
+        """Independent real HSVDs (djk semantics each) advanced together (jkg_dsolve_batched).
+        Returns (list of dict(YU, ZZ, SY, info) per instance, dict of the run: seconds, steps,
+        grid, variant, tune_seconds)."""
+        ns = np.array([np.asarray(Y).shape[0] for Y in Ys], np.int32)
+        for Y, J, n in zip(Ys, Js, ns):
+            if np.asarray(Y).shape != (n, n) or len(J) != n:
+                raise ValueError("instance shapes")
+        y = np.concatenate([np.asfortranarray(Y, dtype=np.float64).ravel(order="F") for Y in Ys])
+        j = np.concatenate([np.asarray(J, dtype=np.int64) for J in Js])
+        n2a = None if n2s is None else np.ascontiguousarray(n2s, dtype=np.int32)
+        yu = np.zeros_like(y)
+        zz = np.zeros_like(y)
+        sy = np.zeros(int(ns.sum()))
+        info = np.zeros(len(Ys), np.int32)
+        o = BatchOpts()
+        _batch_default(C.byref(o))
+        o.fused, o.max_steps, o.grid, o.variant = (1 if fused else 0), max_steps, grid, variant
+        _chk(_dsolve_batched(self._h, len(Ys), _p(ns), None if n2a is None else _p(n2a), _p(y), _p(j), _p(yu),
+                             _p(zz), _p(sy), _p(info), C.byref(o)), "jkg_dsolve_batched")
+        out, mo, so = [], 0, 0
+        for i, n in enumerate(ns):
+            n = int(n)
+            out.append(dict(YU=yu[mo:mo + n * n].reshape((n, n), order="F"),
+                            ZZ=zz[mo:mo + n * n].reshape((n, n), order="F"), SY=sy[so:so + n].copy(),
+                            info=int(info[i])))
+            mo += n * n
+            so += n
+        return out, dict(seconds=o.seconds, steps=o.steps, grid=o.tuned_grid, variant=o.tuned_variant,
+                         tune_seconds=o.tune_seconds)
 
     # ---------------------------------------------------------------- controller training
     def ctl_grad(self, X, T, params, W=None):
