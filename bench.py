@@ -22,6 +22,11 @@ Printed as one JSON line:
                with the trained step controller (jkogb_amd/ctl_params.npy):
                device time, steps, and the singular values' largest
                relative difference from the plain solver in units of N eps
+  batched      4096 independent HSVDs, N uniform in [16, 256], Gaussian Y,
+               random J, N_2 = N/2, advanced together by one persistent launch
+               per step (jkg_dsolve_batched, tuned grid and variant, tol = eps):
+               device time, against single plain solves of a sample of the
+               instances (projected to all 4096) and their SY difference
   cpu_baseline the reference (oracle/_ref/plain/djk.exe, built from the
                reference's own sources) on the same input, host cores
 
@@ -111,6 +116,38 @@ def timed_steps(ctx, solve, y, j, n2, steps, warmup, **kw):
     return dict(steps=k, ms_per_step=ms, gbs=b / (ms * 1e-3) / 1e9, bytes=b)
 
 
+def batched_run(ctx, count, fused, sample=8):
+    """The batched configuration: count instances, N uniform in [16, 256], solved together with the
+    convergence test (tol = eps), against single plain solves of a sample of them."""
+    from jkogb_amd import inputs
+    rng = np.random.default_rng(4)
+    ns = rng.integers(16, 257, count)
+    ys, js = [], []
+    for i, n in enumerate(ns):
+        y, j = inputs.make("c4", n=int(n), seed=10000 + i)
+        ys.append(y)
+        js.append(j)
+    eps = np.finfo(float).eps
+    outs, run = ctx.dsolve_batched(ys, js, fused=fused, tol=eps)
+    infos = np.array([o["info"] for o in outs])
+    t1, worst = 0.0, 0.0
+    for i in rng.choice(count, sample, replace=False):
+        n = int(ns[i])
+        r = ctx.dsolve(ys[i], js[i], n // 2, fused=fused)
+        t1 += r["seconds"]
+        worst = max(worst, float(np.max(np.abs(outs[i]["SY"] - r["SY"]) / np.abs(r["SY"]))) / (n * eps))
+    single = t1 * count / sample
+    return {"workload": "%d independent HSVDs, N uniform in [16,256] (sum N^2 = %d), Gaussian Y, random J, "
+                        "N_2=N/2, one persistent launch per step, tol=eps" % (count, int(np.sum(ns.astype(np.int64) ** 2))),
+            "seconds": round(run["seconds"], 4), "steps": run["steps"], "grid": run["grid"],
+            "variant": run["variant"], "tune_seconds": round(run["tune_seconds"], 3),
+            "instance_steps": {"min": int(infos.min()), "median": int(np.median(infos)), "max": int(infos.max())},
+            "single_plain_projected_seconds": round(single, 2), "speedup_vs_single_plain": round(single / run["seconds"], 2),
+            "sy_err_N_eps_sampled": round(worst, 4),
+            "note": "single_plain_projected: %d instances solved one at a time by jkg_dsolve (the reference's "
+                    "arithmetic), scaled to %d; sy_err: their largest relative SY difference" % (sample, count)}
+
+
 def stage_profile(prof):
     ps = max(prof["steps"], 1)
     return {s: v / ps for s, v in prof["stage_ms"].items()}, ps
@@ -127,6 +164,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=40.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-full", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batch-count", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the step is single-GPU; see DESIGN.md for how it "
@@ -139,9 +178,11 @@ def main():
     n2 = n // 2
     ctx = jkg.Context(0)
     t0 = time.time()
+    log = lambda msg: print("bench: %s (%.0f s)" % (msg, time.time() - t0), file=sys.stderr, flush=True)
     mixed = timed_steps(ctx, ctx.dsolve_mixed, y, j, n2, args.steps, args.warmup, fused=args.fused)
     plain = timed_steps(ctx, ctx.dsolve, y, j, n2, args.steps, args.warmup, fused=args.fused)
     wall = time.time() - t0
+    log("timed steps: mixed %.3f ms, plain %.3f ms" % (mixed["ms_per_step"], plain["ms_per_step"]))
 
     # per-stage device times (each stage launched and timed with HIP events on the library's stream)
     pm, psm = stage_profile(ctx.dprofile_mixed(y, j, n2, steps=args.profile_steps, fused=args.fused))
@@ -160,6 +201,7 @@ def main():
     gram_flops = 2.0 * npl * nmi * n
     rot_gbs = stage_bytes["rotate"] / (pm["rotate"] * 1e-3) / 1e9
 
+    log("stage profiles")
     full = None
     if not args.no_full:
         yc, jc = inputs.make("c2", n=1024, seed=0)
@@ -179,6 +221,11 @@ def main():
                 "reference_build_spread_N_eps": err(rf),
                 "note": "sy_err: largest relative SY difference against the plain (reference-exact) solver; "
                         "reference_build_spread: the same between the reference's unfused and fused builds"}
+        log("full solves at N=1024")
+    batched = None
+    if not args.no_batched:
+        batched = batched_run(ctx, args.batch_count, args.fused)
+        log("batched: %d instances in %.3f s" % (args.batch_count, batched["seconds"]))
     ctx.close()
 
     cpu = None
@@ -234,6 +281,7 @@ def main():
                               "plain": {s: round(v, 4) for s, v in pp.items()}},
         "cpu_baseline": cpu,
         "full_solve": full,
+        "batched": batched,
         "host_wall_s": round(wall, 3),
     }
     print(json.dumps(out))

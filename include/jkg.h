@@ -219,6 +219,8 @@ typedef struct {
   int32_t max_steps; /* < 0: to convergence; else at most this many steps per instance */
   int32_t grid;      /* workgroups of the persistent launch; 0: tuned */
   int32_t variant;   /* kernel variant 0..3; -1: tuned */
+  double tol;        /* 0: djk's termination (bit-exact per instance); > 0: the mixed solver's
+                        convergence test (see jkg_mixed_opts.tol), results agree to rounding */
   /* results */
   int32_t steps;         /* batched steps launched */
   int32_t tuned_grid;    /* grid used */

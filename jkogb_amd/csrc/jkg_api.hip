@@ -1220,7 +1220,7 @@ extern "C" int jkg_dsolve_batched(jkg_ctx *c, int count, const int32_t *ns, cons
   jkg_batch_opts o;
   if (opts) o = *opts;
   else jkg_batch_opts_default(&o);
-  if (o.grid < 0 || o.variant < -1 || o.variant >= batch_variants()) return -11;
+  if (o.grid < 0 || o.variant < -1 || o.variant >= batch_variants() || !(o.tol >= 0.0)) return -11;
   HIPCHK(hipSetDevice(c->device));
   // layout of the pools
   std::vector<BInst> inst(count);
@@ -1298,6 +1298,7 @@ extern "C" int jkg_dsolve_batched(jkg_ctx *c, int count, const int32_t *ns, cons
   BatchMem m;
   BatchArgs a{};
   a.count = count;
+  a.tol = o.tol;
   a.pair_total = pairs;
   BInst *dinst;
   int rc = JKG_OK;

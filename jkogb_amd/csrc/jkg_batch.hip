@@ -3,7 +3,9 @@
 //
 // Every instance follows the plain solver's arithmetic (the reference's
 // DSTEP_EXEC, src/D/dstep.F90:376-443, with DSTEP_LOOP's termination), so each
-// instance's YU/ZZ/SY equal what djk.exe computes for it alone.  One
+// instance's YU/ZZ/SY equal what djk.exe computes for it alone -- unless the
+// caller asks for the mixed solver's convergence test (tol > 0), which drops
+// the pairs whose off-diagonal entries are negligible against the diagonal.  One
 // workgroup runs a whole step of one instance -- weights (DMAGF2T/DMAGF2H +
 // ABODNDF2), an LDS-ranked radix sort of the keys, the greedy walk (AW_NCP2),
 // the 2x2 solves (DHSVD2) and the fused row/column rotations -- with
@@ -36,7 +38,7 @@ __device__ __forceinline__ void bsync() { __syncthreads(); }
 // ------------------------------------------------------------------ (a) weights
 // The trigonometric pairs: DMAGF2T (src/D/dstep.F90:12-37).
 template <bool F>
-__device__ void b_trig_weights(int n, const double *A, const int *J, uint64_t *keys, uint32_t *vals) {
+__device__ void b_trig_weights(int n, const double *A, const int *J, uint64_t *keys, uint32_t *vals, double tol) {
   const int nn = n * (n - 1) / 2, lda = n;
   for (int k = threadIdx.x; k < nn; k += BT) {
     int p, q;
@@ -45,8 +47,9 @@ __device__ void b_trig_weights(int n, const double *A, const int *J, uint64_t *k
     if (jp != jq) continue;
     const size_t P = p - 1, Q = q - 1;
     const double aqp = A[P * lda + Q], apq = A[Q * lda + P], app = A[P * lda + P], aqq = A[Q * lda + Q];
-    const bool go = (aqp != 0.0) || (apq != 0.0) || sign_neg(app) || sign_neg(aqq) ||
-                    ((jp + jq == -2) ? (aqq < app) : (app < aqq));
+    const bool order = sign_neg(app) || sign_neg(aqq) || ((jp + jq == -2) ? (aqq < app) : (app < aqq));
+    bool go = (aqp != 0.0) || (apq != 0.0) || order;
+    if (tol > 0.0 && go && !order && negligible(aqp, apq, app, aqq, tol)) go = false;
     keys[k] = go ? weight_key(dasum2<F>(aqp, apq)) : KEY_NAN;
     vals[k] = pack_pq(p - 1, q - 1);
   }
@@ -57,7 +60,7 @@ __device__ void b_trig_weights(int n, const double *A, const int *J, uint64_t *k
 // reference's order, staged through LDS HR at a time.
 template <bool F, int HR>
 __device__ void b_hyp_tile(int n, const double *A, const int *J, const int *plus, int npl, const int *minus, int nmi,
-                           int tx, int ty, uint64_t *keys, uint32_t *vals, BLds &S) {
+                           int tx, int ty, uint64_t *keys, uint32_t *vals, BLds &S, double tol) {
   const int lda = n;
   const int a = threadIdx.x / BHT, b = threadIdx.x % BHT;
   const int ia = tx * BHT + a, ib = ty * BHT + b;
@@ -77,7 +80,9 @@ __device__ void b_hyp_tile(int n, const double *A, const int *J, const int *plus
   if (valid) {
     const size_t P = p0, Q = q0;
     const double app = A[P * lda + P], aqp = A[P * lda + Q], apq = A[Q * lda + P], aqq = A[Q * lda + Q];
-    if (aqp != 0.0 || apq != 0.0 || sign_neg(app) || sign_neg(aqq)) {
+    const bool order = sign_neg(app) || sign_neg(aqq);
+    const bool drop = tol > 0.0 && !order && negligible(aqp, apq, app, aqq, tol);
+    if ((aqp != 0.0 || apq != 0.0 || order) && !drop) {
       double a2[4] = {app, aqp, apq, aqq}, u2[4], z2[4];
       int j2[2] = {J[p0], J[q0]};
       const int info = dhsvd2<F>(a2, j2, u2, z2, TH1FIX);
@@ -302,10 +307,11 @@ __device__ void b_step(const BatchArgs &a, int idx, BLds &S) {
   const int s = st->step;
   for (int r = tid; r < n; r += BT) colpiv[r] = -1;
   // (a) weights
-  b_trig_weights<F>(n, A, J, k0, v0);
+  b_trig_weights<F>(n, A, J, k0, v0, a.tol);
   const int ntx = (I.npl + BHT - 1) / BHT, nty = (I.nmi + BHT - 1) / BHT;
   if (I.npl > 0 && I.nmi > 0)
-    for (int t = 0; t < ntx * nty; ++t) b_hyp_tile<F, HR>(n, A, J, plus, I.npl, minus, I.nmi, t % ntx, t / ntx, k0, v0, S);
+    for (int t = 0; t < ntx * nty; ++t)
+      b_hyp_tile<F, HR>(n, A, J, plus, I.npl, minus, I.nmi, t % ntx, t / ntx, k0, v0, S, a.tol);
   bsync();
   // (b) sort
   const int cur = b_sort(nn, k0, k1, v0, v1, S);

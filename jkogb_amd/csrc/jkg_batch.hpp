@@ -25,6 +25,7 @@ struct BState {
 
 struct BatchArgs {
   int count;
+  double tol;            // > 0: the mixed solver's convergence test drops negligible pairs
   int *ndone;            // instances finished (counted once each)
   long long pair_total;  // pairs over all instances (the second key/value buffer starts there)
   const BInst *inst;

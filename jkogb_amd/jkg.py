@@ -57,7 +57,7 @@ class MixedOpts(C.Structure):
 class BatchOpts(C.Structure):
     """jkg_batch_opts."""
     _fields_ = [("fused", C.c_int32), ("max_steps", C.c_int32), ("grid", C.c_int32), ("variant", C.c_int32),
-                ("steps", C.c_int32), ("tuned_grid", C.c_int32), ("tuned_variant", C.c_int32),
+                ("tol", C.c_double), ("steps", C.c_int32), ("tuned_grid", C.c_int32), ("tuned_variant", C.c_int32),
                 ("reserved", C.c_int32), ("seconds", C.c_double), ("tune_seconds", C.c_double)]
 
 
@@ -281,7 +281,7 @@ class Context:
         return out
 
     # ---------------------------------------------------------------- batched solver
-    def dsolve_batched(self, Ys, Js, n2s=None, fused=False, max_steps=-1, grid=0, variant=-1):
+    def dsolve_batched(self, Ys, Js, n2s=None, fused=False, max_steps=-1, grid=0, variant=-1, tol=0.0):
         """Independent real HSVDs (djk semantics each) advanced together (jkg_dsolve_batched).
         Returns (list of dict(YU, ZZ, SY, info) per instance, dict of the run: seconds, steps,
         grid, variant, tune_seconds)."""
@@ -298,7 +298,7 @@ class Context:
         info = np.zeros(len(Ys), np.int32)
         o = BatchOpts()
         _batch_default(C.byref(o))
-        o.fused, o.max_steps, o.grid, o.variant = (1 if fused else 0), max_steps, grid, variant
+        o.fused, o.max_steps, o.grid, o.variant, o.tol = (1 if fused else 0), max_steps, grid, variant, tol
         _chk(_dsolve_batched(self._h, len(Ys), _p(ns), None if n2a is None else _p(n2a), _p(y), _p(j), _p(yu),
                              _p(zz), _p(sy), _p(info), C.byref(o)), "jkg_dsolve_batched")
         out, mo, so = [], 0, 0

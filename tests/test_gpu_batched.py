@@ -92,3 +92,20 @@ def test_batched_rejects_bad_j(ctx):
     with pytest.raises(jkg.JKGError) as e:
         ctx.dsolve_batched(Ys, Js)
     assert e.value.code == jkg.JKG_EJ
+
+
+def test_batched_convergence_test(ctx):
+    # with the mixed solver's convergence test: each instance within the spread of the reference's two
+    # FMA builds (+64 N eps) of its plain solve, in no more steps
+    eps = np.finfo(np.float64).eps
+    Ys, Js = batch(7, 12, 20, 90)
+    outs, _ = ctx.dsolve_batched(Ys, Js, tol=eps)
+    for Y, J, o in zip(Ys, Js, outs):
+        n = Y.shape[0]
+        p = ctx.dsolve(Y, J, n // 2)
+        f = ctx.dsolve(Y, J, n // 2, fused=True)
+        m = ctx.dsolve_mixed(Y, J, n // 2)
+        spread = np.max(np.abs(f["SY"] - p["SY"]) / np.abs(p["SY"]))
+        rel = np.max(np.abs(o["SY"] - p["SY"]) / np.abs(p["SY"]))
+        assert rel <= 4 * spread + 64 * n * eps, (n, rel, spread)
+        assert 0 <= o["info"] <= p["info"]
