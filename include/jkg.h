@@ -200,6 +200,18 @@ int jkg_dweights_mixed(jkg_ctx *ctx, int n, const double *a, const int64_t *j, i
 /* jkg_dprofile_mixed: as jkg_dprofile, for the steps of the mixed solver */
 int jkg_dprofile_mixed(jkg_ctx *ctx, int n, int n2, const double *y, const int64_t *j, int fused, int steps,
                        const jkg_mixed_opts *m, double *stage_ms, jkg_result *res);
+/* The complex mixed solver: as jkg_dsolve_mixed for jkg_zsolve's problem (interleaved
+ * complex y, yu, zz).  The Gram product is O(:,J=+1)^H O(:,J=-1), formed as two real
+ * FP64 matrix-core products over the interleaved rows.  No step controller here. */
+int jkg_zsolve_mixed(jkg_ctx *ctx, int n, int n2, const double *y, const int64_t *j, double *yu, double *zz,
+                     double *sy, const jkg_opts *opts, const jkg_mixed_opts *m, jkg_result *res);
+/* jkg_zweights_mixed: as jkg_zweights, with the weights of the complex mixed solver */
+int jkg_zweights_mixed(jkg_ctx *ctx, int n, const double *a, const int64_t *j, int fused, uint64_t *keys);
+/* jkg_zprofile / jkg_zprofile_mixed: as jkg_dprofile / jkg_dprofile_mixed, for a complex y */
+int jkg_zprofile(jkg_ctx *ctx, int n, int n2, const double *y, const int64_t *j, int fused, int steps,
+                 double *stage_ms, jkg_result *res);
+int jkg_zprofile_mixed(jkg_ctx *ctx, int n, int n2, const double *y, const int64_t *j, int fused, int steps,
+                       const jkg_mixed_opts *m, double *stage_ms, jkg_result *res);
 
 /* ---------------------------------------------------------------------------
  * Batched solver: count independent real HSVDs of different sizes, each with

@@ -165,14 +165,15 @@ static int ensure_buffers(jkg_ctx *c, int n, int cplx, int trace_cap) {
   return JKG_OK;
 }
 
-// Buffers of the Gram-form weights: (O^T O)(plus, minus) and the column norms.
+// Buffers of the Gram-form weights: (O^H O)(plus, minus) (complex: real parts,
+// then imaginary parts) and the column norms, then the diagonal.
 static int ensure_mixed(jkg_ctx *c) {
   DeviceStep &d = c->d;
-  if (c->mixed_n == d.n) return JKG_OK;
-  const size_t n = (size_t)d.n;
+  if (c->mixed_n == d.n) return JKG_OK;  // the buffers are freed with the problem's (size or type change)
+  const size_t n = (size_t)d.n, parts = d.cplx ? 2 : 1;
   void *g = nullptr, *cn = nullptr;
-  if (hipMalloc(&g, sizeof(double) * (n / 2) * (n - n / 2)) != hipSuccess ||
-      hipMalloc(&cn, sizeof(double) * 2 * n) != hipSuccess) {
+  if (hipMalloc(&g, sizeof(double) * parts * (n / 2) * (n - n / 2)) != hipSuccess ||
+      hipMalloc(&cn, sizeof(double) * (1 + parts) * n) != hipSuccess) {
     (void)hipGetLastError();
     if (g) (void)hipFree(g);
     return JKG_ENOMEM;
@@ -594,6 +595,24 @@ extern "C" int jkg_zsolve(jkg_ctx *c, int n, int n2, const double *y, const int6
   return solve_common(c, 1, n, n2, y, j, yu, zz, sy, opts, res);
 }
 
+extern "C" int jkg_zsolve_mixed(jkg_ctx *c, int n, int n2, const double *y, const int64_t *j, double *yu,
+                                double *zz, double *sy, const jkg_opts *opts, const jkg_mixed_opts *mopts,
+                                jkg_result *res) {
+  jkg_mixed_opts m;
+  if (mopts) m = *mopts;
+  else jkg_mixed_opts_default(&m);
+  if (m.weights != JKG_W_PAIR && m.weights != JKG_W_GRAM) return -10;
+  if (!(m.tol >= 0.0) || !(m.threshold >= 0.0 && m.threshold < 1.0)) return -10;
+  if (m.controller < 0 || m.controller > 2 || m.ctl_trace_cap < 0) return -10;
+  if (m.controller || m.fp32_warm != 0 || m.key_bits != 64) return JKG_ENOTSUP;
+  if (m.weights == JKG_W_PAIR && m.tol != 0.0) return JKG_ENOTSUP;
+  MixedCfg mc;
+  mc.wmode = m.weights == JKG_W_GRAM ? 1 : 0;
+  mc.tol = m.tol;
+  mc.tau = m.threshold;
+  return solve_common(c, 1, n, n2, y, j, yu, zz, sy, opts, res, mc);
+}
+
 // ---------------------------------------------------------------- step level
 template <bool F>
 __global__ void k_dhsvd2_batch(int count, const double *a, const int *j, double th1, double *u, double *z,
@@ -734,6 +753,23 @@ extern "C" int jkg_zweights(jkg_ctx *c, int n, const double *a, const int64_t *j
   if (rc) return rc;
   step_begin_launch(c->d, c->stream);
   weights_launch(c->d, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(keys, c->d.sort.k0, sizeof(uint64_t) * c->d.nn, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return JKG_OK;
+}
+
+extern "C" int jkg_zweights_mixed(jkg_ctx *c, int n, const double *a, const int64_t *j, int fused,
+                                  uint64_t *keys) {
+  if (!c) return -1;
+  int rc = prepare_single(c, n, 0, a, j, fused, 1);
+  if (rc) return rc;
+  rc = ensure_mixed(c);
+  if (rc) return rc;
+  c->d.wmode = 1;
+  step_begin_launch(c->d, c->stream);
+  weights_launch(c->d, c->stream);
+  c->d.wmode = 0;
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(keys, c->d.sort.k0, sizeof(uint64_t) * c->d.nn, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -947,14 +983,14 @@ extern "C" int jkg_ncp(jkg_ctx *c, int n, int nn, int nm, jkg_aw *dz, int n2, in
 
 // ---------------------------------------------------------------- profiling
 static int profile_common(jkg_ctx *c, int n, int n2, const double *y, const int64_t *j, int fused, int steps,
-                          double *stage_ms, jkg_result *res, int wmode, double tol = 0.0) {
+                          double *stage_ms, jkg_result *res, int wmode, double tol = 0.0, int cplx = 0) {
   if (!c) return -1;
   if (n < 3) return -2;
   if (steps < 1) return -7;
   memset(res, 0, sizeof(*res));
   HIPCHK(hipSetDevice(c->device));
   n2 = jkg_jstep_len(n, n2);
-  int rc = ensure_buffers(c, n, 0, steps + 1);
+  int rc = ensure_buffers(c, n, cplx, steps + 1);
   if (rc) return rc;
   c->d.wmode = wmode;
   c->d.neg_tol = tol;
@@ -1030,6 +1066,23 @@ extern "C" int jkg_dprofile_mixed(jkg_ctx *c, int n, int n2, const double *y, co
   if (m.fp32_warm != 0 || m.key_bits != 64 || m.controller != 0 || m.threshold != 0.0) return JKG_ENOTSUP;
   if (m.weights == JKG_W_PAIR && m.tol != 0.0) return JKG_ENOTSUP;
   return profile_common(c, n, n2, y, j, fused, steps, stage_ms, res, m.weights == JKG_W_GRAM ? 1 : 0, m.tol);
+}
+
+extern "C" int jkg_zprofile(jkg_ctx *c, int n, int n2, const double *y, const int64_t *j, int fused, int steps,
+                            double *stage_ms, jkg_result *res) {
+  return profile_common(c, n, n2, y, j, fused, steps, stage_ms, res, 0, 0.0, 1);
+}
+
+extern "C" int jkg_zprofile_mixed(jkg_ctx *c, int n, int n2, const double *y, const int64_t *j, int fused,
+                                  int steps, const jkg_mixed_opts *mopts, double *stage_ms, jkg_result *res) {
+  jkg_mixed_opts m;
+  if (mopts) m = *mopts;
+  else jkg_mixed_opts_default(&m);
+  if (m.weights != JKG_W_PAIR && m.weights != JKG_W_GRAM) return -8;
+  if (!(m.tol >= 0.0)) return -8;
+  if (m.fp32_warm != 0 || m.key_bits != 64 || m.controller != 0 || m.threshold != 0.0) return JKG_ENOTSUP;
+  if (m.weights == JKG_W_PAIR && m.tol != 0.0) return JKG_ENOTSUP;
+  return profile_common(c, n, n2, y, j, fused, steps, stage_ms, res, m.weights == JKG_W_GRAM ? 1 : 0, m.tol, 1);
 }
 
 // ---------------------------------------------------------------- controller training

@@ -32,9 +32,13 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 // buffers: the next stage's global loads are in flight during the MFMAs).
 // MFMA f64 16x16x4 lane layout: A[i = l % 16][k = l / 16], B[k = l / 16][j =
 // l % 16], D item t: [i = 4 t + l / 16][j = l % 16].
+// Complex columns (CX > 0) are read as 2k interleaved doubles (re, im per row);
+// the diagonal of O is then the pair of flat rows 2c, 2c + 1.  Over the flat
+// rows, sum X Y = Re(X^H Y) (CX = 1), and with Y's pairs turned to (im, -re),
+// sum X Y' = Im(X^H Y) (CX = 2): the complex product is two real ones.
 constexpr int GB_M = 128, GB_N = 128, GB_K = 16, GB_LD = GB_M + 4;
 
-template <bool ZD>
+template <bool ZD, int CX>
 __global__ void __launch_bounds__(256) k_gemm_tn(int k, const double *__restrict__ X, int ldx,
                                                  const int *__restrict__ xcols, int m, const double *__restrict__ Y,
                                                  int ldy, const int *__restrict__ ycols, int nc, double *__restrict__ C,
@@ -57,8 +61,17 @@ __global__ void __launch_bounds__(256) k_gemm_tn(int k, const double *__restrict
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int r = r0 + lh + u;
-      xr[u] = (xcol >= 0 && r < k && !(ZD && r == xcol)) ? xp[r] : 0.0;
-      yr[u] = (ycol >= 0 && r < k && !(ZD && r == ycol)) ? yp[r] : 0.0;
+      const int rd = CX ? (r >> 1) : r;
+      xr[u] = (xcol >= 0 && r < k && !(ZD && rd == xcol)) ? xp[r] : 0.0;
+      yr[u] = (ycol >= 0 && r < k && !(ZD && rd == ycol)) ? yp[r] : 0.0;
+    }
+    if (CX == 2) {  // rows r0 + lh + u are even for even u: (re, im) -> (im, -re)
+#pragma unroll
+      for (int u = 0; u < 8; u += 2) {
+        const double re = yr[u];
+        yr[u] = yr[u + 1];
+        yr[u + 1] = -re;
+      }
     }
   };
   auto sstore = [&](int buf) {
@@ -109,15 +122,15 @@ __global__ void __launch_bounds__(256) k_gemm_tn(int k, const double *__restrict
 
 void gemm_tn_launch(int k, const double *X, int ldx, const int *xcols, int m, const double *Y, int ldy,
                     const int *ycols, int nc, double *C, long csi, long csj, bool zero_diag, const StepState *st,
-                    hipStream_t s) {
+                    hipStream_t s, int cx) {  // cx: 0 real, 1 / 2 real / imaginary part of X^H Y
   if (m <= 0 || nc <= 0) return;
   dim3 grid((nc + GB_N - 1) / GB_N, (m + GB_M - 1) / GB_M);
-  if (zero_diag)
-    hipLaunchKernelGGL(k_gemm_tn<true>, grid, dim3(256), 0, s, k, X, ldx, xcols, m, Y, ldy, ycols, nc, C, csi, csj,
-                       st);
-  else
-    hipLaunchKernelGGL(k_gemm_tn<false>, grid, dim3(256), 0, s, k, X, ldx, xcols, m, Y, ldy, ycols, nc, C, csi,
-                       csj, st);
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, k, X, ldx, xcols, m, Y, ldy, ycols, nc, C, csi, csj, st);
+  };
+  if (cx == 1) go(zero_diag ? k_gemm_tn<true, 1> : k_gemm_tn<false, 1>);
+  else if (cx == 2) go(zero_diag ? k_gemm_tn<true, 2> : k_gemm_tn<false, 2>);
+  else go(zero_diag ? k_gemm_tn<true, 0> : k_gemm_tn<false, 0>);
 }
 
 // ------------------------------------------------------------------ column norms
@@ -219,7 +232,7 @@ void gram_hyp_weights_launch(DeviceStep &d, hipStream_t s) {
   const double *A = (const double *)d.A;
   hipLaunchKernelGGL(k_colnorm_off, dim3((d.n + 3) / 4), dim3(256), 0, s, d.n, A, d.lda, d.coln, d.coln + d.n,
                      d.st);
-  gemm_tn_launch(d.n, A, d.lda, d.plus, d.npl, A, d.lda, d.minus, d.nmi, d.gram, (long)d.nmi, 1L, true, d.st, s);
+  gemm_tn_launch(d.n, A, d.lda, d.plus, d.npl, A, d.lda, d.minus, d.nmi, d.gram, (long)d.nmi, 1L, true, d.st, s, 0);
   dim3 g((d.nmi + WT - 1) / WT, (d.npl + WT - 1) / WT);
   if (d.fused)
     hipLaunchKernelGGL(k_hyp_weights_gram<true>, g, dim3(256), 0, s, d.n, A, d.lda, d.J, d.plus, d.npl, d.minus,
@@ -227,6 +240,130 @@ void gram_hyp_weights_launch(DeviceStep &d, hipStream_t s) {
   else
     hipLaunchKernelGGL(k_hyp_weights_gram<false>, g, dim3(256), 0, s, d.n, A, d.lda, d.J, d.plus, d.npl, d.minus,
                        d.nmi, d.gram, d.nmi, d.coln, d.coln + d.n, d.sort.k0, d.sort.v0, d.st, d.neg_tol);
+}
+
+// ------------------------------------------------------------------ complex
+// The complex hyperbolic weight (ZMAGF2H + ABODNZF2, src/Z/zstep.F90:43-74,
+// src/Z/ztransf.F90:452-637) in the same form: with x' = W11 x + W21 y,
+// y' = W12 x + W22 y (W the pair's complex column transform) and
+// gxy = sum conj(x_r) y_r,
+//   W = d0 + gxx (1 - |W11|^2 - |W12|^2) + gyy (1 - |W21|^2 - |W22|^2)
+//          - 2 Re((conj(W11) W21 + conj(W12) W22) gxy).
+// G = O(:,plus)^H O(:,minus) (real and imaginary parts as two products); for a
+// pair whose J = +1 column is the larger index, gxy = conj(G).
+
+// c[col] = sum_{r != col} |A[r, col]|^2 and dg[col] = A[col, col] (complex), one wave per column.
+__global__ void __launch_bounds__(256) k_zcolnorm_off(int n, const dz *__restrict__ A, int lda,
+                                                      double *__restrict__ c, dz *__restrict__ dg,
+                                                      const StepState *st) {
+  if (!st->active) return;
+  const int col = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (col >= n) return;
+  const dz *a = A + (size_t)col * lda;
+  double s = 0.0;
+  for (int r = lane; r < n; r += 64) {
+    const dz v = a[r];
+    if (r == col) dg[col] = v;
+    else s = __builtin_fma(v.im, v.im, __builtin_fma(v.re, v.re, s));
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (lane == 0) c[col] = s;
+}
+
+constexpr int ZWT = 32;
+
+template <bool F>
+__global__ void __launch_bounds__(256) k_zhyp_weights_gram(int n, const dz *__restrict__ A, int lda,
+                                                           const int *__restrict__ J, const int *__restrict__ plus,
+                                                           int npl, const int *__restrict__ minus, int nmi,
+                                                           const double *__restrict__ Gre,
+                                                           const double *__restrict__ Gim, int ldg,
+                                                           const double *__restrict__ coln,
+                                                           const dz *__restrict__ dg, uint64_t *__restrict__ keys,
+                                                           uint32_t *__restrict__ vals, StepState *st, double tol) {
+  if (!st->active) return;
+  __shared__ dz E1[ZWT][ZWT + 1];  // E1[a][b] = A(minus b, plus a)
+  __shared__ dz E2[ZWT][ZWT + 1];  // E2[b][a] = A(plus a, minus b)
+  __shared__ int pc[ZWT], mc[ZWT];
+  const int i0 = blockIdx.y * ZWT, j0 = blockIdx.x * ZWT, tid = threadIdx.x;
+  if (tid < ZWT) pc[tid] = (i0 + tid < npl) ? plus[i0 + tid] : -1;
+  else if (tid < 2 * ZWT) mc[tid - ZWT] = (j0 + tid - ZWT < nmi) ? minus[j0 + tid - ZWT] : -1;
+  __syncthreads();
+  for (int e = tid; e < ZWT * ZWT; e += 256) {
+    const int hi = e / ZWT, lo = e % ZWT;
+    const int c1 = pc[hi], r1 = mc[lo];
+    E1[hi][lo] = (c1 >= 0 && r1 >= 0) ? A[(size_t)c1 * lda + r1] : zmk(0.0, 0.0);
+    const int c2 = mc[hi], r2 = pc[lo];
+    E2[hi][lo] = (c2 >= 0 && r2 >= 0) ? A[(size_t)c2 * lda + r2] : zmk(0.0, 0.0);
+  }
+  __syncthreads();
+  unsigned nfull = 0;
+  for (int e = tid; e < ZWT * ZWT; e += 256) {
+    const int a = e / ZWT, b = e % ZWT;
+    const int cp = pc[a], cm = mc[b];
+    if (cp < 0 || cm < 0) continue;
+    const int p0 = cp < cm ? cp : cm, q0 = cp < cm ? cm : cp;
+    const dz e1 = E1[a][b], e2 = E2[b][a];  // A(cm, cp), A(cp, cm)
+    const dz aqp = cp < cm ? e1 : e2, apq = cp < cm ? e2 : e1;
+    const dz app = dg[p0], aqq = dg[q0];
+    uint64_t key = KEY_NAN;
+    const bool diag = app.im != 0.0 || aqq.im != 0.0 || sign_neg(app.re) || sign_neg(aqq.re);
+    const bool drop = tol > 0.0 && !diag && negligible(absz(aqp), absz(apq), app.re, aqq.re, tol);
+    if ((!zeq0(aqp) || !zeq0(apq) || diag) && !drop) {
+      dz a2[4] = {app, aqp, apq, aqq}, u2[4], z2[4];
+      int j2[2] = {J[p0], J[q0]};
+      const int info = zhsvd2<F>(a2, j2, u2, z2, TH1FIX);
+      if (info < 0) key = KEY_NAN;
+      else if ((info & 1) == 0) key = weight_key(0.0);
+      else if ((info & 4) == 0) key = weight_key(dasum4<F>(aqp.re, aqp.im, apq.re, apq.im));
+      else {
+        ++nfull;
+        const dz aq = (info & 8) ? a2[1] : zmk(0.0, 0.0), ap = (info & 8) ? a2[2] : zmk(0.0, 0.0);
+        double d = fmad<F>(aqp.re - aq.re, aqp.re + aq.re, 0.0);
+        d = fmad<F>(aqp.im - aq.im, aqp.im + aq.im, d);
+        d = fmad<F>(apq.re - ap.re, apq.re + ap.re, d);
+        d = fmad<F>(apq.im - ap.im, apq.im + ap.im, d);
+        const double gxx = coln[p0] - (aqp.re * aqp.re + aqp.im * aqp.im);
+        const double gyy = coln[q0] - (apq.re * apq.re + apq.im * apq.im);
+        const size_t gi = (size_t)(i0 + a) * ldg + j0 + b;
+        const double gr = Gre[gi], gim = cp < cm ? Gim[gi] : -Gim[gi];
+        const dz w11 = z2[0], w21 = z2[1], w12 = z2[2], w22 = z2[3];
+        const double n11 = w11.re * w11.re + w11.im * w11.im, n12 = w12.re * w12.re + w12.im * w12.im;
+        const double n21 = w21.re * w21.re + w21.im * w21.im, n22 = w22.re * w22.re + w22.im * w22.im;
+        // s = conj(W11) W21 + conj(W12) W22; Re(s gxy)
+        const double sr = w11.re * w21.re + w11.im * w21.im + w12.re * w22.re + w12.im * w22.im;
+        const double si = w11.re * w21.im - w11.im * w21.re + w12.re * w22.im - w12.im * w22.re;
+        const double w = d + gxx * (1.0 - n11 - n12) + gyy * (1.0 - n21 - n22) - 2.0 * (sr * gr - si * gim);
+        key = (w < -1.7976931348623157e308 || !(w == w)) ? KEY_NAN : weight_key(w);
+      }
+    }
+    const uint32_t k = pair_index(n, p0 + 1, q0 + 1);
+    keys[k] = key;
+    vals[k] = pack_pq(p0, q0);
+  }
+  for (int o = 32; o > 0; o >>= 1) nfull += __shfl_xor(nfull, o);
+  if ((tid & 63) == 0 && nfull) atomicAdd((unsigned long long *)&st->acc_abod, (unsigned long long)nfull);
+}
+
+void zgram_hyp_weights_launch(DeviceStep &d, hipStream_t s) {
+  if (d.npl <= 0 || d.nmi <= 0) return;
+  const dz *A = (const dz *)d.A;
+  double *Gre = d.gram, *Gim = d.gram + (size_t)d.npl * d.nmi;
+  dz *dg = (dz *)(d.coln + d.n);
+  hipLaunchKernelGGL(k_zcolnorm_off, dim3((d.n + 3) / 4), dim3(256), 0, s, d.n, A, d.lda, d.coln, dg, d.st);
+  const double *X = (const double *)d.A;
+  gemm_tn_launch(2 * d.n, X, 2 * d.lda, d.plus, d.npl, X, 2 * d.lda, d.minus, d.nmi, Gre, (long)d.nmi, 1L, true,
+                 d.st, s, 1);
+  gemm_tn_launch(2 * d.n, X, 2 * d.lda, d.plus, d.npl, X, 2 * d.lda, d.minus, d.nmi, Gim, (long)d.nmi, 1L, true,
+                 d.st, s, 2);
+  dim3 g((d.nmi + ZWT - 1) / ZWT, (d.npl + ZWT - 1) / ZWT);
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, g, dim3(256), 0, s, d.n, A, d.lda, d.J, d.plus, d.npl, d.minus, d.nmi, Gre, Gim, d.nmi,
+                       d.coln, dg, d.sort.k0, d.sort.v0, d.st, d.neg_tol);
+  };
+  if (d.fused) go(k_zhyp_weights_gram<true>);
+  else go(k_zhyp_weights_gram<false>);
 }
 
 }  // namespace jkg

@@ -472,4 +472,22 @@ __device__ __forceinline__ double dasum2(double x1, double x2) {
   return fmad<F>(a, (a / b), b) * b;
 }
 
+// DASUM4, src/Z/ztransf.F90:18-75: sort the magnitudes, sum squares upwards
+template <bool F>
+__device__ __forceinline__ double dasum4(double x1, double x2, double x3, double x4) {
+  double x[4] = {__builtin_fabs(x1), __builtin_fabs(x2), __builtin_fabs(x3), __builtin_fabs(x4)}, y[4];
+  if (x[0] <= x[1]) { y[0] = x[0]; y[1] = x[1]; } else { y[0] = x[1]; y[1] = x[0]; }
+  if (x[2] <= x[3]) { y[2] = x[2]; y[3] = x[3]; } else { y[2] = x[3]; y[3] = x[2]; }
+  if (y[0] <= y[2]) { x[0] = y[0]; x[1] = y[2]; } else { x[0] = y[2]; x[1] = y[0]; }
+  if (y[1] <= y[3]) { x[2] = y[1]; x[3] = y[3]; } else { x[2] = y[3]; x[3] = y[1]; }
+  y[0] = x[0];
+  if (x[1] <= x[2]) { y[1] = x[1]; y[2] = x[2]; } else { y[1] = x[2]; y[2] = x[1]; }
+  y[3] = x[3];
+  double s = y[0] * y[0];
+  s = fmad<F>(y[1], y[1], s);
+  s = fmad<F>(y[2], y[2], s);
+  s = fmad<F>(y[3], y[3], s);
+  return s;
+}
+
 }  // namespace jkg

@@ -134,10 +134,11 @@ void step_end_launch(DeviceStep &d, hipStream_t s);
 //   C[i * csi + j * csj] = sum_{r < k} X[r, xc(i)] * Y[r, yc(j)]
 // (xc(i) = xcols ? xcols[i] : i; zero_diag reads X[c, c] and Y[c, c] as 0).
 void gram_hyp_weights_launch(DeviceStep &d, hipStream_t s);
+void zgram_hyp_weights_launch(DeviceStep &d, hipStream_t s);
 // the step controller (jkg_mlp.hip): sets StepState::n2 and ::tau for this step
 void controller_launch(DeviceStep &d, hipStream_t s);
 void gemm_tn_launch(int k, const double *X, int ldx, const int *xcols, int m, const double *Y, int ldy,
                     const int *ycols, int nc, double *C, long csi, long csj, bool zero_diag, const StepState *st,
-                    hipStream_t s);
+                    hipStream_t s, int cx = 0);
 
 }  // namespace jkg

@@ -11,24 +11,6 @@
 
 namespace jkg {
 
-// DASUM4, src/Z/ztransf.F90:18-75: sort the magnitudes, sum squares upwards
-template <bool F>
-__device__ __forceinline__ double dasum4(double x1, double x2, double x3, double x4) {
-  double x[4] = {__builtin_fabs(x1), __builtin_fabs(x2), __builtin_fabs(x3), __builtin_fabs(x4)}, y[4];
-  if (x[0] <= x[1]) { y[0] = x[0]; y[1] = x[1]; } else { y[0] = x[1]; y[1] = x[0]; }
-  if (x[2] <= x[3]) { y[2] = x[2]; y[3] = x[3]; } else { y[2] = x[3]; y[3] = x[2]; }
-  if (y[0] <= y[2]) { x[0] = y[0]; x[1] = y[2]; } else { x[0] = y[2]; x[1] = y[0]; }
-  if (y[1] <= y[3]) { x[2] = y[1]; x[3] = y[3]; } else { x[2] = y[3]; x[3] = y[1]; }
-  y[0] = x[0];
-  if (x[1] <= x[2]) { y[1] = x[1]; y[2] = x[2]; } else { y[1] = x[2]; y[2] = x[1]; }
-  y[3] = x[3];
-  double s = y[0] * y[0];
-  s = fmad<F>(y[1], y[1], s);
-  s = fmad<F>(y[2], y[2], s);
-  s = fmad<F>(y[3], y[3], s);
-  return s;
-}
-
 // a complex ratio b/a as the reference forms it: real divisor -> ZDDIV
 template <bool F>
 __device__ __forceinline__ dz zratio(dz num, dz den) {
@@ -95,7 +77,8 @@ __device__ __forceinline__ void zrot_ab_apply(const ZRot &R, dz &x, dz &y) {
 template <bool F>
 __global__ void __launch_bounds__(256) k_ztrig_weights(int n, const dz *__restrict__ A, int lda,
                                                         const int *__restrict__ J, uint64_t *__restrict__ keys,
-                                                        uint32_t *__restrict__ vals, const StepState *st, int nn) {
+                                                        uint32_t *__restrict__ vals, const StepState *st, int nn,
+                                                        double tol) {
   if (!st->active) return;
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < (uint32_t)nn; k += gridDim.x * blockDim.x) {
     int p, q;
@@ -106,7 +89,11 @@ __global__ void __launch_bounds__(256) k_ztrig_weights(int n, const dz *__restri
     const dz aqp = A[P * lda + Q], apq = A[Q * lda + P], app = A[P * lda + P], aqq = A[Q * lda + Q];
     const bool off = !zeq0(aqp) || !zeq0(apq) || app.im != 0.0 || aqq.im != 0.0 || sign_neg(app.re) ||
                      sign_neg(aqq.re);
-    const bool go = off || ((jp + jq == -2) ? (aqq.re < app.re) : (app.re < aqq.re));
+    const bool diag = app.im != 0.0 || aqq.im != 0.0 || sign_neg(app.re) || sign_neg(aqq.re);
+    const bool order = (jp + jq == -2) ? (aqq.re < app.re) : (app.re < aqq.re);
+    bool go = off || order;
+    // tol > 0 (mixed solver only): the convergence test, as for the real pairs
+    if (tol > 0.0 && go && !diag && !order && negligible(absz(aqp), absz(apq), app.re, aqq.re, tol)) go = false;
     keys[k] = go ? weight_key(dasum4<F>(aqp.re, aqp.im, apq.re, apq.im)) : KEY_NAN;
     vals[k] = pack_pq(p - 1, q - 1);
   }
@@ -373,12 +360,17 @@ size_t zrot_bytes() { return sizeof(ZRot); }
 
 void zweights_launch(DeviceStep &d, hipStream_t s) {
   const dz *A = (const dz *)d.A;
+  const double tol = d.wmode ? d.neg_tol : 0.0;
   if (d.fused)
     hipLaunchKernelGGL(k_ztrig_weights<true>, dim3(zgrid(d.nn, 256, 65536)), dim3(256), 0, s, d.n, A, d.lda, d.J,
-                       d.sort.k0, d.sort.v0, d.st, d.nn);
+                       d.sort.k0, d.sort.v0, d.st, d.nn, tol);
   else
     hipLaunchKernelGGL(k_ztrig_weights<false>, dim3(zgrid(d.nn, 256, 65536)), dim3(256), 0, s, d.n, A, d.lda, d.J,
-                       d.sort.k0, d.sort.v0, d.st, d.nn);
+                       d.sort.k0, d.sort.v0, d.st, d.nn, tol);
+  if (d.wmode == 1) {
+    zgram_hyp_weights_launch(d, s);
+    return;
+  }
   if (d.npl > 0 && d.nmi > 0) {
     dim3 g((d.npl + ZHT - 1) / ZHT, (d.nmi + ZHT - 1) / ZHT);
     if (d.fused)

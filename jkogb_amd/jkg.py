@@ -108,6 +108,12 @@ _dsolve_mixed = _sig("jkg_dsolve_mixed", _i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _
 _dweights_mixed = _sig("jkg_dweights_mixed", _i, [_vp, _i, _vp, _vp, _i, _vp])
 _dprofile_mixed = _sig("jkg_dprofile_mixed", _i, [_vp, _i, _i, _vp, _vp, _i, _i, C.POINTER(MixedOpts), _vp,
                                                   C.POINTER(Result)])
+_zsolve_mixed = _sig("jkg_zsolve_mixed", _i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, C.POINTER(Opts),
+                                              C.POINTER(MixedOpts), C.POINTER(Result)])
+_zweights_mixed = _sig("jkg_zweights_mixed", _i, [_vp, _i, _vp, _vp, _i, _vp])
+_zprofile = _sig("jkg_zprofile", _i, [_vp, _i, _i, _vp, _vp, _i, _i, _vp, C.POINTER(Result)])
+_zprofile_mixed = _sig("jkg_zprofile_mixed", _i, [_vp, _i, _i, _vp, _vp, _i, _i, C.POINTER(MixedOpts), _vp,
+                                                  C.POINTER(Result)])
 _batch_default = _sig("jkg_batch_opts_default", None, [C.POINTER(BatchOpts)])
 _dsolve_batched = _sig("jkg_dsolve_batched", _i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                                   C.POINTER(BatchOpts)])
@@ -120,7 +126,8 @@ EXPORTS = ["jkg_create", "jkg_destroy", "jkg_strerror", "jkg_last_hip_error", "j
            "jkg_zsolve", "jkg_srt", "jkg_ncp", "jkg_tru", "jkg_jstep_len", "jkg_dhsvd2", "jkg_zhsvd2",
            "jkg_dweights", "jkg_zweights", "jkg_dbuild", "jkg_dstep", "jkg_dprofile", "jkg_pair_index",
            "jkg_key_weight", "jkg_mixed_opts_default", "jkg_dsolve_mixed", "jkg_dweights_mixed",
-           "jkg_dprofile_mixed", "jkg_ctl_grad", "jkg_ctl_fit", "jkg_batch_opts_default", "jkg_dsolve_batched"]
+           "jkg_dprofile_mixed", "jkg_ctl_grad", "jkg_ctl_fit", "jkg_batch_opts_default", "jkg_dsolve_batched",
+           "jkg_zsolve_mixed", "jkg_zweights_mixed", "jkg_zprofile", "jkg_zprofile_mixed"]
 
 
 def mixed_opts(**kw):
@@ -238,16 +245,19 @@ class Context:
 
     def dsolve_mixed(self, Y, J, n2=0, fused=False, max_steps=-1, trace_cap=None, steps_per_sync=0,
                      ctl_params=None, ctl_table=None, ctl_trace=False, **mixed):
-        """The mixed solver (jkg_dsolve_mixed): same outputs as dsolve, agreeing to rounding.
-        ctl_params (JKG_CTL_NPARAM floats): the MLP controller; ctl_table (8 floats): a phase
-        table; ctl_trace: also return the per-step controller records as out["ctl"]."""
-        Y = np.asfortranarray(Y, dtype=np.float64)
+        """The mixed solver (jkg_dsolve_mixed, or jkg_zsolve_mixed for a complex Y): same outputs
+        as dsolve, agreeing to rounding.  ctl_params (JKG_CTL_NPARAM floats): the MLP controller;
+        ctl_table (8 floats): a phase table; ctl_trace: also return the per-step controller
+        records as out["ctl"]."""
+        cplx = np.iscomplexobj(Y)
+        dt = np.complex128 if cplx else np.float64
+        Y = np.asfortranarray(Y, dtype=dt)
         n = Y.shape[0]
         if Y.shape != (n, n):
             raise ValueError("Y must be square")
         J64 = np.ascontiguousarray(J, dtype=np.int64)
-        YU = np.zeros((n, n), order="F")
-        ZZ = np.zeros((n, n), order="F")
+        YU = np.zeros((n, n), dt, order="F")
+        ZZ = np.zeros((n, n), dt, order="F")
         SY = np.zeros(n)
         if trace_cap is None:
             trace_cap = 64 * n + 64 if max_steps < 0 else max_steps + 1
@@ -272,8 +282,9 @@ class Context:
             ctr = np.zeros((max(trace_cap, 1), CTL_REC), np.float32)
             m.ctl_trace, m.ctl_trace_cap = ctr.ctypes.data, ctr.shape[0]
         r = Result()
-        _chk(_dsolve_mixed(self._h, n, n2, _p(Y), _p(J64), _p(YU), _p(ZZ), _p(SY), C.byref(o), C.byref(m),
-                           C.byref(r)), "jkg_dsolve_mixed")
+        f = _zsolve_mixed if cplx else _dsolve_mixed
+        _chk(f(self._h, n, n2, _p(Y), _p(J64), _p(YU), _p(ZZ), _p(SY), C.byref(o), C.byref(m), C.byref(r)),
+             "jkg_zsolve_mixed" if cplx else "jkg_dsolve_mixed")
         out = dict(YU=YU, ZZ=ZZ, SY=SY, info=r.info, trace=trace[:min(r.steps, trace_cap)], tt=r.tt,
                    pn=(r.npos, r.nneg), n2=r.n2, seconds=r.seconds, steps=r.steps, **_counts(r))
         if ctr is not None:
@@ -340,33 +351,39 @@ class Context:
 
     def dprofile_mixed(self, Y, J, n2=0, steps=4, fused=False, **mixed):
         """As dprofile, for the steps of the mixed solver."""
-        Y = np.asfortranarray(Y, dtype=np.float64)
+        cplx = np.iscomplexobj(Y)
+        Y = np.asfortranarray(Y, dtype=np.complex128 if cplx else np.float64)
         n = Y.shape[0]
         ms = np.zeros(len(STAGES))
         r = Result()
         m = mixed_opts(**mixed)
-        _chk(_dprofile_mixed(self._h, n, n2, _p(Y), _p(np.ascontiguousarray(J, dtype=np.int64)), 1 if fused else 0,
-                             steps, C.byref(m), _p(ms), C.byref(r)), "jkg_dprofile_mixed")
+        f = _zprofile_mixed if cplx else _dprofile_mixed
+        _chk(f(self._h, n, n2, _p(Y), _p(np.ascontiguousarray(J, dtype=np.int64)), 1 if fused else 0, steps,
+               C.byref(m), _p(ms), C.byref(r)), "jkg_zprofile_mixed" if cplx else "jkg_dprofile_mixed")
         return dict(stage_ms=dict(zip(STAGES, ms.tolist())), steps=r.steps, n2=r.n2, seconds=r.seconds,
                     **_counts(r))
 
     def dweights_mixed(self, A, J, fused=False):
         """Sort keys of the mixed solver's pair weights, indexed by pair_index."""
-        A = np.asfortranarray(A, dtype=np.float64)
+        cplx = np.iscomplexobj(A)
+        A = np.asfortranarray(A, dtype=np.complex128 if cplx else np.float64)
         n = A.shape[0]
         keys = np.zeros(n * (n - 1) // 2, np.uint64)
-        _chk(_dweights_mixed(self._h, n, _p(A), _p(np.ascontiguousarray(J, dtype=np.int64)), 1 if fused else 0,
-                             _p(keys)), "jkg_dweights_mixed")
+        f = _zweights_mixed if cplx else _dweights_mixed
+        _chk(f(self._h, n, _p(A), _p(np.ascontiguousarray(J, dtype=np.int64)), 1 if fused else 0, _p(keys)),
+             "jkg_zweights_mixed" if cplx else "jkg_dweights_mixed")
         return keys
 
     def dprofile(self, Y, J, n2=0, steps=4, fused=False):
         """Per-stage device time (ms summed over `steps` steps from Y) and work counts."""
-        Y = np.asfortranarray(Y, dtype=np.float64)
+        cplx = np.iscomplexobj(Y)
+        Y = np.asfortranarray(Y, dtype=np.complex128 if cplx else np.float64)
         n = Y.shape[0]
         ms = np.zeros(len(STAGES))
         r = Result()
-        _chk(_dprofile(self._h, n, n2, _p(Y), _p(np.ascontiguousarray(J, dtype=np.int64)), 1 if fused else 0,
-                       steps, _p(ms), C.byref(r)), "jkg_dprofile")
+        f = _zprofile if cplx else _dprofile
+        _chk(f(self._h, n, n2, _p(Y), _p(np.ascontiguousarray(J, dtype=np.int64)), 1 if fused else 0, steps,
+               _p(ms), C.byref(r)), "jkg_zprofile" if cplx else "jkg_dprofile")
         return dict(stage_ms=dict(zip(STAGES, ms.tolist())), steps=r.steps, n2=r.n2, seconds=r.seconds,
                     **_counts(r))
 
