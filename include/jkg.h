@@ -150,9 +150,11 @@ int jkg_dprofile(jkg_ctx *ctx, int n, int n2, const double *y, const int64_t *j,
 #define JKG_W_GRAM 1 /* Gram form (default) */
 typedef struct {
   int32_t weights;    /* JKG_W_PAIR or JKG_W_GRAM */
-  int32_t fp32_warm;  /* 1: FP32 steps until the off-norm stalls or falls below switch_tol, then FP64 */
+  int32_t fp32_warm;  /* 1 (complex solver): FP32 steps until the off-norm falls below switch_tol or
+                         stalls, then FP64 (see jkg_warm_info) */
   double switch_tol;  /* relative off-norm that ends the FP32 phase (default 1e-5) */
-  int32_t key_bits;   /* 64: full sort keys (default); 32: the top 32 bits, ties by pair index */
+  int32_t key_bits;   /* 64: full sort keys (default); 32: the warm start's keys cut to their top
+                         32 bits, ties by pair index */
   int32_t controller; /* step controller: 0 none (N_2 and `threshold` every step); 1 the MLP, ctl = its
                          JKG_CTL_NPARAM parameters; 2 a phase table, ctl = 8 floats (see below) */
   const float *ctl;   /* controller parameters, or NULL */
@@ -162,8 +164,28 @@ typedef struct {
                          0: the reference's test only */
   float *ctl_trace;   /* per-step controller records (JKG_CTL_REC floats each), or NULL */
   int32_t ctl_trace_cap;
-  int32_t reserved[7];
+  int32_t reserved0;
+  struct jkg_warm_info *warm; /* out (fp32_warm): what the warm start did, or NULL */
+  int32_t reserved[4];
 } jkg_mixed_opts;
+
+/* The FP32 warm start of jkg_zsolve_mixed: steps on FP32 copies of A, U and Z
+ * (weights and 2x2 parameters in FP64) until A's relative off-norm
+ * sqrt(off(A)^2 / |A|_F^2) falls below switch_tol (or has not improved by 1 %
+ * in 64 steps); then U and Z are replaced by their polar factors in the
+ * unitary and J-unitary groups (Newton-Schulz in FP64, X <- X (3I - X* X)/2),
+ * A is recomputed as U^H Y Z in FP64 and FP64 steps finish the solve. */
+typedef struct jkg_warm_info {
+  int32_t steps32;        /* steps of the FP32 phase */
+  int32_t warm_end;       /* 1 off-norm < switch_tol, 2 stalled, 3 converged in FP32, 4 step limit */
+  int32_t ns_u, ns_z;     /* Newton-Schulz updates of U and of Z */
+  int32_t fallback;       /* 1: a polar iteration could not start; the solve restarted in FP64 from Y */
+  int32_t reserved;
+  double off_rel;         /* relative off-norm at the switch */
+  double res_u, res_z;    /* |X* X - I|_F before the last update (U, Z) */
+  double seconds32;       /* device time of the FP32 phase */
+  double seconds_switch;  /* device time of the hand-over */
+} jkg_warm_info;
 
 /* The step controller.  At the start of each step it sets that step's N_2 (a
  * fraction of the problem's N_2) and threshold tau (-16 for log10 tau means

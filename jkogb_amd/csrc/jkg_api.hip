@@ -39,21 +39,26 @@ struct jkg_ctx {
   std::vector<void *> allocs;
   DeviceStep d{};
   double *sigma = nullptr;
-  // captured step (for the FMA policy and weight mode it was captured with)
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  // captured steps: [0] FP64, [1] the FP32 warm start (each for the options it was captured with)
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
   // host values the captured graph holds as launch shapes or kernel arguments
   struct GraphKey {
-    int fused = -1, wmode = -1, npl = -1, ctl_mode = -1, ctl_cap = -1;
-    double neg_tol = -1.0;
+    int fused = -1, wmode = -1, npl = -1, ctl_mode = -1, ctl_cap = -1, cut32 = -1;
+    double neg_tol = -1.0, switch_tol = -1.0;
     bool operator==(const GraphKey &o) const {
       return fused == o.fused && wmode == o.wmode && npl == o.npl && neg_tol == o.neg_tol &&
-             ctl_mode == o.ctl_mode && ctl_cap == o.ctl_cap;
+             ctl_mode == o.ctl_mode && ctl_cap == o.ctl_cap && cut32 == o.cut32 && switch_tol == o.switch_tol;
     }
-  } gkey;
+  } gkey[2];
+  StepState st_init{};  // the loop state of the current problem before its first step
   // mixed solver buffers (allocated on first use)
   std::vector<void *> mallocs;
   int mixed_n = 0;
+  // FP32 warm start: FP32 copies (in d), Newton-Schulz products M, T, P and block partials
+  int warm_n = 0;
+  void *wm_M = nullptr, *wm_T = nullptr, *wm_P = nullptr;
+  double *wm_part = nullptr;
   // step controller and its training (allocated on first use, kept across problems)
   float *ctl_master = nullptr, *ctl_table = nullptr, *ctl_trace = nullptr;
   uint16_t *ctl_q = nullptr;
@@ -66,11 +71,16 @@ struct jkg_ctx {
 };
 
 static void free_all(jkg_ctx *c) {
-  if (c->exec) (void)hipGraphExecDestroy(c->exec);
-  if (c->graph) (void)hipGraphDestroy(c->graph);
-  c->exec = nullptr;
-  c->graph = nullptr;
-  c->gkey = jkg_ctx::GraphKey{};
+  for (int k = 0; k < 2; ++k) {
+    if (c->exec[k]) (void)hipGraphExecDestroy(c->exec[k]);
+    if (c->graph[k]) (void)hipGraphDestroy(c->graph[k]);
+    c->exec[k] = nullptr;
+    c->graph[k] = nullptr;
+    c->gkey[k] = jkg_ctx::GraphKey{};
+  }
+  c->warm_n = 0;
+  c->wm_M = c->wm_T = c->wm_P = nullptr;
+  c->wm_part = nullptr;
   for (void *p : c->allocs) (void)hipFree(p);
   c->allocs.clear();
   for (void *p : c->mallocs) (void)hipFree(p);
@@ -172,7 +182,7 @@ static int ensure_mixed(jkg_ctx *c) {
   if (c->mixed_n == d.n) return JKG_OK;  // the buffers are freed with the problem's (size or type change)
   const size_t n = (size_t)d.n, parts = d.cplx ? 2 : 1;
   void *g = nullptr, *cn = nullptr;
-  if (hipMalloc(&g, sizeof(double) * parts * (n / 2) * (n - n / 2)) != hipSuccess ||
+  if (hipMalloc(&g, sizeof(double) * parts * gram_plane_doubles(d.n)) != hipSuccess ||
       hipMalloc(&cn, sizeof(double) * (1 + parts) * n) != hipSuccess) {
     (void)hipGetLastError();
     if (g) (void)hipFree(g);
@@ -183,6 +193,33 @@ static int ensure_mixed(jkg_ctx *c) {
   d.gram = (double *)g;
   d.coln = (double *)cn;
   c->mixed_n = d.n;
+  return JKG_OK;
+}
+
+// Buffers of the FP32 warm start: A32, U32, Z32 and the hand-over's products.
+static int ensure_warm(jkg_ctx *c) {
+  DeviceStep &d = c->d;
+  if (c->warm_n == d.n) return JKG_OK;  // freed with the problem's buffers
+  const size_t n2 = (size_t)d.n * d.n;
+  void *p[7] = {};
+  const size_t bytes[7] = {8 * n2, 8 * n2, 8 * n2, 16 * n2, 16 * n2, 16 * n2,
+                           sizeof(double) * (size_t)ns_blocks(d.n)};
+  for (int k = 0; k < 7; ++k) {
+    if (hipMalloc(&p[k], bytes[k]) != hipSuccess) {
+      (void)hipGetLastError();
+      for (int i = 0; i < k; ++i) (void)hipFree(p[i]);
+      return JKG_ENOMEM;
+    }
+  }
+  for (int k = 0; k < 7; ++k) c->mallocs.push_back(p[k]);
+  d.A32 = p[0];
+  d.U32 = p[1];
+  d.Z32 = p[2];
+  c->wm_M = p[3];
+  c->wm_T = p[4];
+  c->wm_P = p[5];
+  c->wm_part = (double *)p[6];
+  c->warm_n = d.n;
   return JKG_OK;
 }
 
@@ -377,6 +414,7 @@ static int upload_problem(jkg_ctx *c, int n, int n2, const void *y, const int64_
   s.n2max = n2;
   s.n2f = 1.f;
   s.ltau = (tau > 0.0) ? (float)std::log10(tau) : -16.f;
+  s.off_best = HUGE_VAL;
   {
     double y2 = 0.0;  // ||Y||_F^2 (the controller's scale)
     const double *yd = (const double *)y;
@@ -384,12 +422,15 @@ static int upload_problem(jkg_ctx *c, int n, int n2, const void *y, const int64_
     for (size_t e = 0; e < cnt; ++e) y2 += yd[e] * yd[e];
     s.ynorm2 = y2 > 0.0 ? y2 : 1.0;
   }
-  HIPCHK(hipMemcpyAsync(d.st, &s, sizeof(s), hipMemcpyHostToDevice, c->stream));
+  c->st_init = s;
+  HIPCHK(hipMemcpyAsync(d.st, &c->st_init, sizeof(s), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));  // the host vectors go out of scope
   return JKG_OK;
 }
 
+// Captures (or reuses) the step graph of the current precision: slot 1 while d.prec32.
 static int capture_step(jkg_ctx *c, int fused) {
+  const int slot = c->d.prec32 ? 1 : 0;
   jkg_ctx::GraphKey key;
   key.fused = fused;
   key.wmode = c->d.wmode;
@@ -397,52 +438,49 @@ static int capture_step(jkg_ctx *c, int fused) {
   key.neg_tol = c->d.neg_tol;
   key.ctl_mode = c->d.ctl_mode;
   key.ctl_cap = c->d.ctl_trace_cap;
-  if (c->exec && c->gkey == key) return JKG_OK;
-  if (c->exec) (void)hipGraphExecDestroy(c->exec);
-  if (c->graph) (void)hipGraphDestroy(c->graph);
-  c->exec = nullptr;
-  c->graph = nullptr;
+  key.cut32 = slot ? c->d.cut32 : 0;
+  key.switch_tol = slot ? c->d.switch_tol : 0.0;
+  if (c->exec[slot] && c->gkey[slot] == key) return JKG_OK;
+  if (c->exec[slot]) (void)hipGraphExecDestroy(c->exec[slot]);
+  if (c->graph[slot]) (void)hipGraphDestroy(c->graph[slot]);
+  c->exec[slot] = nullptr;
+  c->graph[slot] = nullptr;
   c->d.fused = fused;
   HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   step_launch(c->d, c->stream);
-  HIPCHK(hipStreamEndCapture(c->stream, &c->graph));
-  HIPCHK(hipGraphInstantiate(&c->exec, c->graph, nullptr, nullptr, 0));
-  c->gkey = key;
+  HIPCHK(hipStreamEndCapture(c->stream, &c->graph[slot]));
+  HIPCHK(hipGraphInstantiate(&c->exec[slot], c->graph[slot], nullptr, nullptr, 0));
+  c->gkey[slot] = key;
   return JKG_OK;
 }
 
-static int run_loop(jkg_ctx *c, const jkg_opts *o, jkg_result *res) {
+// Launches the captured step of the current precision until the device reports
+// done, or until `remaining` (if >= 0) steps are used up.  The reference needs
+// about 10 N steps; a run that has not ended after 1000 N + 1000 steps is
+// stopped and reported (a guard, never reached by a converging run).
+static int launch_until_done(jkg_ctx *c, const jkg_opts *o, long &remaining, long &launched) {
   DeviceStep &d = c->d;
-  int rc = capture_step(c, o->fused ? 1 : 0);
-  if (rc) return rc;
-  init_identity_launch(d, c->stream);
-  HIPCHK(hipGetLastError());
+  hipGraphExec_t exec = c->exec[d.prec32 ? 1 : 0];
   int batch = o->steps_per_sync > 0 ? o->steps_per_sync : 16;
-  HIPCHK(hipEventRecord(c->ev0, c->stream));
-  // The reference needs about 10 N steps; a run that has not ended after
-  // 1000 N + 1000 steps is stopped and reported (a guard, never reached by a
-  // converging run).
-  // With a step limit K, exactly K step graphs are launched and then the
-  // step-begin kernel alone, which records the end (no idle graphs).
   const long cap = 1000L * d.n + 1000;
-  long launched = 0, remaining = (o->max_steps >= 0) ? (long)o->max_steps : -1;
   for (;;) {
     const long nb = (remaining >= 0 && remaining < batch) ? remaining : batch;
-    for (long k = 0; k < nb; ++k) HIPCHK(hipGraphLaunch(c->exec, c->stream));
+    for (long k = 0; k < nb; ++k) HIPCHK(hipGraphLaunch(exec, c->stream));
     launched += nb;
     if (remaining >= 0) remaining -= nb;
-    if (remaining == 0) {
-      step_begin_launch(d, c->stream);
-      HIPCHK(hipGetLastError());
-      break;
-    }
+    if (remaining == 0) return JKG_OK;
     int done = 0;
     HIPCHK(hipMemcpyAsync(&done, &d.st->done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    if (done) break;
+    if (done) return JKG_OK;
     if (launched > cap) return JKG_ENOCONV;
     if (o->steps_per_sync <= 0 && batch < 64) batch *= 2;
   }
+}
+
+// After the last step: the end event, finalisation and the result record.
+static int finish_run(jkg_ctx *c, jkg_result *res) {
+  DeviceStep &d = c->d;
   HIPCHK(hipEventRecord(c->ev1, c->stream));
   finalize_launch(d, c->stream, c->sigma);
   HIPCHK(hipGetLastError());
@@ -459,9 +497,71 @@ static int run_loop(jkg_ctx *c, const jkg_opts *o, jkg_result *res) {
   res->sum_b2 = (int64_t)s.acc_b2;
   res->sum_b4 = (int64_t)s.acc_b4;
   res->sum_abod = (int64_t)s.acc_abod;
-  (void)launched;
   return JKG_OK;
 }
+
+// With a step limit K, exactly K step graphs are launched and then the
+// step-begin kernel alone, which records the end (no idle graphs).
+static int run_loop(jkg_ctx *c, const jkg_opts *o, jkg_result *res) {
+  DeviceStep &d = c->d;
+  d.prec32 = 0;
+  int rc = capture_step(c, o->fused ? 1 : 0);
+  if (rc) return rc;
+  init_identity_launch(d, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  long launched = 0, remaining = (o->max_steps >= 0) ? (long)o->max_steps : -1;
+  rc = launch_until_done(c, o, remaining, launched);
+  if (rc) return rc;
+  if (remaining == 0) {
+    step_begin_launch(d, c->stream);
+    HIPCHK(hipGetLastError());
+  }
+  return finish_run(c, res);
+}
+
+// Newton-Schulz iteration for the polar factor of X in the unitary group (J
+// null) or the J-unitary group: X <- X (3 I - X* X) / 2 until ||X* X - I||_F
+// (measured before each update) is below 1e-12 N or stops halving.  Returns 0
+// if the start is too far from the group (||.||_F >= 1/2) or the iteration
+// grows.
+static int polar_ns(jkg_ctx *c, void *X, const int *J, int *iters, double *res) {
+  const int n = c->d.n;
+  hipStream_t s = c->stream;
+  const int nb = ns_blocks(n);
+  std::vector<double> part(nb);
+  double prev = HUGE_VAL;
+  *iters = 0;
+  *res = HUGE_VAL;
+  for (int it = 0; it < 12; ++it) {
+    if (J) {
+      jrows_launch(n, X, J, c->wm_T, s);
+      zgemm_h_launch(n, X, c->wm_T, c->wm_M, s);  // X^H J X
+    } else {
+      zgemm_h_launch(n, X, X, c->wm_M, s);  // X^H X
+    }
+    ns_step_launch(n, c->wm_M, J, c->wm_T, c->wm_part, s);  // T = (3 I - S M) / 2
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(part.data(), c->wm_part, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    double r2 = 0.0;
+    for (double v : part) r2 += v;
+    const double r = std::sqrt(r2);
+    *res = r;
+    if (!(r < 0.5) || (it > 0 && r > prev)) return 0;
+    if (r < 1e-14) break;  // already in the group to rounding
+    zadj_launch(n, X, c->wm_P, s);
+    zgemm_h_launch(n, c->wm_P, c->wm_T, X, s);  // (X^H)^H T = X T
+    HIPCHK(hipGetLastError());
+    ++*iters;
+    if (r < 1e-12 * n || r > 0.5 * prev) break;
+    prev = r;
+  }
+  return 1;
+}
+
+struct MixedCfg;
+static int run_warm(jkg_ctx *c, const jkg_opts *o, jkg_result *res, const MixedCfg &mc);
 
 // What the mixed solver adds to a solve (all zero: the plain solver).
 struct MixedCfg {
@@ -471,7 +571,92 @@ struct MixedCfg {
   const float *ctl = nullptr;
   float *ctl_trace = nullptr;
   int ctl_trace_cap = 0;
+  // FP32 warm start
+  int warm = 0, cut32 = 0;
+  double switch_tol = 0.0;
+  jkg_warm_info *wi = nullptr;
 };
+
+// The FP32 warm start, its hand-over to FP64 (polar factors of U and Z in
+// their groups, A = U^H Y Z) and the FP64 steps.  If a polar iteration cannot
+// start (the FP32 factors are too far from their groups), the solve restarts
+// in FP64 from Y (reported as a fallback).
+static int run_warm(jkg_ctx *c, const jkg_opts *o, jkg_result *res, const MixedCfg &mc) {
+  DeviceStep &d = c->d;
+  const int fused = o->fused ? 1 : 0;
+  jkg_warm_info wi;
+  memset(&wi, 0, sizeof(wi));
+  d.cut32 = mc.cut32;
+  d.switch_tol = mc.switch_tol;
+  d.prec32 = 1;
+  int rc = capture_step(c, fused);
+  d.prec32 = 0;
+  if (!rc) rc = capture_step(c, fused);
+  if (rc) return rc;
+  hipEvent_t em = nullptr, es = nullptr;
+  HIPCHK(hipEventCreate(&em));
+  HIPCHK(hipEventCreate(&es));
+  auto done_events = [&]() {
+    (void)hipEventDestroy(em);
+    (void)hipEventDestroy(es);
+  };
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  warm_begin_launch(d, c->stream);
+  long launched = 0, remaining = (o->max_steps >= 0) ? (long)o->max_steps : -1;
+  d.prec32 = 1;
+  rc = launch_until_done(c, o, remaining, launched);
+  d.prec32 = 0;
+  if (rc) {
+    done_events();
+    return rc;
+  }
+  HIPCHK(hipEventRecord(em, c->stream));
+  StepState s;
+  HIPCHK(hipMemcpyAsync(&s, d.st, sizeof(s), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  wi.steps32 = s.nrec;
+  wi.warm_end = s.warm_end ? s.warm_end : (s.done ? 3 : 4);  // 3: converged in FP32, 4: step limit
+  wi.off_rel = s.off_rel;
+  warm_promote_launch(d, c->stream);
+  const int ok = polar_ns(c, d.U, nullptr, &wi.ns_u, &wi.res_u) && polar_ns(c, d.Z, d.J, &wi.ns_z, &wi.res_z);
+  if (ok) {
+    zgemm_h_launch(d.n, d.U, d.A, c->wm_M, c->stream);  // U^H Y
+    zadj_launch(d.n, c->wm_M, c->wm_P, c->stream);
+    zgemm_h_launch(d.n, c->wm_P, d.Z, d.A, c->stream);  // (U^H Y) Z
+    s.step = s.nrec;
+    s.done = 0;
+    s.info = 0;
+    s.active = 0;
+  } else {  // restart from Y in FP64
+    wi.fallback = 1;
+    init_identity_launch(d, c->stream);
+    s = c->st_init;
+    remaining = (o->max_steps >= 0) ? (long)o->max_steps : -1;
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(d.st, &s, sizeof(s), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipEventRecord(es, c->stream));
+  if (remaining != 0) rc = launch_until_done(c, o, remaining, launched);
+  if (rc) {
+    done_events();
+    return rc;
+  }
+  if (remaining == 0) {
+    step_begin_launch(d, c->stream);
+    HIPCHK(hipGetLastError());
+  }
+  rc = finish_run(c, res);
+  float m1 = 0.f, m2 = 0.f;
+  if (!rc) {
+    (void)hipEventElapsedTime(&m1, c->ev0, em);
+    (void)hipEventElapsedTime(&m2, em, es);
+  }
+  wi.seconds32 = m1 * 1e-3;
+  wi.seconds_switch = m2 * 1e-3;
+  done_events();
+  if (mc.wi) *mc.wi = wi;
+  return rc;
+}
 
 static int solve_common(jkg_ctx *c, int cplx, int n, int n2, const void *y, const int64_t *j, void *yu, void *zz,
                         double *sy, const jkg_opts *opts, jkg_result *res, const MixedCfg &mc = MixedCfg{}) {
@@ -505,6 +690,10 @@ static int solve_common(jkg_ctx *c, int cplx, int n, int n2, const void *y, cons
     rc = ensure_mixed(c);
     if (rc) return rc;
   }
+  if (mc.warm) {
+    rc = ensure_warm(c);
+    if (rc) return rc;
+  }
   c->d.ctl_mode = mc.ctl_mode;
   c->d.ctl_trace = nullptr;
   c->d.ctl_trace_cap = 0;
@@ -534,7 +723,7 @@ static int solve_common(jkg_ctx *c, int cplx, int n, int n2, const void *y, cons
   res->npos = npos;
   res->nneg = nneg;
   res->tt = (int)((int64_t)npos * (npos - 1) / 2 + (int64_t)nneg * (nneg - 1) / 2);
-  rc = run_loop(c, &o, res);
+  rc = mc.warm ? run_warm(c, &o, res, mc) : run_loop(c, &o, res);
   if (rc) return rc;
   DeviceStep &d = c->d;
   if (o.trace && o.trace_cap > 0) {
@@ -604,12 +793,19 @@ extern "C" int jkg_zsolve_mixed(jkg_ctx *c, int n, int n2, const double *y, cons
   if (m.weights != JKG_W_PAIR && m.weights != JKG_W_GRAM) return -10;
   if (!(m.tol >= 0.0) || !(m.threshold >= 0.0 && m.threshold < 1.0)) return -10;
   if (m.controller < 0 || m.controller > 2 || m.ctl_trace_cap < 0) return -10;
-  if (m.controller || m.fp32_warm != 0 || m.key_bits != 64) return JKG_ENOTSUP;
-  if (m.weights == JKG_W_PAIR && m.tol != 0.0) return JKG_ENOTSUP;
+  if (m.fp32_warm < 0 || m.fp32_warm > 1 || (m.key_bits != 32 && m.key_bits != 64)) return -10;
+  if (m.fp32_warm && !(m.switch_tol > 0.0 && m.switch_tol < 1.0)) return -10;
+  if (m.controller) return JKG_ENOTSUP;
+  if (m.key_bits == 32 && !m.fp32_warm) return JKG_ENOTSUP;  // 32-bit keys: the warm start only
+  if (m.weights == JKG_W_PAIR && (m.tol != 0.0 || m.fp32_warm)) return JKG_ENOTSUP;
   MixedCfg mc;
   mc.wmode = m.weights == JKG_W_GRAM ? 1 : 0;
   mc.tol = m.tol;
   mc.tau = m.threshold;
+  mc.warm = m.fp32_warm;
+  mc.cut32 = m.key_bits == 32;
+  mc.switch_tol = m.switch_tol;
+  mc.wi = m.warm;
   return solve_common(c, 1, n, n2, y, j, yu, zz, sy, opts, res, mc);
 }
 

@@ -152,4 +152,26 @@ __device__ __forceinline__ bool zeq0(dz a) { return a.re == 0.0 && a.im == 0.0; 
 __device__ __forceinline__ bool zne1(dz a) { return a.re != 1.0 || a.im != 0.0; }
 __device__ __forceinline__ double absz(dz a) { return hypot_ref(a.re, a.im); }  // ABS(complex) = cabs
 
+// Complex FP32 value of the mixed solver's warm start (A, U, Z held in FP32;
+// every 2x2 parameter and weight is formed in FP64 from the promoted entries).
+struct fz {
+  float re, im;
+};
+__device__ __forceinline__ dz ldz(const dz &a) { return a; }
+__device__ __forceinline__ dz ldz(const fz &a) { return dz{(double)a.re, (double)a.im}; }
+template <class E>
+__device__ __forceinline__ E emk(double re, double im);
+template <>
+__device__ __forceinline__ dz emk<dz>(double re, double im) { return dz{re, im}; }
+template <>
+__device__ __forceinline__ fz emk<fz>(double re, double im) { return fz{(float)re, (float)im}; }
+
+// Sort keys cut to their top 32 bits (the warm start's shorter keys): the low
+// four radix digits are then zero in every key and the sort skips their passes;
+// ties go by pair index (the sort is stable).  A cut NaN key is KEY_NAN32,
+// above every finite key in either width.
+constexpr uint64_t KEY_NAN32 = 0xFFFFFFFF00000000ull;
+__device__ __forceinline__ bool key_nan(uint64_t k) { return k >= KEY_NAN32; }
+__device__ __forceinline__ uint64_t key_cut(uint64_t k, bool cut) { return cut ? (k & KEY_NAN32) : k; }
+
 }  // namespace jkg

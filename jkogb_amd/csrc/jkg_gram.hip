@@ -38,12 +38,18 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 // sum X Y' = Im(X^H Y) (CX = 2): the complex product is two real ones.
 constexpr int GB_M = 128, GB_N = 128, GB_K = 16, GB_LD = GB_M + 4;
 
-template <bool ZD, int CX>
-__global__ void __launch_bounds__(256) k_gemm_tn(int k, const double *__restrict__ X, int ldx,
-                                                 const int *__restrict__ xcols, int m, const double *__restrict__ Y,
+// Split K: workgroup z sums the rows [z kchunk, (z + 1) kchunk) into its own
+// plane C + z plane (the consumer adds the planes in a fixed order, so results
+// do not depend on scheduling); kchunk is a multiple of GB_K.
+template <bool ZD, int CX, class T>
+__global__ void __launch_bounds__(256) k_gemm_tn(int k, int kchunk, long plane, const T *__restrict__ X, int ldx,
+                                                 const int *__restrict__ xcols, int m, const T *__restrict__ Y,
                                                  int ldy, const int *__restrict__ ycols, int nc, double *__restrict__ C,
                                                  long csi, long csj, const StepState *st) {
   if (st && !st->active) return;
+  const int kb = blockIdx.z * kchunk;
+  const int ke = (k - kb < kchunk) ? k : kb + kchunk;
+  C += blockIdx.z * plane;
   __shared__ double Xs[2][GB_K][GB_LD];
   __shared__ double Ys[2][GB_K][GB_LD];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -54,16 +60,16 @@ __global__ void __launch_bounds__(256) k_gemm_tn(int k, const double *__restrict
   const int xi = i0 + lc, yj = j0 + lc;
   const int xcol = (xi < m) ? (xcols ? xcols[xi] : xi) : -1;
   const int ycol = (yj < nc) ? (ycols ? ycols[yj] : yj) : -1;
-  const double *xp = X + (size_t)(xcol < 0 ? 0 : xcol) * ldx;
-  const double *yp = Y + (size_t)(ycol < 0 ? 0 : ycol) * ldy;
+  const T *xp = X + (size_t)(xcol < 0 ? 0 : xcol) * ldx;
+  const T *yp = Y + (size_t)(ycol < 0 ? 0 : ycol) * ldy;
   double xr[8], yr[8];
   auto gload = [&](int r0) {
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
-      const int r = r0 + lh + u;
+      const int r = kb + r0 + lh + u;
       const int rd = CX ? (r >> 1) : r;
-      xr[u] = (xcol >= 0 && r < k && !(ZD && rd == xcol)) ? xp[r] : 0.0;
-      yr[u] = (ycol >= 0 && r < k && !(ZD && rd == ycol)) ? yp[r] : 0.0;
+      xr[u] = (xcol >= 0 && r < ke && !(ZD && rd == xcol)) ? (double)xp[r] : 0.0;
+      yr[u] = (ycol >= 0 && r < ke && !(ZD && rd == ycol)) ? (double)yp[r] : 0.0;
     }
     if (CX == 2) {  // rows r0 + lh + u are even for even u: (re, im) -> (im, -re)
 #pragma unroll
@@ -89,7 +95,7 @@ __global__ void __launch_bounds__(256) k_gemm_tn(int k, const double *__restrict
   gload(0);
   sstore(0);
   __syncthreads();
-  const int nst = (k + GB_K - 1) / GB_K;
+  const int nst = (ke - kb + GB_K - 1) / GB_K;
   for (int s = 0; s < nst; ++s) {
     const int buf = s & 1;
     if (s + 1 < nst) gload((s + 1) * GB_K);
@@ -120,17 +126,47 @@ __global__ void __launch_bounds__(256) k_gemm_tn(int k, const double *__restrict
       }
 }
 
+// K split of a product with an m x nc output: enough workgroups to cover the
+// chip twice (2 fit a CU), at most GRAM_KS_MAX, at least 256 rows each.
+int gram_ksplit(int m, int nc, int k) {
+  const int tiles = ((m + GB_M - 1) / GB_M) * ((nc + GB_N - 1) / GB_N);
+  int ks = (512 + tiles - 1) / (tiles > 0 ? tiles : 1);
+  const int kmax = k / 256 > 1 ? k / 256 : 1;
+  if (ks > kmax) ks = kmax;
+  if (ks > GRAM_KS_MAX) ks = GRAM_KS_MAX;
+  return ks < 1 ? 1 : ks;
+}
+
+// Doubles of the Gram planes for an N x N problem, per part (real or imaginary):
+// npl nmi <= floor(N/2) ceil(N/2) for any J split, and ks <= GRAM_KS_MAX.
+size_t gram_plane_doubles(int n) { return (size_t)GRAM_KS_MAX * (size_t)(n / 2) * (size_t)(n - n / 2); }
+
+static inline int kchunk_of(int k, int ks) { return (((k + ks - 1) / ks) + GB_K - 1) / GB_K * GB_K; }
+
 void gemm_tn_launch(int k, const double *X, int ldx, const int *xcols, int m, const double *Y, int ldy,
                     const int *ycols, int nc, double *C, long csi, long csj, bool zero_diag, const StepState *st,
-                    hipStream_t s, int cx) {  // cx: 0 real, 1 / 2 real / imaginary part of X^H Y
+                    hipStream_t s, int cx, int ks) {  // cx: 0 real, 1 / 2 real / imaginary part of X^H Y
   if (m <= 0 || nc <= 0) return;
-  dim3 grid((nc + GB_N - 1) / GB_N, (m + GB_M - 1) / GB_M);
+  dim3 grid((nc + GB_N - 1) / GB_N, (m + GB_M - 1) / GB_M, ks);
+  const int kc = kchunk_of(k, ks);
+  const long plane = (long)m * nc;
   auto go = [&](auto kern) {
-    hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, k, X, ldx, xcols, m, Y, ldy, ycols, nc, C, csi, csj, st);
+    hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, k, kc, plane, X, ldx, xcols, m, Y, ldy, ycols, nc, C, csi, csj,
+                       st);
   };
-  if (cx == 1) go(zero_diag ? k_gemm_tn<true, 1> : k_gemm_tn<false, 1>);
-  else if (cx == 2) go(zero_diag ? k_gemm_tn<true, 2> : k_gemm_tn<false, 2>);
-  else go(zero_diag ? k_gemm_tn<true, 0> : k_gemm_tn<false, 0>);
+  if (cx == 1) go(zero_diag ? k_gemm_tn<true, 1, double> : k_gemm_tn<false, 1, double>);
+  else if (cx == 2) go(zero_diag ? k_gemm_tn<true, 2, double> : k_gemm_tn<false, 2, double>);
+  else go(zero_diag ? k_gemm_tn<true, 0, double> : k_gemm_tn<false, 0, double>);
+}
+
+// The complex Gram product of the FP32 warm start: FP32 entries, FP64 products and sums.
+static void zgemm_tn_f32_launch(int k, const float *X, int ldx, const int *xcols, int m, const int *ycols, int nc,
+                                double *C, long csi, long csj, const StepState *st, hipStream_t s, int cx, int ks) {
+  if (m <= 0 || nc <= 0) return;
+  dim3 grid((nc + GB_N - 1) / GB_N, (m + GB_M - 1) / GB_M, ks);
+  auto kern = (cx == 1) ? k_gemm_tn<true, 1, float> : k_gemm_tn<true, 2, float>;
+  hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, k, kchunk_of(k, ks), (long)m * nc, X, ldx, xcols, m, X, ldx,
+                     ycols, nc, C, csi, csj, st);
 }
 
 // ------------------------------------------------------------------ column norms
@@ -165,7 +201,7 @@ template <bool F>
 __global__ void __launch_bounds__(256) k_hyp_weights_gram(int n, const double *__restrict__ A, int lda,
                                                           const int *__restrict__ J, const int *__restrict__ plus,
                                                           int npl, const int *__restrict__ minus, int nmi,
-                                                          const double *__restrict__ G, int ldg,
+                                                          const double *__restrict__ G, int ldg, int gks,
                                                           const double *__restrict__ coln,
                                                           const double *__restrict__ dg,
                                                           uint64_t *__restrict__ keys, uint32_t *__restrict__ vals,
@@ -211,7 +247,9 @@ __global__ void __launch_bounds__(256) k_hyp_weights_gram(int n, const double *_
         double d = fmad<F>(aqp - aq, aqp + aq, 0.0);
         d = fmad<F>(apq - ap, apq + ap, d);
         const double gxx = coln[p0] - aqp * aqp, gyy = coln[q0] - apq * apq;
-        const double gxy = G[(size_t)(i0 + a) * ldg + j0 + b];
+        const size_t gi = (size_t)(i0 + a) * ldg + j0 + b, plane = (size_t)npl * nmi;
+        double gxy = G[gi];
+        for (int z = 1; z < gks; ++z) gxy += G[gi + z * plane];
         const double b11 = z2[0], b21 = z2[1], b12 = z2[2], b22 = z2[3];
         const double w = d + gxx * (1.0 - b11 * b11 - b12 * b12) + gyy * (1.0 - b21 * b21 - b22 * b22) -
                          2.0 * gxy * (b11 * b21 + b12 * b22);
@@ -232,14 +270,16 @@ void gram_hyp_weights_launch(DeviceStep &d, hipStream_t s) {
   const double *A = (const double *)d.A;
   hipLaunchKernelGGL(k_colnorm_off, dim3((d.n + 3) / 4), dim3(256), 0, s, d.n, A, d.lda, d.coln, d.coln + d.n,
                      d.st);
-  gemm_tn_launch(d.n, A, d.lda, d.plus, d.npl, A, d.lda, d.minus, d.nmi, d.gram, (long)d.nmi, 1L, true, d.st, s, 0);
+  const int ks = gram_ksplit(d.npl, d.nmi, d.n);
+  gemm_tn_launch(d.n, A, d.lda, d.plus, d.npl, A, d.lda, d.minus, d.nmi, d.gram, (long)d.nmi, 1L, true, d.st, s, 0,
+                 ks);
   dim3 g((d.nmi + WT - 1) / WT, (d.npl + WT - 1) / WT);
   if (d.fused)
     hipLaunchKernelGGL(k_hyp_weights_gram<true>, g, dim3(256), 0, s, d.n, A, d.lda, d.J, d.plus, d.npl, d.minus,
-                       d.nmi, d.gram, d.nmi, d.coln, d.coln + d.n, d.sort.k0, d.sort.v0, d.st, d.neg_tol);
+                       d.nmi, d.gram, d.nmi, ks, d.coln, d.coln + d.n, d.sort.k0, d.sort.v0, d.st, d.neg_tol);
   else
     hipLaunchKernelGGL(k_hyp_weights_gram<false>, g, dim3(256), 0, s, d.n, A, d.lda, d.J, d.plus, d.npl, d.minus,
-                       d.nmi, d.gram, d.nmi, d.coln, d.coln + d.n, d.sort.k0, d.sort.v0, d.st, d.neg_tol);
+                       d.nmi, d.gram, d.nmi, ks, d.coln, d.coln + d.n, d.sort.k0, d.sort.v0, d.st, d.neg_tol);
 }
 
 // ------------------------------------------------------------------ complex
@@ -253,16 +293,17 @@ void gram_hyp_weights_launch(DeviceStep &d, hipStream_t s) {
 // pair whose J = +1 column is the larger index, gxy = conj(G).
 
 // c[col] = sum_{r != col} |A[r, col]|^2 and dg[col] = A[col, col] (complex), one wave per column.
-__global__ void __launch_bounds__(256) k_zcolnorm_off(int n, const dz *__restrict__ A, int lda,
+template <class E>
+__global__ void __launch_bounds__(256) k_zcolnorm_off(int n, const E *__restrict__ A, int lda,
                                                       double *__restrict__ c, dz *__restrict__ dg,
                                                       const StepState *st) {
   if (!st->active) return;
   const int col = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (col >= n) return;
-  const dz *a = A + (size_t)col * lda;
+  const E *a = A + (size_t)col * lda;
   double s = 0.0;
   for (int r = lane; r < n; r += 64) {
-    const dz v = a[r];
+    const dz v = ldz(a[r]);
     if (r == col) dg[col] = v;
     else s = __builtin_fma(v.im, v.im, __builtin_fma(v.re, v.re, s));
   }
@@ -271,17 +312,64 @@ __global__ void __launch_bounds__(256) k_zcolnorm_off(int n, const dz *__restric
   if (lane == 0) c[col] = s;
 }
 
+// The warm start's end test, after the column norms: the relative off-norm
+// sqrt(sum c / (sum c + sum |a_ii|^2)) of this step's A; below switch_tol, or
+// no 1 % improvement in WARM_STALL steps, ends the FP32 phase before this step
+// (the rest of the step is skipped; the loop reports done).
+__global__ void __launch_bounds__(256) k_warm_offnorm(int n, const double *__restrict__ c,
+                                                      const dz *__restrict__ dg, StepState *st, double switch_tol) {
+  if (!st->active) return;
+  __shared__ double so[256], sd[256];
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    a += c[i];
+    b = __builtin_fma(dg[i].re, dg[i].re, __builtin_fma(dg[i].im, dg[i].im, b));
+  }
+  so[threadIdx.x] = a;
+  sd[threadIdx.x] = b;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) {
+      so[threadIdx.x] += so[threadIdx.x + h];
+      sd[threadIdx.x] += sd[threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double off = so[0], tot = so[0] + sd[0];
+    const double rel = (tot > 0.0) ? __builtin_sqrt(off / tot) : 0.0;
+    const int step = st->step;
+    st->off_rel = rel;
+    if (rel < 0.99 * st->off_best) {
+      st->off_best = rel;
+      st->off_best_step = step;
+    }
+    int end = 0;
+    if (rel < switch_tol) end = 1;
+    else if (step - st->off_best_step >= WARM_STALL) end = 2;
+    if (end) {
+      st->warm_end = end;
+      st->done = 1;
+      st->info = step;
+      st->active = 0;
+      st->sel_done = 1;
+      st->nn_sort = 0;
+    }
+  }
+}
+
 constexpr int ZWT = 32;
 
-template <bool F>
-__global__ void __launch_bounds__(256) k_zhyp_weights_gram(int n, const dz *__restrict__ A, int lda,
+template <bool F, class E>
+__global__ void __launch_bounds__(256) k_zhyp_weights_gram(int n, const E *__restrict__ A, int lda,
                                                            const int *__restrict__ J, const int *__restrict__ plus,
                                                            int npl, const int *__restrict__ minus, int nmi,
                                                            const double *__restrict__ Gre,
-                                                           const double *__restrict__ Gim, int ldg,
+                                                           const double *__restrict__ Gim, int ldg, int gks,
                                                            const double *__restrict__ coln,
                                                            const dz *__restrict__ dg, uint64_t *__restrict__ keys,
-                                                           uint32_t *__restrict__ vals, StepState *st, double tol) {
+                                                           uint32_t *__restrict__ vals, StepState *st, double tol,
+                                                           int cut) {
   if (!st->active) return;
   __shared__ dz E1[ZWT][ZWT + 1];  // E1[a][b] = A(minus b, plus a)
   __shared__ dz E2[ZWT][ZWT + 1];  // E2[b][a] = A(plus a, minus b)
@@ -293,9 +381,9 @@ __global__ void __launch_bounds__(256) k_zhyp_weights_gram(int n, const dz *__re
   for (int e = tid; e < ZWT * ZWT; e += 256) {
     const int hi = e / ZWT, lo = e % ZWT;
     const int c1 = pc[hi], r1 = mc[lo];
-    E1[hi][lo] = (c1 >= 0 && r1 >= 0) ? A[(size_t)c1 * lda + r1] : zmk(0.0, 0.0);
+    E1[hi][lo] = (c1 >= 0 && r1 >= 0) ? ldz(A[(size_t)c1 * lda + r1]) : zmk(0.0, 0.0);
     const int c2 = mc[hi], r2 = pc[lo];
-    E2[hi][lo] = (c2 >= 0 && r2 >= 0) ? A[(size_t)c2 * lda + r2] : zmk(0.0, 0.0);
+    E2[hi][lo] = (c2 >= 0 && r2 >= 0) ? ldz(A[(size_t)c2 * lda + r2]) : zmk(0.0, 0.0);
   }
   __syncthreads();
   unsigned nfull = 0;
@@ -326,8 +414,13 @@ __global__ void __launch_bounds__(256) k_zhyp_weights_gram(int n, const dz *__re
         d = fmad<F>(apq.im - ap.im, apq.im + ap.im, d);
         const double gxx = coln[p0] - (aqp.re * aqp.re + aqp.im * aqp.im);
         const double gyy = coln[q0] - (apq.re * apq.re + apq.im * apq.im);
-        const size_t gi = (size_t)(i0 + a) * ldg + j0 + b;
-        const double gr = Gre[gi], gim = cp < cm ? Gim[gi] : -Gim[gi];
+        const size_t gi = (size_t)(i0 + a) * ldg + j0 + b, plane = (size_t)npl * nmi;
+        double gr = Gre[gi], gi2 = Gim[gi];
+        for (int z = 1; z < gks; ++z) {
+          gr += Gre[gi + z * plane];
+          gi2 += Gim[gi + z * plane];
+        }
+        const double gim = cp < cm ? gi2 : -gi2;
         const dz w11 = z2[0], w21 = z2[1], w12 = z2[2], w22 = z2[3];
         const double n11 = w11.re * w11.re + w11.im * w11.im, n12 = w12.re * w12.re + w12.im * w12.im;
         const double n21 = w21.re * w21.re + w21.im * w21.im, n22 = w22.re * w22.re + w22.im * w22.im;
@@ -339,7 +432,7 @@ __global__ void __launch_bounds__(256) k_zhyp_weights_gram(int n, const dz *__re
       }
     }
     const uint32_t k = pair_index(n, p0 + 1, q0 + 1);
-    keys[k] = key;
+    keys[k] = key_cut(key, cut);
     vals[k] = pack_pq(p0, q0);
   }
   for (int o = 32; o > 0; o >>= 1) nfull += __shfl_xor(nfull, o);
@@ -347,23 +440,37 @@ __global__ void __launch_bounds__(256) k_zhyp_weights_gram(int n, const dz *__re
 }
 
 void zgram_hyp_weights_launch(DeviceStep &d, hipStream_t s) {
+  const int ks = gram_ksplit(d.npl, d.nmi, 2 * d.n);
+  double *Gre = d.gram, *Gim = d.gram + (size_t)ks * d.npl * d.nmi;
+  dz *dg = (dz *)(d.coln + d.n);
+  const int cut = d.prec32 ? d.cut32 : 0;
+  dim3 g((d.nmi + ZWT - 1) / ZWT, (d.npl + ZWT - 1) / ZWT);
+  auto weights = [&](auto kern, const auto *A) {
+    hipLaunchKernelGGL(kern, g, dim3(256), 0, s, d.n, A, d.lda, d.J, d.plus, d.npl, d.minus, d.nmi, Gre, Gim, d.nmi,
+                       ks, d.coln, dg, d.sort.k0, d.sort.v0, d.st, d.neg_tol, cut);
+  };
+  if (d.prec32) {
+    const fz *A = (const fz *)d.A32;
+    hipLaunchKernelGGL(k_zcolnorm_off<fz>, dim3((d.n + 3) / 4), dim3(256), 0, s, d.n, A, d.lda, d.coln, dg, d.st);
+    hipLaunchKernelGGL(k_warm_offnorm, dim3(1), dim3(256), 0, s, d.n, d.coln, dg, d.st, d.switch_tol);
+    if (d.npl <= 0 || d.nmi <= 0) return;
+    const float *X = (const float *)d.A32;
+    zgemm_tn_f32_launch(2 * d.n, X, 2 * d.lda, d.plus, d.npl, d.minus, d.nmi, Gre, (long)d.nmi, 1L, d.st, s, 1, ks);
+    zgemm_tn_f32_launch(2 * d.n, X, 2 * d.lda, d.plus, d.npl, d.minus, d.nmi, Gim, (long)d.nmi, 1L, d.st, s, 2, ks);
+    if (d.fused) weights(k_zhyp_weights_gram<true, fz>, A);
+    else weights(k_zhyp_weights_gram<false, fz>, A);
+    return;
+  }
   if (d.npl <= 0 || d.nmi <= 0) return;
   const dz *A = (const dz *)d.A;
-  double *Gre = d.gram, *Gim = d.gram + (size_t)d.npl * d.nmi;
-  dz *dg = (dz *)(d.coln + d.n);
-  hipLaunchKernelGGL(k_zcolnorm_off, dim3((d.n + 3) / 4), dim3(256), 0, s, d.n, A, d.lda, d.coln, dg, d.st);
+  hipLaunchKernelGGL(k_zcolnorm_off<dz>, dim3((d.n + 3) / 4), dim3(256), 0, s, d.n, A, d.lda, d.coln, dg, d.st);
   const double *X = (const double *)d.A;
   gemm_tn_launch(2 * d.n, X, 2 * d.lda, d.plus, d.npl, X, 2 * d.lda, d.minus, d.nmi, Gre, (long)d.nmi, 1L, true,
-                 d.st, s, 1);
+                 d.st, s, 1, ks);
   gemm_tn_launch(2 * d.n, X, 2 * d.lda, d.plus, d.npl, X, 2 * d.lda, d.minus, d.nmi, Gim, (long)d.nmi, 1L, true,
-                 d.st, s, 2);
-  dim3 g((d.nmi + ZWT - 1) / ZWT, (d.npl + ZWT - 1) / ZWT);
-  auto go = [&](auto kern) {
-    hipLaunchKernelGGL(kern, g, dim3(256), 0, s, d.n, A, d.lda, d.J, d.plus, d.npl, d.minus, d.nmi, Gre, Gim, d.nmi,
-                       d.coln, dg, d.sort.k0, d.sort.v0, d.st, d.neg_tol);
-  };
-  if (d.fused) go(k_zhyp_weights_gram<true>);
-  else go(k_zhyp_weights_gram<false>);
+                 d.st, s, 2, ks);
+  if (d.fused) weights(k_zhyp_weights_gram<true, dz>, A);
+  else weights(k_zhyp_weights_gram<false, dz>, A);
 }
 
 }  // namespace jkg

@@ -234,7 +234,7 @@ __global__ void __launch_bounds__(SEL_THREADS_RES) k_sel_resolve(int n, const ui
       int lo = 0, hi = nn;
       while (lo < hi) {
         int mid = (lo + hi) >> 1;
-        if (keys[mid] == KEY_NAN) hi = mid;
+        if (key_nan(keys[mid])) hi = mid;
         else lo = mid + 1;
       }
       st->it = lo;

@@ -63,6 +63,9 @@ struct StepState {
   unsigned long long t_begin;  // wall clock (100 MHz) at the start of this step
   float ema, n2f, ltau;        // smoothed TW rate; the action of this step (N_2 / n2max, log10 tau)
   float feat[16];              // controller features after the last step
+  // FP32 warm start: relative off-norm of A (this step's), the best seen and its step, why the phase ended
+  double off_rel, off_best;
+  int off_best_step, warm_end;
   // work counters summed over the steps (bytes and flops of the run)
   unsigned long long acc_sl, acc_it, acc_b2, acc_b4, acc_abod;
 };
@@ -94,7 +97,7 @@ struct DeviceStep {
   int select_limit;       // entries walked per selection round before filtering
   // mixed solver
   int wmode;              // hyperbolic weights: 0 per pair (ABODNDF2), 1 Gram form
-  double *gram;           // npl x nmi: (O^T O)(plus, minus), O = A with a zero diagonal
+  double *gram;           // ks planes of npl x nmi: (O^T O)(plus, minus), O = A with a zero diagonal
   double *coln;           // 2N: off-diagonal column norms^2, then the diagonal of A
   double neg_tol;         // > 0: negligible pairs are dropped (mixed solver's convergence test)
   // step controller: 0 none, 1 MLP (ctl_master/ctl_q), 2 phase table (ctl_table)
@@ -104,7 +107,16 @@ struct DeviceStep {
   const float *ctl_table;
   float *ctl_trace;       // per step: 16 features, N_2 fraction, log10 tau, step time (us), TW
   int ctl_trace_cap;
+  // FP32 warm start (complex mixed solver): while prec32, the step works on the
+  // FP32 copies (interleaved complex float, leading dimension lda); cut32: sort
+  // keys cut to 32 bits; the phase ends once the relative off-norm of A32 is
+  // below switch_tol or has not improved by 1 % in WARM_STALL steps
+  void *A32, *U32, *Z32;
+  int prec32, cut32;
+  double switch_tol;
 };
+
+constexpr int WARM_STALL = 64;
 
 constexpr int CTL_REC = 20;  // floats per controller trace record
 
@@ -137,8 +149,20 @@ void gram_hyp_weights_launch(DeviceStep &d, hipStream_t s);
 void zgram_hyp_weights_launch(DeviceStep &d, hipStream_t s);
 // the step controller (jkg_mlp.hip): sets StepState::n2 and ::tau for this step
 void controller_launch(DeviceStep &d, hipStream_t s);
+// FP32 warm start and its hand-over (jkg_warm.hip)
+void warm_begin_launch(DeviceStep &d, hipStream_t s);
+void warm_promote_launch(DeviceStep &d, hipStream_t s);
+void zgemm_h_launch(int n, const void *X, const void *Y, void *C, hipStream_t s);
+void zadj_launch(int n, const void *X, void *P, hipStream_t s);
+void jrows_launch(int n, const void *X, const int *J, void *Y, hipStream_t s);
+int ns_blocks(int n);
+void ns_step_launch(int n, const void *M, const int *J, void *B, double *part, hipStream_t s);
 void gemm_tn_launch(int k, const double *X, int ldx, const int *xcols, int m, const double *Y, int ldy,
                     const int *ycols, int nc, double *C, long csi, long csj, bool zero_diag, const StepState *st,
-                    hipStream_t s, int cx = 0);
+                    hipStream_t s, int cx = 0, int ks = 1);
+// split K of the Gram products: ks planes of npl x nmi doubles (per part), added by the weight kernels
+constexpr int GRAM_KS_MAX = 8;
+int gram_ksplit(int m, int nc, int k);
+size_t gram_plane_doubles(int n);
 
 }  // namespace jkg

@@ -72,13 +72,57 @@ __device__ __forceinline__ void zrot_ab_apply(const ZRot &R, dz &x, dz &y) {
   y = zzmul<F>(yy, R.m2);
 }
 
+// The same transforms on FP32 data (the warm start): the descriptor is formed
+// in FP64 and rounded once; the products run in FP32.
+struct ZRotF {
+  fz r1, r2, m1, m2;
+  int sa, sb;
+};
+__device__ __forceinline__ fz tofz(dz a) { return fz{(float)a.re, (float)a.im}; }
+__device__ __forceinline__ const ZRot &as_rot(const ZRot &R, const dz *) { return R; }
+__device__ __forceinline__ ZRotF as_rot(const ZRot &R, const fz *) {
+  return ZRotF{tofz(R.r1), tofz(R.r2), tofz(R.m1), tofz(R.m2), R.sa, R.sb};
+}
+template <bool F>
+__device__ __forceinline__ fz zzfmaf(fz a, fz b, fz c) {
+  float f = fmadf<F>(-a.im, b.im, c.re);
+  const float r = fmadf<F>(a.re, b.re, f);
+  f = fmadf<F>(a.im, b.re, c.im);
+  return fz{r, fmadf<F>(a.re, b.im, f)};
+}
+template <bool F>
+__device__ __forceinline__ fz zzmulf(fz a, fz b) {
+  const float r = fmadf<F>(a.re, b.re, -(a.im * b.im));
+  return fz{r, fmadf<F>(a.re, b.im, a.im * b.re)};
+}
+template <bool F>
+__device__ __forceinline__ void zrot_ba_apply(const ZRotF &R, fz &x, fz &y) {
+  fz xx, yy;
+  if (R.sa) {
+    xx = zzfmaf<F>(R.r1, y, x);
+    yy = R.sb ? zzfmaf<F>(R.r2, x, y) : zzfmaf<F>(R.r1, y, x);
+  } else {
+    xx = zzfmaf<F>(R.r1, x, y);
+    yy = R.sb ? zzfmaf<F>(R.r2, x, y) : zzfmaf<F>(R.r2, y, x);
+  }
+  x = zzmulf<F>(xx, R.m1);
+  y = zzmulf<F>(yy, R.m2);
+}
+template <bool F>
+__device__ __forceinline__ void zrot_ab_apply(const ZRotF &R, fz &x, fz &y) {
+  const fz xx = R.sa ? zzfmaf<F>(y, R.r1, x) : zzfmaf<F>(x, R.r1, y);
+  const fz yy = R.sb ? zzfmaf<F>(x, R.r2, y) : zzfmaf<F>(y, R.r2, x);
+  x = zzmulf<F>(xx, R.m1);
+  y = zzmulf<F>(yy, R.m2);
+}
+
 // ---------------------------------------------------------------- weights
 // ZMAGF2T, src/Z/zstep.F90:12-39
-template <bool F>
-__global__ void __launch_bounds__(256) k_ztrig_weights(int n, const dz *__restrict__ A, int lda,
+template <bool F, class E>
+__global__ void __launch_bounds__(256) k_ztrig_weights(int n, const E *__restrict__ A, int lda,
                                                         const int *__restrict__ J, uint64_t *__restrict__ keys,
                                                         uint32_t *__restrict__ vals, const StepState *st, int nn,
-                                                        double tol) {
+                                                        double tol, int cut) {
   if (!st->active) return;
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < (uint32_t)nn; k += gridDim.x * blockDim.x) {
     int p, q;
@@ -86,7 +130,8 @@ __global__ void __launch_bounds__(256) k_ztrig_weights(int n, const dz *__restri
     const int jp = J[p - 1], jq = J[q - 1];
     if (jp != jq) continue;
     const size_t P = p - 1, Q = q - 1;
-    const dz aqp = A[P * lda + Q], apq = A[Q * lda + P], app = A[P * lda + P], aqq = A[Q * lda + Q];
+    const dz aqp = ldz(A[P * lda + Q]), apq = ldz(A[Q * lda + P]), app = ldz(A[P * lda + P]),
+             aqq = ldz(A[Q * lda + Q]);
     const bool off = !zeq0(aqp) || !zeq0(apq) || app.im != 0.0 || aqq.im != 0.0 || sign_neg(app.re) ||
                      sign_neg(aqq.re);
     const bool diag = app.im != 0.0 || aqq.im != 0.0 || sign_neg(app.re) || sign_neg(aqq.re);
@@ -94,7 +139,7 @@ __global__ void __launch_bounds__(256) k_ztrig_weights(int n, const dz *__restri
     bool go = off || order;
     // tol > 0 (mixed solver only): the convergence test, as for the real pairs
     if (tol > 0.0 && go && !diag && !order && negligible(absz(aqp), absz(apq), app.re, aqq.re, tol)) go = false;
-    keys[k] = go ? weight_key(dasum4<F>(aqp.re, aqp.im, apq.re, apq.im)) : KEY_NAN;
+    keys[k] = key_cut(go ? weight_key(dasum4<F>(aqp.re, aqp.im, apq.re, apq.im)) : KEY_NAN, cut);
     vals[k] = pack_pq(p - 1, q - 1);
   }
 }
@@ -187,8 +232,8 @@ __global__ void __launch_bounds__(256) k_zhyp_weights(int n, const dz *__restric
 }
 
 // ---------------------------------------------------------------- pivots
-template <bool F>
-__global__ void __launch_bounds__(256) k_zpivots(int n, const dz *__restrict__ A, int lda, const int *__restrict__ J,
+template <bool F, class E>
+__global__ void __launch_bounds__(256) k_zpivots(int n, const E *__restrict__ A, int lda, const int *__restrict__ J,
                                                   const int *__restrict__ piv_p, const int *__restrict__ piv_q,
                                                   const int *__restrict__ piv_pos, const uint64_t *k0,
                                                   const uint64_t *k1, const uint32_t *v0, const uint32_t *v1,
@@ -204,7 +249,7 @@ __global__ void __launch_bounds__(256) k_zpivots(int n, const dz *__restrict__ A
     SortResult sr = sort_result_dev(k0, k1, v0, v1, sort_skip);
     piv_w[i] = key_weight(sr.keys[piv_pos[i] - 1]);
   }
-  dz b[4] = {A[P * lda + P], A[P * lda + Q], A[Q * lda + P], A[Q * lda + Q]};
+  dz b[4] = {ldz(A[P * lda + P]), ldz(A[P * lda + Q]), ldz(A[Q * lda + P]), ldz(A[Q * lda + Q])};
   int k[2] = {J[p], J[q]};
   dz v[4], w[4];
   const int info = zhsvd2<F>(b, k, v, w, TH1FIX);
@@ -226,13 +271,15 @@ __global__ void __launch_bounds__(256) k_zpivots(int n, const dz *__restrict__ A
 }
 
 // ---------------------------------------------------------------- rotate
-template <bool F>
-__global__ void __launch_bounds__(512) k_zrotate(int n, dz *__restrict__ A, int lda, dz *__restrict__ U,
-                                                 dz *__restrict__ Z, const int *__restrict__ piv_p,
+template <bool F, class E>
+__global__ void __launch_bounds__(512) k_zrotate(int n, E *__restrict__ A, int lda, E *__restrict__ U,
+                                                 E *__restrict__ Z, const int *__restrict__ piv_p,
                                                  const int *__restrict__ piv_q, const int *__restrict__ piv_info,
                                                  const double *__restrict__ piv_d, const ZRot *__restrict__ zrot,
                                                  const int *__restrict__ colpiv, const StepState *st) {
-  extern __shared__ dz LZ[];
+  extern __shared__ __attribute__((aligned(16))) unsigned char lz_raw[];
+  E *LZ = (E *)lz_raw;
+  const E *tag = nullptr;  // selects the descriptor's precision
   if (!st->active) return;
   const int sl = st->sl;
   if (sl <= 0) return;
@@ -246,8 +293,8 @@ __global__ void __launch_bounds__(512) k_zrotate(int n, dz *__restrict__ A, int 
   const bool diag_set = (j >= 0) && err == 0 && info >= 0 && !(info & 8);
   const int tid = threadIdx.x, bs = blockDim.x;
   if (nrow > 0 || col_rot || diag_set) {
-    dz *col0 = A + (size_t)c * lda;
-    dz *col1 = (c1 >= 0) ? A + (size_t)c1 * lda : nullptr;
+    E *col0 = A + (size_t)c * lda;
+    E *col1 = (c1 >= 0) ? A + (size_t)c1 * lda : nullptr;
     for (int r = tid; r < n; r += bs) {
       LZ[r] = col0[r];
       if (col1) LZ[n + r] = col1[r];
@@ -257,9 +304,9 @@ __global__ void __launch_bounds__(512) k_zrotate(int n, dz *__restrict__ A, int 
       for (int i = tid; i < sl; i += bs) {
         const int inf = piv_info[i];
         if (inf < 0 || !(inf & 2)) continue;
-        const ZRot R = zrot[3 * i + 0];
+        const auto R = as_rot(zrot[3 * i + 0], tag);
         const int p = piv_p[i], q = piv_q[i];
-        dz x = LZ[p], y = LZ[q];
+        E x = LZ[p], y = LZ[q];
         zrot_ba_apply<F>(R, x, y);
         LZ[p] = x;
         LZ[q] = y;
@@ -275,15 +322,14 @@ __global__ void __launch_bounds__(512) k_zrotate(int n, dz *__restrict__ A, int 
     }
     if (col1) {
       const int p = c, q = c1;
-      ZRot R;
-      if (col_rot) R = zrot[3 * j + 2];
+      const auto R = as_rot(zrot[3 * (col_rot ? j : 0) + 2], tag);  // read only if col_rot
       const double d0 = diag_set ? piv_d[2 * j] : 0.0, d1 = diag_set ? piv_d[2 * j + 1] : 0.0;
       for (int r = tid; r < n; r += bs) {
-        dz x = LZ[r], y = LZ[n + r];
+        E x = LZ[r], y = LZ[n + r];
         if (col_rot) zrot_ab_apply<F>(R, x, y);
         if (diag_set) {
-          if (r == p) { x = zmk(d0, 0.0); y = zmk(0.0, 0.0); }
-          if (r == q) { x = zmk(0.0, 0.0); y = zmk(d1, 0.0); }
+          if (r == p) { x = emk<E>(d0, 0.0); y = emk<E>(0.0, 0.0); }
+          if (r == q) { x = emk<E>(0.0, 0.0); y = emk<E>(d1, 0.0); }
         }
         col0[r] = x;
         col1[r] = y;
@@ -294,20 +340,20 @@ __global__ void __launch_bounds__(512) k_zrotate(int n, dz *__restrict__ A, int 
   }
   if (j >= 0 && info >= 0) {
     if (info & 2) {
-      const ZRot R = zrot[3 * j + 1];
-      dz *u0 = U + (size_t)c * lda, *u1 = U + (size_t)c1 * lda;
+      const auto R = as_rot(zrot[3 * j + 1], tag);
+      E *u0 = U + (size_t)c * lda, *u1 = U + (size_t)c1 * lda;
       for (int r = tid; r < n; r += bs) {
-        dz x = u0[r], y = u1[r];
+        E x = u0[r], y = u1[r];
         zrot_ab_apply<F>(R, x, y);
         u0[r] = x;
         u1[r] = y;
       }
     }
     if (info & 4) {
-      const ZRot R = zrot[3 * j + 2];
-      dz *z0 = Z + (size_t)c * lda, *z1 = Z + (size_t)c1 * lda;
+      const auto R = as_rot(zrot[3 * j + 2], tag);
+      E *z0 = Z + (size_t)c * lda, *z1 = Z + (size_t)c1 * lda;
       for (int r = tid; r < n; r += bs) {
-        dz x = z0[r], y = z1[r];
+        E x = z0[r], y = z1[r];
         zrot_ab_apply<F>(R, x, y);
         z0[r] = x;
         z1[r] = y;
@@ -359,18 +405,26 @@ static inline int zgrid(size_t work, int bs, int cap) {
 size_t zrot_bytes() { return sizeof(ZRot); }
 
 void zweights_launch(DeviceStep &d, hipStream_t s) {
-  const dz *A = (const dz *)d.A;
   const double tol = d.wmode ? d.neg_tol : 0.0;
-  if (d.fused)
-    hipLaunchKernelGGL(k_ztrig_weights<true>, dim3(zgrid(d.nn, 256, 65536)), dim3(256), 0, s, d.n, A, d.lda, d.J,
-                       d.sort.k0, d.sort.v0, d.st, d.nn, tol);
-  else
-    hipLaunchKernelGGL(k_ztrig_weights<false>, dim3(zgrid(d.nn, 256, 65536)), dim3(256), 0, s, d.n, A, d.lda, d.J,
-                       d.sort.k0, d.sort.v0, d.st, d.nn, tol);
+  const dim3 gt(zgrid(d.nn, 256, 65536));
+  auto trig = [&](auto kern, const auto *A) {
+    hipLaunchKernelGGL(kern, gt, dim3(256), 0, s, d.n, A, d.lda, d.J, d.sort.k0, d.sort.v0, d.st, d.nn, tol,
+                       d.prec32 ? d.cut32 : 0);
+  };
+  if (d.prec32) {
+    const fz *A = (const fz *)d.A32;
+    if (d.fused) trig(k_ztrig_weights<true, fz>, A);
+    else trig(k_ztrig_weights<false, fz>, A);
+  } else {
+    const dz *A = (const dz *)d.A;
+    if (d.fused) trig(k_ztrig_weights<true, dz>, A);
+    else trig(k_ztrig_weights<false, dz>, A);
+  }
   if (d.wmode == 1) {
     zgram_hyp_weights_launch(d, s);
     return;
   }
+  const dz *A = (const dz *)d.A;
   if (d.npl > 0 && d.nmi > 0) {
     dim3 g((d.npl + ZHT - 1) / ZHT, (d.nmi + ZHT - 1) / ZHT);
     if (d.fused)
@@ -384,34 +438,45 @@ void zweights_launch(DeviceStep &d, hipStream_t s) {
 
 void zpivots_launch(DeviceStep &d, hipStream_t s) {
   const int g = (d.n2max + 255) / 256;
-  if (d.fused)
-    hipLaunchKernelGGL(k_zpivots<true>, dim3(g), dim3(256), 0, s, d.n, (const dz *)d.A, d.lda, d.J, d.piv_p,
-                       d.piv_q, d.piv_pos, d.sort.k0, d.sort.k1, d.sort.v0, d.sort.v1, d.sort_skip, d.piv_w,
-                       d.piv_info, d.piv_d, (ZRot *)d.zrot, d.colpiv, d.st);
-  else
-    hipLaunchKernelGGL(k_zpivots<false>, dim3(g), dim3(256), 0, s, d.n, (const dz *)d.A, d.lda, d.J, d.piv_p,
-                       d.piv_q, d.piv_pos, d.sort.k0, d.sort.k1, d.sort.v0, d.sort.v1, d.sort_skip, d.piv_w,
-                       d.piv_info, d.piv_d, (ZRot *)d.zrot, d.colpiv, d.st);
+  auto go = [&](auto kern, const auto *A) {
+    hipLaunchKernelGGL(kern, dim3(g), dim3(256), 0, s, d.n, A, d.lda, d.J, d.piv_p, d.piv_q, d.piv_pos, d.sort.k0,
+                       d.sort.k1, d.sort.v0, d.sort.v1, d.sort_skip, d.piv_w, d.piv_info, d.piv_d, (ZRot *)d.zrot,
+                       d.colpiv, d.st);
+  };
+  if (d.prec32) {
+    if (d.fused) go(k_zpivots<true, fz>, (const fz *)d.A32);
+    else go(k_zpivots<false, fz>, (const fz *)d.A32);
+  } else {
+    if (d.fused) go(k_zpivots<true, dz>, (const dz *)d.A);
+    else go(k_zpivots<false, dz>, (const dz *)d.A);
+  }
 }
 
 void zrotate_launch(DeviceStep &d, hipStream_t s) {
-  const size_t lds = (size_t)2 * d.n * sizeof(dz);
-  if (d.fused)
-    hipLaunchKernelGGL(k_zrotate<true>, dim3(d.n), dim3(512), lds, s, d.n, (dz *)d.A, d.lda, (dz *)d.U, (dz *)d.Z,
-                       d.piv_p, d.piv_q, d.piv_info, d.piv_d, (const ZRot *)d.zrot, d.colpiv, d.st);
-  else
-    hipLaunchKernelGGL(k_zrotate<false>, dim3(d.n), dim3(512), lds, s, d.n, (dz *)d.A, d.lda, (dz *)d.U, (dz *)d.Z,
-                       d.piv_p, d.piv_q, d.piv_info, d.piv_d, (const ZRot *)d.zrot, d.colpiv, d.st);
+  auto go = [&](auto kern, auto *A, auto *U, auto *Z, size_t esz) {
+    hipLaunchKernelGGL(kern, dim3(d.n), dim3(512), 2 * d.n * esz, s, d.n, A, d.lda, U, Z, d.piv_p, d.piv_q,
+                       d.piv_info, d.piv_d, (const ZRot *)d.zrot, d.colpiv, d.st);
+  };
+  if (d.prec32) {
+    fz *A = (fz *)d.A32, *U = (fz *)d.U32, *Z = (fz *)d.Z32;
+    if (d.fused) go(k_zrotate<true, fz>, A, U, Z, sizeof(fz));
+    else go(k_zrotate<false, fz>, A, U, Z, sizeof(fz));
+  } else {
+    dz *A = (dz *)d.A, *U = (dz *)d.U, *Z = (dz *)d.Z;
+    if (d.fused) go(k_zrotate<true, dz>, A, U, Z, sizeof(dz));
+    else go(k_zrotate<false, dz>, A, U, Z, sizeof(dz));
+  }
 }
 
 int zstep_prepare(DeviceStep &d) {
   const size_t lds = (size_t)2 * d.n * sizeof(dz);
   if (lds > 160 * 1024) return -1;
-  if (hipFuncSetAttribute((const void *)k_zrotate<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
-          hipSuccess ||
-      hipFuncSetAttribute((const void *)k_zrotate<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
-          hipSuccess)
-    return -2;
+  const void *k[4] = {(const void *)k_zrotate<true, dz>, (const void *)k_zrotate<false, dz>,
+                      (const void *)k_zrotate<true, fz>, (const void *)k_zrotate<false, fz>};
+  for (int i = 0; i < 4; ++i)
+    if (hipFuncSetAttribute(k[i], hipFuncAttributeMaxDynamicSharedMemorySize, (int)(i < 2 ? lds : lds / 2)) !=
+        hipSuccess)
+      return -2;
   return 0;
 }
 

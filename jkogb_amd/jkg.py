@@ -51,7 +51,15 @@ class MixedOpts(C.Structure):
     _fields_ = [("weights", C.c_int32), ("fp32_warm", C.c_int32), ("switch_tol", C.c_double),
                 ("key_bits", C.c_int32), ("controller", C.c_int32), ("ctl", C.c_void_p),
                 ("threshold", C.c_double), ("tol", C.c_double), ("ctl_trace", C.c_void_p),
-                ("ctl_trace_cap", C.c_int32), ("reserved", C.c_int32 * 7)]
+                ("ctl_trace_cap", C.c_int32), ("reserved0", C.c_int32), ("warm", C.c_void_p),
+                ("reserved", C.c_int32 * 4)]
+
+
+class WarmInfo(C.Structure):
+    """jkg_warm_info: what the FP32 warm start of the complex mixed solver did."""
+    _fields_ = [("steps32", C.c_int32), ("warm_end", C.c_int32), ("ns_u", C.c_int32), ("ns_z", C.c_int32),
+                ("fallback", C.c_int32), ("reserved", C.c_int32), ("off_rel", C.c_double), ("res_u", C.c_double),
+                ("res_z", C.c_double), ("seconds32", C.c_double), ("seconds_switch", C.c_double)]
 
 
 class BatchOpts(C.Structure):
@@ -281,6 +289,10 @@ class Context:
         if ctl_trace:
             ctr = np.zeros((max(trace_cap, 1), CTL_REC), np.float32)
             m.ctl_trace, m.ctl_trace_cap = ctr.ctypes.data, ctr.shape[0]
+        wi = None
+        if m.fp32_warm:
+            wi = WarmInfo()
+            m.warm = C.addressof(wi)
         r = Result()
         f = _zsolve_mixed if cplx else _dsolve_mixed
         _chk(f(self._h, n, n2, _p(Y), _p(J64), _p(YU), _p(ZZ), _p(SY), C.byref(o), C.byref(m), C.byref(r)),
@@ -289,6 +301,8 @@ class Context:
                    pn=(r.npos, r.nneg), n2=r.n2, seconds=r.seconds, steps=r.steps, **_counts(r))
         if ctr is not None:
             out["ctl"] = ctr[:min(r.steps, ctr.shape[0])].copy()
+        if wi is not None:
+            out["warm"] = {f: getattr(wi, f) for f, _ in WarmInfo._fields_ if f != "reserved"}
         return out
 
     # ---------------------------------------------------------------- batched solver

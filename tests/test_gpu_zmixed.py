@@ -146,3 +146,72 @@ def test_zmixed_profile_stages(ctx):
     assert p["steps"] == 3 and q["steps"] == 3
     assert all(v >= 0.0 for v in p["stage_ms"].values())
     assert p["sum_abod"] > 0
+
+
+# ---------------------------------------------------------------- FP32 warm start
+def check_against_plain(ctx, Y, J, out):
+    n = Y.shape[0]
+    plain = ctx.dsolve(Y, J, n // 2)
+    fused = ctx.dsolve(Y, J, n // 2, fused=True)
+    spread = np.max(np.abs(fused["SY"] - plain["SY"]) / np.abs(plain["SY"]))
+    bound = 4 * spread + 64 * n * EPS
+    rel = np.max(np.abs(out["SY"] - plain["SY"]) / np.abs(plain["SY"]))
+    assert rel <= bound, (rel, spread)
+    rec, orth, jorth = zresiduals(Y, J, out)
+    prec, porth, pjorth = zresiduals(Y, J, plain)
+    assert rec <= min(4 * prec + 2 * n * EPS, 64 * n * EPS), (rec, prec)
+    assert orth <= min(4 * porth + 2 * n * EPS, 64 * n * EPS), (orth, porth)
+    assert jorth <= 4 * pjorth + 2 * n * EPS, (jorth, pjorth)
+    return plain
+
+
+@pytest.mark.parametrize("key_bits", [64, 32])
+@pytest.mark.parametrize("n", [96, 131, 192])
+def test_warm_start_matches_plain(ctx, n, key_bits):
+    # FP32 steps to a relative off-norm of 1e-5, polar factors of U and Z in FP64, A = U^H Y Z, FP64 steps:
+    # the result is an HSVD of Y to the same accuracy as the FP64 solvers'
+    Y, J = inputs("c3", n, 21)
+    out = ctx.dsolve_mixed(Y, J, n // 2, fp32_warm=1, key_bits=key_bits)
+    w = out["warm"]
+    assert out["info"] >= 0
+    assert w["fallback"] == 0 and w["warm_end"] in (1, 3), w
+    assert w["steps32"] > 0 and w["ns_u"] >= 1 and w["ns_z"] >= 1, w
+    assert w["off_rel"] < 1e-5 or w["warm_end"] == 3, w
+    assert w["res_u"] < 1e-3 and w["res_z"] < 1e-2, w  # FP32-accurate factors before the first update
+    check_against_plain(ctx, Y, J, out)
+
+
+def test_warm_start_to_the_end_of_fp32(ctx):
+    # a tolerance at the edge of what FP32 steps reach: the phase ends on it, on a stall or on convergence in
+    # FP32, and the hand-over still gives the FP64 solvers' accuracy
+    n = 80
+    Y, J = inputs("c3", n, 22)
+    out = ctx.dsolve_mixed(Y, J, n // 2, fp32_warm=1, switch_tol=1e-300)
+    w = out["warm"]
+    assert w["warm_end"] in (1, 2, 3) and w["fallback"] == 0, w
+    assert w["off_rel"] < 1e-5, w
+    check_against_plain(ctx, Y, J, out)
+
+
+def test_warm_start_step_limit(ctx):
+    # a step limit inside the FP32 phase: exactly that many steps, then the hand-over
+    n = 64
+    Y, J = inputs("c3", n, 23)
+    out = ctx.dsolve_mixed(Y, J, n // 2, fp32_warm=1, max_steps=5)
+    assert out["steps"] == 5 and out["warm"]["warm_end"] == 4 and out["warm"]["steps32"] == 5
+    full = ctx.dsolve_mixed(Y, J, n // 2, fp32_warm=1)
+    assert full["steps"] > 5
+
+
+def test_warm_start_rejects(ctx):
+    from jkogb_amd import jkg
+    Y, J = inputs("c3", 16, 1)
+    for kw in ({"fp32_warm": 1, "switch_tol": 0.0}, {"fp32_warm": 1, "switch_tol": 1.0}, {"fp32_warm": 2},
+               {"key_bits": 16}):
+        with pytest.raises(jkg.JKGError) as e:
+            ctx.dsolve_mixed(Y, J, 8, **kw)
+        assert e.value.code == -10, kw
+    with pytest.raises(jkg.JKGError):  # 32-bit keys only with the warm start
+        ctx.dsolve_mixed(Y, J, 8, key_bits=32)
+    with pytest.raises(jkg.JKGError):  # the warm start is the complex solver's
+        ctx.dsolve_mixed(Y.real.copy(), J, 8, fp32_warm=1)
