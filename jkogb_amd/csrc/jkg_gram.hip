@@ -160,13 +160,103 @@ void gemm_tn_launch(int k, const double *X, int ldx, const int *xcols, int m, co
 }
 
 // The complex Gram product of the FP32 warm start: FP32 entries, FP64 products and sums.
+// The warm start's complex Gram product on the FP32 matrix cores (twice the FP64
+// rate): the tiling of k_gemm_tn with FP32 operands in LDS and FP32 sums over
+// each K chunk; the chunks' planes are added in FP64 by the weight kernel.
+// MFMA f32 16x16x4 lane layout: A[i = l % 16][k = l / 16], B[k = l / 16][j =
+// l % 16], D item t: [i = 4 (l / 16) + t][j = l % 16].
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+template <int CX>
+__global__ void __launch_bounds__(256) k_zgram_f32(int k, int kchunk, long plane, const float *__restrict__ X, int ldx,
+                                                   const int *__restrict__ xcols, int m,
+                                                   const int *__restrict__ ycols, int nc, double *__restrict__ C,
+                                                   long csi, long csj, const StepState *st) {
+  if (!st->active) return;
+  __shared__ float Xs[2][GB_K][GB_LD];
+  __shared__ float Ys[2][GB_K][GB_LD];
+  const int kb = blockIdx.z * kchunk;
+  const int ke = (k - kb < kchunk) ? k : kb + kchunk;
+  C += blockIdx.z * plane;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int i0 = blockIdx.y * GB_M, j0 = blockIdx.x * GB_N;
+  const int lc = tid >> 1, lh = (tid & 1) * 8;
+  const int xi = i0 + lc, yj = j0 + lc;
+  const int xcol = (xi < m) ? xcols[xi] : -1;
+  const int ycol = (yj < nc) ? ycols[yj] : -1;
+  const float *xp = X + (size_t)(xcol < 0 ? 0 : xcol) * ldx;
+  const float *yp = X + (size_t)(ycol < 0 ? 0 : ycol) * ldx;
+  float xr[8], yr[8];
+  auto gload = [&](int r0) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int r = kb + r0 + lh + u;
+      xr[u] = (xcol >= 0 && r < ke && (r >> 1) != xcol) ? xp[r] : 0.0f;
+      yr[u] = (ycol >= 0 && r < ke && (r >> 1) != ycol) ? yp[r] : 0.0f;
+    }
+    if (CX == 2) {
+#pragma unroll
+      for (int u = 0; u < 8; u += 2) {
+        const float re = yr[u];
+        yr[u] = yr[u + 1];
+        yr[u + 1] = -re;
+      }
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      Xs[buf][lh + u][lc] = xr[u];
+      Ys[buf][lh + u][lc] = yr[u];
+    }
+  };
+  f4v acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = (f4v){0.f, 0.f, 0.f, 0.f};
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  const int nst = (ke - kb + GB_K - 1) / GB_K;
+  for (int s = 0; s < nst; ++s) {
+    const int buf = s & 1;
+    if (s + 1 < nst) gload((s + 1) * GB_K);
+#pragma unroll
+    for (int kk = 0; kk < GB_K; kk += 4) {
+      float af[4], bf[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) af[a] = Xs[buf][kk + (lane >> 4)][wm * 64 + a * 16 + (lane & 15)];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) bf[b] = Ys[buf][kk + (lane >> 4)][wn * 64 + b * 16 + (lane & 15)];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[a], bf[b], acc[a][b], 0, 0, 0);
+    }
+    if (s + 1 < nst) sstore(buf ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int i = i0 + wm * 64 + a * 16 + 4 * (lane >> 4) + t;
+        const int j = j0 + wn * 64 + b * 16 + (lane & 15);
+        if (i < m && j < nc) C[i * csi + j * csj] = (double)acc[a][b][t];
+      }
+}
+
 static void zgemm_tn_f32_launch(int k, const float *X, int ldx, const int *xcols, int m, const int *ycols, int nc,
                                 double *C, long csi, long csj, const StepState *st, hipStream_t s, int cx, int ks) {
   if (m <= 0 || nc <= 0) return;
   dim3 grid((nc + GB_N - 1) / GB_N, (m + GB_M - 1) / GB_M, ks);
-  auto kern = (cx == 1) ? k_gemm_tn<true, 1, float> : k_gemm_tn<true, 2, float>;
-  hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, k, kchunk_of(k, ks), (long)m * nc, X, ldx, xcols, m, X, ldx,
-                     ycols, nc, C, csi, csj, st);
+  auto kern = (cx == 1) ? k_zgram_f32<1> : k_zgram_f32<2>;
+  hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, k, kchunk_of(k, ks), (long)m * nc, X, ldx, xcols, m, ycols, nc, C,
+                     csi, csj, st);
 }
 
 // ------------------------------------------------------------------ column norms
