@@ -22,6 +22,12 @@ Printed as one JSON line:
                with the trained step controller (jkogb_amd/ctl_params.npy):
                device time, steps, and the singular values' largest
                relative difference from the plain solver in units of N eps
+  complex      the complex configuration (N=2048, Re/Im ~ N(0, 1/2), J balanced,
+               N_2=1024): per-step device time of the plain, FP64 mixed and FP32
+               warm-start steps (20 steps from the start each), and complete
+               solves of the FP64 mixed solver and of the warm start (FP32 steps
+               until the relative off-norm is below 1e-5, then FP64), with the
+               warm start's SY difference from the FP64 mixed solver's
   batched      4096 independent HSVDs, N uniform in [16, 256], Gaussian Y,
                random J, N_2 = N/2, advanced together by one persistent launch
                per step (jkg_dsolve_batched, tuned grid and variant, tol = eps):
@@ -116,6 +122,48 @@ def timed_steps(ctx, solve, y, j, n2, steps, warmup, **kw):
     return dict(steps=k, ms_per_step=ms, gbs=b / (ms * 1e-3) / 1e9, bytes=b)
 
 
+def complex_run(ctx, fused, log, steps=20):
+    """The complex configuration: per-step times and complete solves of the FP64 mixed solver
+    and of its FP32 warm start."""
+    from jkogb_amd import inputs
+    y, j = inputs.make("c3", seed=0)
+    n = y.shape[0]
+    eps = np.finfo(float).eps
+
+    def per_step(solve, **kw):
+        solve(y, j, n // 2, max_steps=2, fused=fused, **kw)
+        r = solve(y, j, n // 2, max_steps=steps, fused=fused, **kw)
+        return 1e3 * r["seconds"] / r["steps"]
+
+    plain_ms = per_step(ctx.dsolve)
+    mixed_ms = per_step(ctx.dsolve_mixed)
+    warm_ms = per_step(ctx.dsolve_mixed, fp32_warm=1, key_bits=32, switch_tol=1e-300)  # the FP32 phase alone
+    last = [0]
+
+    def prog(step, done):
+        if done or step - last[0] >= 5000:
+            last[0] = step
+            log("complex solve: step %d%s" % (step, " (done)" if done else ""))
+
+    m = ctx.dsolve_mixed(y, j, n // 2, fused=fused, trace_cap=1, progress=prog)
+    last[0] = 0
+    w = ctx.dsolve_mixed(y, j, n // 2, fused=fused, trace_cap=1, progress=prog, fp32_warm=1, key_bits=32)
+    wi = w["warm"]
+    return {"workload": "complex N=2048, Re/Im ~ N(0,1/2), J 1024/1024 permuted, N_2=1024",
+            "ms_per_step": {"plain": round(plain_ms, 4), "mixed_fp64": round(mixed_ms, 4),
+                            "warm_fp32_phase": round(warm_ms, 4)},
+            "step_speedup_mixed": round(plain_ms / mixed_ms, 3),
+            "mixed": {"steps": m["steps"], "seconds": round(m["seconds"], 3)},
+            "warm": {"steps": w["steps"], "seconds": round(w["seconds"], 3), "fp32_steps": wi["steps32"],
+                     "fp32_seconds": round(wi["seconds32"], 3), "switch_seconds": round(wi["seconds_switch"], 4),
+                     "off_rel_at_switch": wi["off_rel"], "newton_schulz": [wi["ns_u"], wi["ns_z"]],
+                     "fallback": wi["fallback"],
+                     "sy_err_vs_mixed_N_eps": round(float(np.max(np.abs(w["SY"] - m["SY"]) / np.abs(m["SY"])))
+                                                    / (n * eps), 4)},
+            "note": "ms_per_step: the first %d steps of each solver (warm_fp32_phase: FP32 steps only); "
+                    "a complete plain solve at this size (~22k steps) is not part of the default run" % steps}
+
+
 def batched_run(ctx, count, fused, sample=8):
     """The batched configuration: count instances, N uniform in [16, 256], solved together with the
     convergence test (tol = eps), against single plain solves of a sample of them."""
@@ -165,6 +213,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-full", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-complex", action="store_true")
     ap.add_argument("--batch-count", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -222,6 +271,10 @@ def main():
                 "note": "sy_err: largest relative SY difference against the plain (reference-exact) solver; "
                         "reference_build_spread: the same between the reference's unfused and fused builds"}
         log("full solves at N=1024")
+    cplx = None
+    if not args.no_complex:
+        cplx = complex_run(ctx, args.fused, log)
+        log("complex: mixed %.1f s, warm start %.1f s" % (cplx["mixed"]["seconds"], cplx["warm"]["seconds"]))
     batched = None
     if not args.no_batched:
         batched = batched_run(ctx, args.batch_count, args.fused)
@@ -281,6 +334,7 @@ def main():
                               "plain": {s: round(v, 4) for s, v in pp.items()}},
         "cpu_baseline": cpu,
         "full_solve": full,
+        "complex": cplx,
         "batched": batched,
         "host_wall_s": round(wall, 3),
     }

@@ -13,7 +13,7 @@
 !     STOP exits with 0) -- the message text is the reference's;
 !   * the CSV is written after the loop, with the mean GPU step time in the
 !     BUILDs column and 0 in TRANSFs (the steps are not timed one by one);
-!   * ID_NCP=3 (best start offset) is not implemented on the GPU.
+!   * ID_NCP=3 (best start offset) is limited to N <= 8192.
 ! Environment: JKG_DEVICE (default 0), JKG_FUSED=1 selects FMAD = fma.
 PROGRAM jk
   USE, INTRINSIC :: ISO_C_BINDING

@@ -15,6 +15,8 @@ MODULE jkg_c
   TYPE, BIND(C) :: jkg_opts
      INTEGER(c_int32_t) :: id_ncp, id_tru, fused, max_steps, steps_per_sync, trace_cap
      TYPE(c_ptr) :: trace
+     TYPE(c_funptr) :: progress = c_null_funptr
+     TYPE(c_ptr) :: user = c_null_ptr
   END TYPE jkg_opts
 
   TYPE, BIND(C) :: jkg_result

@@ -45,13 +45,18 @@ typedef struct {
 } jkg_steprec;
 
 typedef struct {
-  int32_t id_ncp;         /* 0 or 1 or 2: greedy walk (AW_NCP1/2 select the same pivots); 3: JKG_ENOTSUP */
+  int32_t id_ncp;         /* 0, 1 or 2: greedy walk (AW_NCP1/2 select the same pivots); 3: AW_NCP3, N <= 8192 */
   int32_t id_tru;         /* 0, 1 or 2 (pair enumeration; does not change results) */
   int32_t fused;          /* 0: FMAD(a,b,c) = a*b+c, two roundings (the reference's default build); 1: fma */
   int32_t max_steps;      /* < 0: run to convergence; else stop after this many steps */
   int32_t steps_per_sync; /* steps launched between two host polls of the device's done flag; 0: auto */
   int32_t trace_cap;      /* records available in trace */
   jkg_steprec *trace;     /* host buffer for the per-step trace, or NULL */
+  /* progress hook, or NULL: called on the calling thread at every host poll of the
+   * device (every steps_per_sync steps) with the index of the current step and
+   * whether the loop has ended; for long solves that report as they go */
+  void (*progress)(void *user, int32_t step, int32_t done);
+  void *user;
 } jkg_opts;
 
 typedef struct {
@@ -97,11 +102,15 @@ int jkg_zsolve(jkg_ctx *ctx, int n, int n2, const double *y, const int64_t *j, d
  *            descending); info >= 0 on success.
  * jkg_ncp  = ANCP (AW_NCP1/AW_NCP2): greedy disjoint pivots from the sorted
  *            list; step[0..*sl) are 1-based list positions.
+ * jkg_ncp3 = ANCP bound to AW_NCP3 (src/shared/atypes.F90:488-592, ID_NCP=3):
+ *            the greedy walk from every start offset, the selection of the
+ *            largest total weight (ties: the smallest start); N <= 8192.
  * jkg_tru  = ATRU (TRU1/TRU2, src/shared/jstep.F90:9-120); returns TT or -2.
  * jkg_jstep_len = JSTEP_LEN (src/shared/jstep.F90:124-133).
  * --------------------------------------------------------------------------- */
 int jkg_srt(jkg_ctx *ctx, int nn, int nm, jkg_aw *dz, int64_t *info);
 int jkg_ncp(jkg_ctx *ctx, int n, int nn, int nm, jkg_aw *dz, int n2, int32_t *sl, int32_t *step, int64_t *info);
+int jkg_ncp3(jkg_ctx *ctx, int n, int nn, int nm, jkg_aw *dz, int n2, int32_t *sl, int32_t *step, int64_t *info);
 int jkg_tru(int id_tru, int n, const int32_t *j, int nn, int32_t *p, int32_t *q, int32_t pn[2]);
 int jkg_jstep_len(int n, int n2);
 

@@ -44,11 +44,12 @@ struct jkg_ctx {
   hipGraphExec_t exec[2] = {nullptr, nullptr};
   // host values the captured graph holds as launch shapes or kernel arguments
   struct GraphKey {
-    int fused = -1, wmode = -1, npl = -1, ctl_mode = -1, ctl_cap = -1, cut32 = -1;
+    int fused = -1, wmode = -1, npl = -1, ctl_mode = -1, ctl_cap = -1, cut32 = -1, ncp3 = -1;
     double neg_tol = -1.0, switch_tol = -1.0;
     bool operator==(const GraphKey &o) const {
       return fused == o.fused && wmode == o.wmode && npl == o.npl && neg_tol == o.neg_tol &&
-             ctl_mode == o.ctl_mode && ctl_cap == o.ctl_cap && cut32 == o.cut32 && switch_tol == o.switch_tol;
+             ctl_mode == o.ctl_mode && ctl_cap == o.ctl_cap && cut32 == o.cut32 && switch_tol == o.switch_tol &&
+             ncp3 == o.ncp3;
     }
   } gkey[2];
   StepState st_init{};  // the loop state of the current problem before its first step
@@ -59,6 +60,7 @@ struct jkg_ctx {
   int warm_n = 0;
   void *wm_M = nullptr, *wm_T = nullptr, *wm_P = nullptr;
   double *wm_part = nullptr;
+  int ncp3_n = 0;  // ID_NCP=3: per-offset totals allocated for this N
   // step controller and its training (allocated on first use, kept across problems)
   float *ctl_master = nullptr, *ctl_table = nullptr, *ctl_trace = nullptr;
   uint16_t *ctl_q = nullptr;
@@ -81,6 +83,7 @@ static void free_all(jkg_ctx *c) {
   c->warm_n = 0;
   c->wm_M = c->wm_T = c->wm_P = nullptr;
   c->wm_part = nullptr;
+  c->ncp3_n = 0;
   for (void *p : c->allocs) (void)hipFree(p);
   c->allocs.clear();
   for (void *p : c->mallocs) (void)hipFree(p);
@@ -221,6 +224,23 @@ static int ensure_warm(jkg_ctx *c) {
   c->wm_part = (double *)p[6];
   c->warm_n = d.n;
   return JKG_OK;
+}
+
+// ID_NCP=3: the per-offset totals (NN doubles) and the walk kernel's LDS size.
+static int ensure_ncp3(jkg_ctx *c) {
+  DeviceStep &d = c->d;
+  if (d.n > NCP3_NMAX) return JKG_ENOTSUP;
+  if (c->ncp3_n != d.n) {
+    void *w = nullptr;
+    if (hipMalloc(&w, sizeof(double) * (size_t)(d.nn > 0 ? d.nn : 1)) != hipSuccess) {
+      (void)hipGetLastError();
+      return JKG_ENOMEM;
+    }
+    c->mallocs.push_back(w);
+    d.ncp3_wsum = (double *)w;
+    c->ncp3_n = d.n;
+  }
+  return ncp3_prepare(d.n, d.n2max) ? JKG_ENOTSUP : JKG_OK;
 }
 
 // Controller parameters (FP32 master + BF16 copy), phase table and per-step trace.
@@ -439,6 +459,7 @@ static int capture_step(jkg_ctx *c, int fused) {
   key.ctl_mode = c->d.ctl_mode;
   key.ctl_cap = c->d.ctl_trace_cap;
   key.cut32 = slot ? c->d.cut32 : 0;
+  key.ncp3 = c->d.ncp3;
   key.switch_tol = slot ? c->d.switch_tol : 0.0;
   if (c->exec[slot] && c->gkey[slot] == key) return JKG_OK;
   if (c->exec[slot]) (void)hipGraphExecDestroy(c->exec[slot]);
@@ -469,9 +490,11 @@ static int launch_until_done(jkg_ctx *c, const jkg_opts *o, long &remaining, lon
     launched += nb;
     if (remaining >= 0) remaining -= nb;
     if (remaining == 0) return JKG_OK;
-    int done = 0;
-    HIPCHK(hipMemcpyAsync(&done, &d.st->done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    int head[3] = {0, 0, 0};  // StepState: step, active, done
+    HIPCHK(hipMemcpyAsync(head, d.st, sizeof(head), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    const int done = head[2];
+    if (o->progress) o->progress(o->user, head[0], done);
     if (done) return JKG_OK;
     if (launched > cap) return JKG_ENOCONV;
     if (o->steps_per_sync <= 0 && batch < 64) batch *= 2;
@@ -673,7 +696,6 @@ static int solve_common(jkg_ctx *c, int cplx, int n, int n2, const void *y, cons
   jkg_opts o;
   if (opts) o = *opts;
   else jkg_opts_default(&o);
-  if (o.id_ncp == 3) return JKG_ENOTSUP;
   if (o.id_ncp < 0 || o.id_ncp > 3) return -9;
   if (o.id_tru < 0 || o.id_tru > 2) return -9;
   memset(res, 0, sizeof(*res));
@@ -693,6 +715,12 @@ static int solve_common(jkg_ctx *c, int cplx, int n, int n2, const void *y, cons
   if (mc.warm) {
     rc = ensure_warm(c);
     if (rc) return rc;
+  }
+  c->d.ncp3 = 0;
+  if (o.id_ncp == 3) {
+    rc = ensure_ncp3(c);
+    if (rc) return rc;
+    c->d.ncp3 = 1;
   }
   c->d.ctl_mode = mc.ctl_mode;
   c->d.ctl_trace = nullptr;
@@ -910,6 +938,7 @@ static int prepare_single(jkg_ctx *c, int n, int n2, const void *a, const int64_
   c->d.wmode = 0;
   c->d.neg_tol = 0.0;
   c->d.ctl_mode = 0;
+  c->d.ncp3 = 0;
   int npos, nneg;
   return upload_problem(c, n, jkg_jstep_len(n, n2), a, j, &npos, &nneg, -1, c->trace_cap);
 }
@@ -1120,6 +1149,15 @@ extern "C" int jkg_srt(jkg_ctx *c, int nn, int nm, jkg_aw *dz, int64_t *info) {
 }
 
 // ANCP (AW_NCP1/AW_NCP2): the device's greedy walk over a given sorted list.
+// weight_key() on the host: IEEE bits mapped to an unsigned key that grows as w decreases (-0 -> +0)
+static uint64_t host_weight_key(double w) {
+  uint64_t u;
+  memcpy(&u, &w, 8);
+  if (w == 0.0) u = 0;
+  const uint64_t ord = (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+  return ~ord;
+}
+
 extern "C" int jkg_ncp(jkg_ctx *c, int n, int nn, int nm, jkg_aw *dz, int n2, int32_t *sl, int32_t *step,
                        int64_t *info) {
   if (!c) return -1;
@@ -1144,12 +1182,7 @@ extern "C" int jkg_ncp(jkg_ctx *c, int n, int nn, int nm, jkg_aw *dz, int n2, in
   std::vector<uint64_t> key(nn);
   std::vector<uint32_t> val(nn), pos(nn);
   for (int i = 0; i < nn; ++i) {
-    double w = dz[i].w;
-    uint64_t u;
-    memcpy(&u, &w, 8);
-    if (w == 0.0) u = 0;
-    uint64_t ord = (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
-    key[i] = ~ord;
+    key[i] = host_weight_key(dz[i].w);
     val[i] = pack_pq(dz[i].p - 1, dz[i].q - 1);
     pos[i] = (uint32_t)(i + 1);
   }
@@ -1177,6 +1210,55 @@ extern "C" int jkg_ncp(jkg_ctx *c, int n, int nn, int nm, jkg_aw *dz, int n2, in
   return JKG_OK;
 }
 
+// AW_NCP3 (src/shared/atypes.F90:488-592) behind the ANCP interface: the same
+// arguments and checks as jkg_ncp; the selection is the best total weight over
+// every start offset of the sorted list.
+extern "C" int jkg_ncp3(jkg_ctx *c, int n, int nn, int nm, jkg_aw *dz, int n2, int32_t *sl, int32_t *step,
+                        int64_t *info) {
+  if (!c) return -1;
+  *sl = 0;
+  *info = 0;
+  if (n < 0) { *info = -1; return -2; }
+  if (nn < 0) { *info = -2; return -3; }
+  if (nm < nn) { *info = -3; return -4; }
+  if (n2 < 0) { *info = -5; return -7; }
+  if (n == 0 || nn == 0 || n2 == 0) return JKG_OK;
+  for (int i = 0; i < nn; ++i)
+    if (!(dz[i].w == dz[i].w) || dz[i].p < 1 || dz[i].q <= dz[i].p || dz[i].q > n) return -5;
+  HIPCHK(hipSetDevice(c->device));
+  int nb = std::max(n, 3);
+  int rc = ensure_buffers(c, nb, 0, 1);
+  if (rc) return rc;
+  if ((size_t)nn > (size_t)c->d.nn || n2 > c->d.n2max) return -3;
+  rc = ensure_ncp3(c);
+  if (rc) return rc;
+  DeviceStep &d = c->d;
+  std::vector<uint64_t> key(nn);
+  std::vector<uint32_t> val(nn);
+  for (int i = 0; i < nn; ++i) {
+    key[i] = host_weight_key(dz[i].w);
+    val[i] = pack_pq(dz[i].p - 1, dz[i].q - 1);
+  }
+  // the list as a finished sort: every pass skipped, values given
+  const int skip[9] = {1, 1, 1, 1, 1, 1, 1, 1, 1};
+  HIPCHK(hipMemcpyAsync(d.sort.k0, key.data(), 8 * (size_t)nn, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d.sort.v0, val.data(), 4 * (size_t)nn, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d.sort_skip, skip, sizeof(skip), hipMemcpyHostToDevice, c->stream));
+  StepState s{};
+  s.n2 = n2;
+  s.active = 1;
+  HIPCHK(hipMemcpyAsync(d.st, &s, sizeof(s), hipMemcpyHostToDevice, c->stream));
+  DeviceStep dd = d;
+  dd.nn = nn;  // the binary search for IT runs over the given list
+  ncp3_launch(dd, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(&s, d.st, sizeof(s), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  *sl = s.sl;
+  if (s.sl > 0) HIPCHK(hipMemcpy(step, d.piv_pos, sizeof(int) * s.sl, hipMemcpyDeviceToHost));
+  return JKG_OK;
+}
+
 // ---------------------------------------------------------------- profiling
 static int profile_common(jkg_ctx *c, int n, int n2, const double *y, const int64_t *j, int fused, int steps,
                           double *stage_ms, jkg_result *res, int wmode, double tol = 0.0, int cplx = 0) {
@@ -1191,6 +1273,7 @@ static int profile_common(jkg_ctx *c, int n, int n2, const double *y, const int6
   c->d.wmode = wmode;
   c->d.neg_tol = tol;
   c->d.ctl_mode = 0;
+  c->d.ncp3 = 0;
   if (wmode) {
     rc = ensure_mixed(c);
     if (rc) return rc;

@@ -841,6 +841,10 @@ void weights_launch(DeviceStep &d, hipStream_t s) {
 constexpr int SEL_ROUNDS = 2;
 
 void select_launch(DeviceStep &d, hipStream_t s) {
+  if (d.ncp3) {
+    ncp3_launch(d, s);
+    return;
+  }
   const int ntiles = (d.nn + FILT_TILE - 1) / FILT_TILE;
   const size_t lds = (size_t)d.n + 1;
   for (int r = 0; r < SEL_ROUNDS; ++r) {

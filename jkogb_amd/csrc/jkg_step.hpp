@@ -33,6 +33,7 @@ struct Rot {
 };
 
 struct StepState {
+  // (the host polls the first three fields in one copy)
   int step;       // index S of the step being executed
   int active;     // this step runs (set at step begin: !done)
   int done;       // the loop has ended
@@ -114,7 +115,14 @@ struct DeviceStep {
   void *A32, *U32, *Z32;
   int prec32, cut32;
   double switch_tol;
+  // ID_NCP=3 (jkg_ncp3.hip): the best-of-all-offsets selection replaces the greedy walk
+  int ncp3;
+  double *ncp3_wsum;      // NN: total weight of the walk from each start offset
 };
+
+constexpr int NCP3_NMAX = 8192;
+int ncp3_prepare(int n, int n2max);
+void ncp3_launch(DeviceStep &d, hipStream_t s);
 
 constexpr int WARM_STALL = 64;
 

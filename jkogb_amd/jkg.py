@@ -35,9 +35,23 @@ STEPREC_DTYPE = np.dtype([("step", "<i4"), ("oldlen", "<i4"), ("newlen", "<i4"),
                           ("hyp", "<i4"), ("it", "<i4"), ("tw", "<f8")])
 
 
+PROGRESS = C.CFUNCTYPE(None, C.c_void_p, C.c_int32, C.c_int32)
+
+
 class Opts(C.Structure):
     _fields_ = [("id_ncp", C.c_int32), ("id_tru", C.c_int32), ("fused", C.c_int32), ("max_steps", C.c_int32),
-                ("steps_per_sync", C.c_int32), ("trace_cap", C.c_int32), ("trace", C.c_void_p)]
+                ("steps_per_sync", C.c_int32), ("trace_cap", C.c_int32), ("trace", C.c_void_p),
+                ("progress", PROGRESS), ("user", C.c_void_p)]
+
+
+def _hook(o, progress):
+    """Installs progress(step, done) as the solve's progress hook; returns the ctypes
+    callback, which the caller keeps alive for the call."""
+    if progress is None:
+        return None
+    cb = PROGRESS(lambda user, step, done: progress(int(step), bool(done)))
+    o.progress = cb
+    return cb
 
 
 class Result(C.Structure):
@@ -101,6 +115,7 @@ _dsolve = _sig("jkg_dsolve", _i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, C.POINTE
 _zsolve = _sig("jkg_zsolve", _i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, C.POINTER(Opts), C.POINTER(Result)])
 _srt = _sig("jkg_srt", _i, [_vp, _i, _i, _vp, C.POINTER(C.c_int64)])
 _ncp = _sig("jkg_ncp", _i, [_vp, _i, _i, _i, _vp, _i, C.POINTER(C.c_int32), _vp, C.POINTER(C.c_int64)])
+_ncp3 = _sig("jkg_ncp3", _i, [_vp, _i, _i, _i, _vp, _i, C.POINTER(C.c_int32), _vp, C.POINTER(C.c_int64)])
 _tru = _sig("jkg_tru", _i, [_i, _i, _vp, _i, _vp, _vp, _vp])
 _jstep_len = _sig("jkg_jstep_len", _i, [_i, _i])
 _dhsvd2 = _sig("jkg_dhsvd2", _i, [_vp, _i, _vp, _vp, _d, _i, _vp, _vp, _vp, _vp])
@@ -135,7 +150,7 @@ EXPORTS = ["jkg_create", "jkg_destroy", "jkg_strerror", "jkg_last_hip_error", "j
            "jkg_dweights", "jkg_zweights", "jkg_dbuild", "jkg_dstep", "jkg_dprofile", "jkg_pair_index",
            "jkg_key_weight", "jkg_mixed_opts_default", "jkg_dsolve_mixed", "jkg_dweights_mixed",
            "jkg_dprofile_mixed", "jkg_ctl_grad", "jkg_ctl_fit", "jkg_batch_opts_default", "jkg_dsolve_batched",
-           "jkg_zsolve_mixed", "jkg_zweights_mixed", "jkg_zprofile", "jkg_zprofile_mixed"]
+           "jkg_zsolve_mixed", "jkg_zweights_mixed", "jkg_zprofile", "jkg_zprofile_mixed", "jkg_ncp3"]
 
 
 def mixed_opts(**kw):
@@ -228,8 +243,9 @@ class Context:
 
     # ---------------------------------------------------------------- solve
     def dsolve(self, Y, J, n2=0, id_ncp=0, id_tru=0, fused=False, max_steps=-1, trace_cap=None,
-               steps_per_sync=0):
-        """HSVD of a real Y (N x N): dict(YU, ZZ, SY, info, trace, tt, pn, n2, seconds)."""
+               steps_per_sync=0, progress=None):
+        """HSVD of a real or complex Y (N x N): dict(YU, ZZ, SY, info, trace, tt, pn, n2, seconds).
+        progress(step, done), if given, is called at every host poll of the device."""
         cplx = np.iscomplexobj(Y)
         dt = np.complex128 if cplx else np.float64
         Y = np.asfortranarray(Y, dtype=dt)
@@ -244,6 +260,7 @@ class Context:
             trace_cap = 64 * n + 64 if max_steps < 0 else max_steps + 1
         trace = np.zeros(max(trace_cap, 1), STEPREC_DTYPE)
         o = Opts(id_ncp, id_tru, 1 if fused else 0, max_steps, steps_per_sync, trace_cap, _p(trace))
+        keep = _hook(o, progress)
         r = Result()
         f = _zsolve if cplx else _dsolve
         _chk(f(self._h, n, n2, _p(Y), _p(J64), _p(YU), _p(ZZ), _p(SY), C.byref(o), C.byref(r)),
@@ -252,7 +269,7 @@ class Context:
                     pn=(r.npos, r.nneg), n2=r.n2, seconds=r.seconds, steps=r.steps, **_counts(r))
 
     def dsolve_mixed(self, Y, J, n2=0, fused=False, max_steps=-1, trace_cap=None, steps_per_sync=0,
-                     ctl_params=None, ctl_table=None, ctl_trace=False, **mixed):
+                     ctl_params=None, ctl_table=None, ctl_trace=False, progress=None, **mixed):
         """The mixed solver (jkg_dsolve_mixed, or jkg_zsolve_mixed for a complex Y): same outputs
         as dsolve, agreeing to rounding.  ctl_params (JKG_CTL_NPARAM floats): the MLP controller;
         ctl_table (8 floats): a phase table; ctl_trace: also return the per-step controller
@@ -272,7 +289,7 @@ class Context:
         trace = np.zeros(max(trace_cap, 1), STEPREC_DTYPE)
         o = Opts(0, 0, 1 if fused else 0, max_steps, steps_per_sync, trace_cap, _p(trace))
         m = mixed_opts(**mixed)
-        keep = []
+        keep = [_hook(o, progress)]
         if ctl_params is not None:
             cp = np.ascontiguousarray(ctl_params, dtype=np.float32)
             if cp.size != CTL_NPARAM:
@@ -409,13 +426,15 @@ class Context:
         _chk(_srt(self._h, nn, len(dz), _p(dz), C.byref(info)), "jkg_srt")
         return info.value
 
-    def ncp(self, n, dz, n2, nn=None):
-        """ANCP: greedy disjoint pivots of the sorted dz[:nn]; returns 1-based positions."""
+    def ncp(self, n, dz, n2, nn=None, id_ncp=2):
+        """ANCP: disjoint pivots of the sorted dz[:nn] -- the greedy walk (id_ncp 1 or 2) or the best
+        total weight over all start offsets (id_ncp 3, AW_NCP3); returns 1-based positions."""
         nn = len(dz) if nn is None else nn
         sl = C.c_int32(0)
         step = np.zeros(max(n2, 1), np.int32)
         info = C.c_int64(0)
-        _chk(_ncp(self._h, n, nn, len(dz), _p(dz), n2, C.byref(sl), _p(step), C.byref(info)), "jkg_ncp")
+        f, name = (_ncp3, "jkg_ncp3") if id_ncp == 3 else (_ncp, "jkg_ncp")
+        _chk(f(self._h, n, nn, len(dz), _p(dz), n2, C.byref(sl), _p(step), C.byref(info)), name)
         return step[:sl.value].copy()
 
     # ---------------------------------------------------------------- step level

@@ -333,3 +333,63 @@ def test_context_reuse_with_another_j_split(ctx, orc):
         assert out["info"] == ref["info"]
         assert same_bits(out["SY"], ref["SY"])
         assert same_bits(out["YU"], ref["YU"])
+
+
+def test_progress_hook(ctx):
+    # the hook sees the current step at every poll, increasing, and the end exactly once
+    calls = []
+    Y, J = inputs("c2", 64, 5)
+    r = ctx.dsolve(Y, J, 32, steps_per_sync=8, progress=lambda s, d: calls.append((s, d)))
+    assert calls and calls[-1][1] and sum(d for _, d in calls) == 1
+    assert all(a[0] <= b[0] for a, b in zip(calls, calls[1:]))
+    assert calls[-1][0] == r["info"]
+
+
+# --------------------------------------------------------------- ID_NCP=3
+def test_ncp3_plugin_bit_exact(ctx, orc):
+    # AW_NCP3 behind the ANCP plug-in against the oracle, with ties, +-0 and negative weights
+    import ctypes as C
+    rng = np.random.default_rng(33)
+    for n in (5, 12, 40, 97):
+        pairs = [(p, q) for q in range(2, n + 1) for p in range(1, q)]
+        for trial in range(3):
+            w = np.round(rng.standard_normal(len(pairs)), 1)
+            w[rng.random(len(pairs)) < 0.15] = -0.0
+            perm = rng.permutation(len(pairs))
+            dz = np.zeros(len(pairs), orc.AW_DTYPE)
+            for k, i in enumerate(perm):
+                p, q = pairs[i]
+                dz[k] = (w[i], p, q, q - p, k + 1)
+            a = dz.copy()
+            ctx.srt(a)
+            n2 = [n // 2, max(1, n // 3), 1][trial]
+            step = ctx.ncp(n, a, n2, id_ncp=3)
+            b = np.zeros(2 * len(a), orc.AW_DTYPE)
+            b[:len(a)] = a
+            sl = C.c_int32(0)
+            st = np.zeros(n2, np.int32)
+            orc.lib().orc_ncp.restype = C.c_int64
+            orc.lib().orc_ncp(3, n, len(a), len(b), orc._p(b), n2, C.byref(sl), orc._p(st))
+            assert np.array_equal(step, st[:sl.value]), (n, trial)
+
+
+def test_ncp3_solve_bit_exact_vs_reference(ctx):
+    # a whole djk.exe run with ID_NCP=3 (the reference's own build)
+    g = golden("solve_c1_r24_ncp3.npz")
+    r = ctx.dsolve(g["Y"], g["J"], int(g["n2"]), id_ncp=int(g["id_ncp"]), id_tru=int(g["id_tru"]))
+    check_solve(r, g["SY"], g["YU"], g["ZZ"], g["trace"], int(g["steps"]))
+
+
+@pytest.mark.parametrize("cfg,n,fused", [("c1", 32, False), ("c2", 40, True), ("c5", 37, False)])
+def test_ncp3_solve_bit_exact_vs_oracle(ctx, orc, cfg, n, fused):
+    y, j = inputs(cfg, n, 41)
+    r = ctx.dsolve(y, j, n // 2, id_ncp=3, fused=fused)
+    o = orc.solve(y, j, n // 2, id_ncp=3, fused=fused)
+    check_solve(r, o["SY"], o["YU"], o["ZZ"], steps=o["info"])
+
+
+def test_ncp3_complex_solve_bit_exact_vs_oracle(ctx, orc):
+    y, j = inputs("c3", 20, 42)
+    r = ctx.dsolve(y, j, 10, id_ncp=3)
+    o = orc.solve(y, j, 10, id_ncp=3)
+    check_solve(r, o["SY"], o["YU"], o["ZZ"], steps=o["info"])
