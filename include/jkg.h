@@ -245,6 +245,21 @@ int jkg_zprofile_mixed(jkg_ctx *ctx, int n, int n2, const double *y, const int64
                        const jkg_mixed_opts *m, double *stage_ms, jkg_result *res);
 
 /* ---------------------------------------------------------------------------
+ * Test matrices with a prescribed hyperbolic spectrum (the role of the
+ * reference's jkdatgen, src/var/old/jkdatgen/jkdatgen.F90:1-274 and xa2g.F90:
+ * a factor G of A = G J G^T with given eigenvalues).  jkg_dgen writes an
+ * N x N column-major g and j = sign(lam) such that g diag(j) g^T =
+ * Q diag(lam) Q^T for an orthogonal Q, so the hyperbolic singular values of
+ * (g, j) are sqrt|lam|: g = Q diag(sqrt|lam|) H, Q and H (H^T J H = J)
+ * products of `rounds` rounds of random 2x2 rotations on random matchings
+ * (rounds <= 0: 2 ceil(log2 N) + 2), hyperbolic angles uniform in
+ * [-tmax, tmax] (tmax bounds the conditioning of H).  lam: N nonzero finite
+ * values; the same seed gives the same g.  No counterpart of the reference's
+ * extended-precision Bunch-Parlett factorisation: same contract, other bits.
+ * --------------------------------------------------------------------------- */
+int jkg_dgen(jkg_ctx *ctx, int n, const double *lam, int rounds, double tmax, uint64_t seed, double *g, int64_t *j);
+
+/* ---------------------------------------------------------------------------
  * Batched solver: count independent real HSVDs of different sizes, each with
  * djk's semantics and arithmetic (YU, ZZ, SY and INFO of instance i are what
  * jkg_dsolve returns for it alone), advanced together by one persistent launch

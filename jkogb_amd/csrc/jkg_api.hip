@@ -1259,6 +1259,91 @@ extern "C" int jkg_ncp3(jkg_ctx *c, int n, int nn, int nm, jkg_aw *dz, int n2, i
   return JKG_OK;
 }
 
+// ---------------------------------------------------------------- test matrices
+namespace {
+uint64_t splitmix64(uint64_t &x) {
+  uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+double unit(uint64_t &x) { return (double)(splitmix64(x) >> 11) * 0x1p-53; }  // [0, 1)
+}  // namespace
+
+// G = Q diag(sqrt|lam|) H with Q orthogonal, H J-orthogonal, J = sign(lam): the
+// hyperbolic singular values of (G, J) are sqrt|lam| (see jkg_gen.hip).
+extern "C" int jkg_dgen(jkg_ctx *c, int n, const double *lam, int rounds, double tmax, uint64_t seed, double *g,
+                        int64_t *j) {
+  if (!c) return -1;
+  if (n < 1) return -2;
+  if (!lam) return -3;
+  if (!(tmax >= 0.0) || tmax > 20.0) return -5;
+  if (!g) return -7;
+  if (!j) return -8;
+  for (int i = 0; i < n; ++i)
+    if (!(lam[i] == lam[i]) || lam[i] == 0.0 || __builtin_isinf(lam[i])) return -3;
+  if (rounds <= 0) {
+    int lg = 0;
+    while ((1 << lg) < n) ++lg;
+    rounds = 2 * lg + 2;
+  }
+  HIPCHK(hipSetDevice(c->device));
+  if (gen_rows_prepare(n)) return JKG_ENOTSUP;
+  const int np = n / 2;
+  const size_t bG = sizeof(double) * (size_t)n * n, bd = sizeof(double) * n, bJ = sizeof(int) * n,
+               bp = sizeof(int) * 2 * (size_t)np * 2 * rounds, bc = sizeof(double) * 2 * (size_t)np * 2 * rounds;
+  int rc = tmp_bytes(c, bG + bd + bJ + bp + bc + 256);
+  if (rc) return rc;
+  char *base = (char *)c->tmp;
+  double *dG = (double *)base, *dd = (double *)(base + bG), *dcs = (double *)(base + bG + bd);
+  int *dJ = (int *)(base + bG + bd + bc), *dp = (int *)(base + bG + bd + bc + bJ);
+  std::vector<double> d(n), cs(2 * (size_t)np * 2 * rounds);
+  std::vector<int> J(n), pairs(2 * (size_t)np * 2 * rounds), perm(n);
+  for (int i = 0; i < n; ++i) {
+    d[i] = std::sqrt(std::fabs(lam[i]));
+    J[i] = lam[i] > 0.0 ? 1 : -1;
+    j[i] = J[i];
+  }
+  // 2 * rounds matchings: rounds for H (columns), then rounds for Q (rows); pairs as
+  // [round][a[0..np), b[0..np)], angles as [round][(c, s) x np]
+  uint64_t x = seed;
+  for (int r = 0; r < 2 * rounds; ++r) {
+    for (int i = 0; i < n; ++i) perm[i] = i;
+    for (int i = n - 1; i > 0; --i) std::swap(perm[i], perm[(int)(splitmix64(x) % (uint64_t)(i + 1))]);
+    int *pa = &pairs[(size_t)r * 2 * np], *pb = pa + np;
+    double *rcs = &cs[(size_t)r * 2 * np];
+    for (int k = 0; k < np; ++k) {
+      pa[k] = perm[2 * k];
+      pb[k] = perm[2 * k + 1];
+      const bool hyp = r < rounds && J[pa[k]] != J[pb[k]];
+      if (hyp) {
+        const double t = tmax * (2.0 * unit(x) - 1.0);
+        rcs[2 * k] = std::cosh(t);
+        rcs[2 * k + 1] = std::sinh(t);
+      } else {
+        const double t = M_PI * (2.0 * unit(x) - 1.0);
+        rcs[2 * k] = std::cos(t);
+        rcs[2 * k + 1] = std::sin(t);
+      }
+    }
+  }
+  HIPCHK(hipMemcpyAsync(dd, d.data(), bd, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(dJ, J.data(), bJ, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(dp, pairs.data(), bp, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(dcs, cs.data(), bc, hipMemcpyHostToDevice, c->stream));
+  gen_diag_launch(n, dd, dG, c->stream);
+  for (int r = 0; r < rounds; ++r)
+    gen_cols_launch(n, np, dG, dp + (size_t)r * 2 * np, dp + (size_t)r * 2 * np + np, dcs + (size_t)r * 2 * np, dJ,
+                    c->stream);
+  for (int r = rounds; r < 2 * rounds; ++r)
+    gen_rows_launch(n, np, dG, dp + (size_t)r * 2 * np, dp + (size_t)r * 2 * np + np, dcs + (size_t)r * 2 * np,
+                    c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(g, dG, bG, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return JKG_OK;
+}
+
 // ---------------------------------------------------------------- profiling
 static int profile_common(jkg_ctx *c, int n, int n2, const double *y, const int64_t *j, int fused, int steps,
                           double *stage_ms, jkg_result *res, int wmode, double tol = 0.0, int cplx = 0) {

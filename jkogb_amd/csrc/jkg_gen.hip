@@ -1,0 +1,93 @@
+// jkg_gen.hip -- test matrices with a prescribed hyperbolic spectrum (the role
+// of the reference's jkdatgen, src/var/old/jkdatgen/jkdatgen.F90:1-274,
+// gendat.F90, xa2g.F90: a factor G of a symmetric A = G J G^T whose eigenvalues
+// lambda are given, so the hyperbolic singular values of (G, J) are sqrt|lambda|).
+//
+// The reference forms A = U diag(lambda) U^T (XLAGSY) and factors it by
+// Bunch-Parlett in extended precision (XSYBPC).  Here G is built directly:
+//   G = Q diag(sqrt|lambda|) H,   J = sign(lambda),
+// with Q orthogonal and H J-orthogonal (H^T J H = J), both products of rounds
+// of 2x2 rotations on random perfect matchings of the indices: Q by Givens
+// rotations of rows, H by column transforms that are hyperbolic (cosh, sinh)
+// for a pair of opposite signs and trigonometric otherwise.  Then
+// G J G^T = Q diag(lambda) Q^T exactly in exact arithmetic; in FP64 each round
+// adds O(eps) relative to the row and column norms.  Rounds are HBM-bound
+// passes: one workgroup per column pair (H) or per column (Q, every pair of
+// the round applied in LDS).
+#include "jkg_device.hpp"
+#include "jkg_step.hpp"
+
+namespace jkg {
+
+// G = diag(d) (column-major, leading dimension n)
+__global__ void k_gen_diag(int n, const double *__restrict__ d, double *__restrict__ G) {
+  const size_t total = (size_t)n * n;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t c = e / n, r = e % n;
+    G[e] = (r == c) ? d[r] : 0.0;
+  }
+}
+
+// One round of column transforms: pair k = (a, b) with (c, s); hyperbolic if J_a != J_b:
+//   [x y] <- [x y] [[c, s], [s, c]]  (c = cosh t, s = sinh t)
+//   [x y] <- [x y] [[c, -s], [s, c]] (c = cos t,  s = sin t)
+__global__ void __launch_bounds__(256) k_gen_cols(int n, double *__restrict__ G, const int *__restrict__ pa,
+                                                  const int *__restrict__ pb, const double *__restrict__ cs,
+                                                  const int *__restrict__ J) {
+  const int k = blockIdx.x;
+  const int a = pa[k], b = pb[k];
+  const double c = cs[2 * k], s = cs[2 * k + 1];
+  const bool hyp = J[a] != J[b];
+  double *x = G + (size_t)a * n, *y = G + (size_t)b * n;
+  for (int r = threadIdx.x; r < n; r += blockDim.x) {
+    const double xv = x[r], yv = y[r];
+    x[r] = __builtin_fma(c, xv, s * yv);
+    y[r] = hyp ? __builtin_fma(s, xv, c * yv) : __builtin_fma(c, yv, -(s * xv));
+  }
+}
+
+// One round of row rotations, applied to every column (one workgroup per column, the
+// column staged in LDS): (g_a, g_b) <- (c g_a - s g_b, s g_a + c g_b) for every pair.
+__global__ void __launch_bounds__(256) k_gen_rows(int n, int npairs, double *__restrict__ G,
+                                                  const int *__restrict__ pa, const int *__restrict__ pb,
+                                                  const double *__restrict__ cs) {
+  extern __shared__ double col[];
+  double *g = G + (size_t)blockIdx.x * n;
+  for (int r = threadIdx.x; r < n; r += blockDim.x) col[r] = g[r];
+  __syncthreads();
+  for (int k = threadIdx.x; k < npairs; k += blockDim.x) {
+    const int a = pa[k], b = pb[k];
+    const double c = cs[2 * k], s = cs[2 * k + 1];
+    const double ga = col[a], gb = col[b];
+    col[a] = __builtin_fma(c, ga, -(s * gb));
+    col[b] = __builtin_fma(s, ga, c * gb);
+  }
+  __syncthreads();
+  for (int r = threadIdx.x; r < n; r += blockDim.x) g[r] = col[r];
+}
+
+void gen_diag_launch(int n, const double *d, double *G, hipStream_t s) {
+  size_t g = ((size_t)n * n + 255) / 256;
+  hipLaunchKernelGGL(k_gen_diag, dim3((unsigned)(g > 8192 ? 8192 : g)), dim3(256), 0, s, n, d, G);
+}
+
+void gen_cols_launch(int n, int npairs, double *G, const int *pa, const int *pb, const double *cs, const int *J,
+                     hipStream_t s) {
+  if (npairs > 0) hipLaunchKernelGGL(k_gen_cols, dim3(npairs), dim3(256), 0, s, n, G, pa, pb, cs, J);
+}
+
+int gen_rows_prepare(int n) {
+  const size_t lds = sizeof(double) * (size_t)n;
+  if (lds > 160 * 1024) return -1;
+  return hipFuncSetAttribute((const void *)k_gen_rows, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) ==
+                 hipSuccess
+             ? 0
+             : -2;
+}
+
+void gen_rows_launch(int n, int npairs, double *G, const int *pa, const int *pb, const double *cs, hipStream_t s) {
+  if (npairs > 0)
+    hipLaunchKernelGGL(k_gen_rows, dim3(n), dim3(256), sizeof(double) * n, s, n, npairs, G, pa, pb, cs);
+}
+
+}  // namespace jkg

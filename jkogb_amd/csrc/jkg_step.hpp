@@ -120,6 +120,13 @@ struct DeviceStep {
   double *ncp3_wsum;      // NN: total weight of the walk from each start offset
 };
 
+// test-matrix generator (jkg_gen.hip)
+void gen_diag_launch(int n, const double *d, double *G, hipStream_t s);
+void gen_cols_launch(int n, int npairs, double *G, const int *pa, const int *pb, const double *cs, const int *J,
+                     hipStream_t s);
+int gen_rows_prepare(int n);
+void gen_rows_launch(int n, int npairs, double *G, const int *pa, const int *pb, const double *cs, hipStream_t s);
+
 constexpr int NCP3_NMAX = 8192;
 int ncp3_prepare(int n, int n2max);
 void ncp3_launch(DeviceStep &d, hipStream_t s);

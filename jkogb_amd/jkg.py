@@ -116,6 +116,7 @@ _zsolve = _sig("jkg_zsolve", _i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, C.POINTE
 _srt = _sig("jkg_srt", _i, [_vp, _i, _i, _vp, C.POINTER(C.c_int64)])
 _ncp = _sig("jkg_ncp", _i, [_vp, _i, _i, _i, _vp, _i, C.POINTER(C.c_int32), _vp, C.POINTER(C.c_int64)])
 _ncp3 = _sig("jkg_ncp3", _i, [_vp, _i, _i, _i, _vp, _i, C.POINTER(C.c_int32), _vp, C.POINTER(C.c_int64)])
+_dgen = _sig("jkg_dgen", _i, [_vp, _i, _vp, _i, _d, C.c_uint64, _vp, _vp])
 _tru = _sig("jkg_tru", _i, [_i, _i, _vp, _i, _vp, _vp, _vp])
 _jstep_len = _sig("jkg_jstep_len", _i, [_i, _i])
 _dhsvd2 = _sig("jkg_dhsvd2", _i, [_vp, _i, _vp, _vp, _d, _i, _vp, _vp, _vp, _vp])
@@ -150,7 +151,7 @@ EXPORTS = ["jkg_create", "jkg_destroy", "jkg_strerror", "jkg_last_hip_error", "j
            "jkg_dweights", "jkg_zweights", "jkg_dbuild", "jkg_dstep", "jkg_dprofile", "jkg_pair_index",
            "jkg_key_weight", "jkg_mixed_opts_default", "jkg_dsolve_mixed", "jkg_dweights_mixed",
            "jkg_dprofile_mixed", "jkg_ctl_grad", "jkg_ctl_fit", "jkg_batch_opts_default", "jkg_dsolve_batched",
-           "jkg_zsolve_mixed", "jkg_zweights_mixed", "jkg_zprofile", "jkg_zprofile_mixed", "jkg_ncp3"]
+           "jkg_zsolve_mixed", "jkg_zweights_mixed", "jkg_zprofile", "jkg_zprofile_mixed", "jkg_ncp3", "jkg_dgen"]
 
 
 def mixed_opts(**kw):
@@ -417,6 +418,16 @@ class Context:
                _p(ms), C.byref(r)), "jkg_zprofile" if cplx else "jkg_dprofile")
         return dict(stage_ms=dict(zip(STAGES, ms.tolist())), steps=r.steps, n2=r.n2, seconds=r.seconds,
                     **_counts(r))
+
+    # ---------------------------------------------------------------- test matrices
+    def dgen(self, lam, rounds=0, tmax=0.5, seed=1):
+        """(G, J) with hyperbolic singular values sqrt|lam| (jkg_dgen): G = Q diag(sqrt|lam|) H."""
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        n = lam.size
+        G = np.zeros((n, n), order="F")
+        J = np.zeros(n, np.int64)
+        _chk(_dgen(self._h, n, _p(lam), rounds, tmax, seed, _p(G), _p(J)), "jkg_dgen")
+        return G, J
 
     # ---------------------------------------------------------------- plug-ins
     def srt(self, dz, nn=None):
