@@ -16,7 +16,10 @@ Printed as one JSON line:
   value        algorithmic HBM GB/s of the mixed solver's step (bytes below)
   ms_per_step  the mixed solver's device time per step
   plain        the same for the plain solver; step_speedup = plain / mixed
-  roofline     the dominant stage of the mixed step, against HBM peak
+  roofline     the dominant stage of the mixed step, against HBM peak; traffic:
+               its HBM bytes per step measured by rocprofv3 FETCH_SIZE and
+               WRITE_SIZE, scaled by a plain copy's readings (the summary in
+               profiles/hbm_traffic_n4096_step_stages.md)
   rooflines    the Gram product (FP64 MFMA) and the rotations (HBM)
   full_solve   complete HSVDs at N = 1024 (C2 input): plain, mixed, and mixed
                with the trained step controller (jkogb_amd/ctl_params.npy):
@@ -31,8 +34,9 @@ Printed as one JSON line:
   batched      4096 independent HSVDs, N uniform in [16, 256], Gaussian Y,
                random J, N_2 = N/2, advanced together by one persistent launch
                per step (jkg_dsolve_batched, tuned grid and variant, tol = eps):
-               device time, against single plain solves of a sample of the
-               instances (projected to all 4096) and their SY difference
+               device time, against single mixed solves of a sample of the
+               instances (projected to all 4096), and the sample's singular
+               values against sqrt|eig(Y J Y^T)|
   cpu_baseline the reference (oracle/_ref/plain/djk.exe, built from the
                reference's own sources) on the same input, host cores
 
@@ -122,7 +126,7 @@ def timed_steps(ctx, solve, y, j, n2, steps, warmup, **kw):
     return dict(steps=k, ms_per_step=ms, gbs=b / (ms * 1e-3) / 1e9, bytes=b)
 
 
-def complex_run(ctx, fused, log, steps=20):
+def complex_run(ctx, fused, log, steps=20, full=True):
     """The complex configuration: per-step times and complete solves of the FP64 mixed solver
     and of its FP32 warm start."""
     from jkogb_amd import inputs
@@ -137,7 +141,10 @@ def complex_run(ctx, fused, log, steps=20):
 
     plain_ms = per_step(ctx.dsolve)
     mixed_ms = per_step(ctx.dsolve_mixed)
-    warm_ms = per_step(ctx.dsolve_mixed, fp32_warm=1, key_bits=32, switch_tol=1e-300)  # the FP32 phase alone
+    # the FP32 phase alone: its own device time (the step-limited run ends with the hand-over)
+    ctx.dsolve_mixed(y, j, n // 2, max_steps=2, fused=fused, fp32_warm=1, key_bits=32, switch_tol=1e-300)
+    r32 = ctx.dsolve_mixed(y, j, n // 2, max_steps=steps, fused=fused, fp32_warm=1, key_bits=32, switch_tol=1e-300)
+    warm_ms = 1e3 * r32["warm"]["seconds32"] / max(r32["warm"]["steps32"], 1)
     last = [0]
 
     def prog(step, done):
@@ -145,28 +152,36 @@ def complex_run(ctx, fused, log, steps=20):
             last[0] = step
             log("complex solve: step %d%s" % (step, " (done)" if done else ""))
 
-    m = ctx.dsolve_mixed(y, j, n // 2, fused=fused, trace_cap=1, progress=prog)
-    last[0] = 0
-    w = ctx.dsolve_mixed(y, j, n // 2, fused=fused, trace_cap=1, progress=prog, fp32_warm=1, key_bits=32)
+    if not full:
+        m = w = None
+    else:
+        m = ctx.dsolve_mixed(y, j, n // 2, fused=fused, trace_cap=1, progress=prog)
+        last[0] = 0
+        w = ctx.dsolve_mixed(y, j, n // 2, fused=fused, trace_cap=1, progress=prog, fp32_warm=1, key_bits=32)
+    out = {"workload": "complex N=2048, Re/Im ~ N(0,1/2), J 1024/1024 permuted, N_2=1024",
+           "ms_per_step": {"plain": round(plain_ms, 4), "mixed_fp64": round(mixed_ms, 4),
+                           "warm_fp32_phase": round(warm_ms, 4)},
+           "step_speedup_mixed": round(plain_ms / mixed_ms, 3),
+           "note": "ms_per_step: the first %d steps of each solver (warm_fp32_phase: the FP32 phase's own device "
+                   "time); a complete plain solve at this size (~22k steps) is not part of the default run" % steps}
+    if m is None:
+        return out
     wi = w["warm"]
-    return {"workload": "complex N=2048, Re/Im ~ N(0,1/2), J 1024/1024 permuted, N_2=1024",
-            "ms_per_step": {"plain": round(plain_ms, 4), "mixed_fp64": round(mixed_ms, 4),
-                            "warm_fp32_phase": round(warm_ms, 4)},
-            "step_speedup_mixed": round(plain_ms / mixed_ms, 3),
-            "mixed": {"steps": m["steps"], "seconds": round(m["seconds"], 3)},
+    out.update({"mixed": {"steps": m["steps"], "seconds": round(m["seconds"], 3)},
             "warm": {"steps": w["steps"], "seconds": round(w["seconds"], 3), "fp32_steps": wi["steps32"],
                      "fp32_seconds": round(wi["seconds32"], 3), "switch_seconds": round(wi["seconds_switch"], 4),
                      "off_rel_at_switch": wi["off_rel"], "newton_schulz": [wi["ns_u"], wi["ns_z"]],
                      "fallback": wi["fallback"],
                      "sy_err_vs_mixed_N_eps": round(float(np.max(np.abs(w["SY"] - m["SY"]) / np.abs(m["SY"])))
                                                     / (n * eps), 4)},
-            "note": "ms_per_step: the first %d steps of each solver (warm_fp32_phase: FP32 steps only); "
-                    "a complete plain solve at this size (~22k steps) is not part of the default run" % steps}
+                "speedup_warm_vs_mixed": round(m["seconds"] / w["seconds"], 3)})
+    return out
 
 
 def batched_run(ctx, count, fused, sample=8):
     """The batched configuration: count instances, N uniform in [16, 256], solved together with the
-    convergence test (tol = eps), against single plain solves of a sample of them."""
+    convergence test (tol = eps); a sample of them solved one at a time by the mixed solver (the same
+    convergence test), and the batch's singular values of the sample against sqrt|eig(Y J Y^T)|."""
     from jkogb_amd import inputs
     rng = np.random.default_rng(4)
     ns = rng.integers(16, 257, count)
@@ -180,20 +195,57 @@ def batched_run(ctx, count, fused, sample=8):
     infos = np.array([o["info"] for o in outs])
     t1, worst = 0.0, 0.0
     for i in rng.choice(count, sample, replace=False):
-        n = int(ns[i])
-        r = ctx.dsolve(ys[i], js[i], n // 2, fused=fused)
+        n, y, j = int(ns[i]), ys[i], js[i]
+        r = ctx.dsolve_mixed(y, j, n // 2, fused=fused)
         t1 += r["seconds"]
-        worst = max(worst, float(np.max(np.abs(outs[i]["SY"] - r["SY"]) / np.abs(r["SY"]))) / (n * eps))
+        # exact values: sqrt|eig(Y J Y^T)|, the eigenvalue signs giving the classes; compared sorted per class
+        ev = np.linalg.eigvalsh(y @ np.diag(j.astype(float)) @ y.T)
+        for cls in (1, -1):
+            a = np.sort(np.abs(outs[i]["SY"][j == cls]))
+            e = np.sort(np.sqrt(np.abs(ev[ev > 0] if cls > 0 else ev[ev < 0])))
+            worst = max(worst, float(np.max(np.abs(a - e) / e)) / (n * eps))
     single = t1 * count / sample
     return {"workload": "%d independent HSVDs, N uniform in [16,256] (sum N^2 = %d), Gaussian Y, random J, "
                         "N_2=N/2, one persistent launch per step, tol=eps" % (count, int(np.sum(ns.astype(np.int64) ** 2))),
             "seconds": round(run["seconds"], 4), "steps": run["steps"], "grid": run["grid"],
             "variant": run["variant"], "tune_seconds": round(run["tune_seconds"], 3),
             "instance_steps": {"min": int(infos.min()), "median": int(np.median(infos)), "max": int(infos.max())},
-            "single_plain_projected_seconds": round(single, 2), "speedup_vs_single_plain": round(single / run["seconds"], 2),
-            "sy_err_N_eps_sampled": round(worst, 4),
-            "note": "single_plain_projected: %d instances solved one at a time by jkg_dsolve (the reference's "
-                    "arithmetic), scaled to %d; sy_err: their largest relative SY difference" % (sample, count)}
+            "single_projected_seconds": round(single, 2), "speedup_vs_single": round(single / run["seconds"], 2),
+            "sy_err_vs_exact_N_eps_sampled": round(worst, 4),
+            "note": "single_projected: %d sampled instances solved one at a time by jkg_dsolve_mixed (the same "
+                    "convergence test), scaled to %d; sy_err: the batch's largest relative difference from "
+                    "sqrt|eig(Y J Y^T)| (float64, per J class) on the sample" % (sample, count)}
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_n4096_step_stages.md")
+
+
+def measured_traffic():
+    """HBM bytes per step of each stage, from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary in
+    profiles/ (its json block), or None if the summary is absent."""
+    try:
+        text = open(TRAFFIC_PROFILE).read()
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        return json.loads(block)["bytes_per_step"]
+    except (OSError, IndexError, KeyError, ValueError):
+        return None
+
+
+def traffic_workload(y, j, n2):
+    """What the rocprofv3 FETCH_SIZE / WRITE_SIZE runs measure: a plain device copy of 512 MiB
+    (the counters' scale: bytes moved are known) and two stage-profiled steps of each solver."""
+    import torch
+    from jkogb_amd import jkg
+    src = torch.empty(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
+    src.normal_()
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    with jkg.Context(0) as ctx:
+        ctx.dprofile_mixed(y, j, n2, steps=2)
+        ctx.dprofile(y, j, n2, steps=2)
+    print("traffic workload done", file=sys.stderr)
 
 
 def stage_profile(prof):
@@ -214,7 +266,11 @@ def main():
     ap.add_argument("--no-full", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-complex", action="store_true")
+    ap.add_argument("--no-complex-full", action="store_true", help="complex configuration: per-step times only")
     ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--traffic-workload", action="store_true",
+                    help="only the workload of the HBM-traffic counter runs: a 512 MiB device copy (three times, "
+                         "the counters' calibration) and two profiled steps of each solver at N=4096")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the step is single-GPU; see DESIGN.md for how it "
@@ -225,6 +281,9 @@ def main():
     n = args.n
     y, j = inputs.make("c5", n=n, seed=0)
     n2 = n // 2
+    if args.traffic_workload:
+        traffic_workload(y, j, n2)
+        return
     ctx = jkg.Context(0)
     t0 = time.time()
     log = lambda msg: print("bench: %s (%.0f s)" % (msg, time.time() - t0), file=sys.stderr, flush=True)
@@ -273,8 +332,9 @@ def main():
         log("full solves at N=1024")
     cplx = None
     if not args.no_complex:
-        cplx = complex_run(ctx, args.fused, log)
-        log("complex: mixed %.1f s, warm start %.1f s" % (cplx["mixed"]["seconds"], cplx["warm"]["seconds"]))
+        cplx = complex_run(ctx, args.fused, log, full=not args.no_complex_full)
+        if "mixed" in cplx:
+            log("complex: mixed %.1f s, warm start %.1f s" % (cplx["mixed"]["seconds"], cplx["warm"]["seconds"]))
     batched = None
     if not args.no_batched:
         batched = batched_run(ctx, args.batch_count, args.fused)
@@ -297,6 +357,7 @@ def main():
             cpu = {"value": None, "unit": "GB/s", "cores": threads, "kind": "reference",
                    "sample": "no step of the reference completed within %.0f s at N=4096" % args.cpu_budget}
 
+    traffic = measured_traffic()
     out = {
         "metric": "HSVD wall time + HBM GB/s at N=4096, mixed/learned vs plain FP64, same accuracy",
         "value": round(mixed["gbs"], 3),
@@ -319,7 +380,7 @@ def main():
         "step_speedup": round(plain["ms_per_step"] / mixed["ms_per_step"], 3),
         "roofline": {"bound": "hbm", "kernel": "%s stage of the mixed step" % dom,
                      "achieved": round(dom_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(dom_gbs / HBM_PEAK_GBS, 5), "traffic": None,
+                     "frac": round(dom_gbs / HBM_PEAK_GBS, 5), "traffic": (traffic or {}).get(dom),
                      "bytes_per_launch": int(stage_bytes.get(dom, 0.0)), "ms_per_launch": round(pm[dom], 4)},
         "rooflines": [
             {"bound": "mfma", "kernel": "weights stage (k_colnorm_off + k_gemm_tn + k_hyp_weights_gram)",
@@ -327,7 +388,7 @@ def main():
              "unit": "TFLOP/s", "frac": round(gram_flops / (pm["weights"] * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4),
              "flops_per_launch": gram_flops, "ms_per_launch": round(pm["weights"], 4)},
             {"bound": "hbm", "kernel": "k_rotate", "achieved": round(rot_gbs, 1), "peak": HBM_PEAK_GBS,
-             "unit": "GB/s", "frac": round(rot_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+             "unit": "GB/s", "frac": round(rot_gbs / HBM_PEAK_GBS, 4), "traffic": (traffic or {}).get("rotate"),
              "bytes_per_launch": int(stage_bytes["rotate"]), "ms_per_launch": round(pm["rotate"], 4)},
         ],
         "stage_ms_per_step": {"mixed": {s: round(v, 4) for s, v in pm.items()},

@@ -111,6 +111,13 @@ int jkg_zsolve(jkg_ctx *ctx, int n, int n2, const double *y, const int64_t *j, d
 int jkg_srt(jkg_ctx *ctx, int nn, int nm, jkg_aw *dz, int64_t *info);
 int jkg_ncp(jkg_ctx *ctx, int n, int nn, int nm, jkg_aw *dz, int n2, int32_t *sl, int32_t *step, int64_t *info);
 int jkg_ncp3(jkg_ctx *ctx, int n, int nn, int nm, jkg_aw *dz, int n2, int32_t *sl, int32_t *step, int64_t *info);
+/* jkg_set_select: the greedy walk's schedule for later solves on ctx (no
+ * counterpart in the reference; a tuning knob): `rounds` walks per step (1..16;
+ * 0: by the list length, about one per 512K pairs, 2 to 12), each but the last
+ * stopping after `limit` (>= 2048) entries of its list and followed by a
+ * whole-GPU pass that keeps only the entries with both indices free.  The
+ * pivots do not depend on it.  Default: rounds by size, 49152 entries. */
+int jkg_set_select(jkg_ctx *ctx, int rounds, int limit);
 int jkg_tru(int id_tru, int n, const int32_t *j, int nn, int32_t *p, int32_t *q, int32_t pn[2]);
 int jkg_jstep_len(int n, int n2);
 

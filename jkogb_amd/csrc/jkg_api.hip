@@ -44,12 +44,13 @@ struct jkg_ctx {
   hipGraphExec_t exec[2] = {nullptr, nullptr};
   // host values the captured graph holds as launch shapes or kernel arguments
   struct GraphKey {
-    int fused = -1, wmode = -1, npl = -1, ctl_mode = -1, ctl_cap = -1, cut32 = -1, ncp3 = -1;
+    int fused = -1, wmode = -1, npl = -1, ctl_mode = -1, ctl_cap = -1, cut32 = -1, ncp3 = -1, sel_rounds = -1,
+        sel_limit = -1;
     double neg_tol = -1.0, switch_tol = -1.0;
     bool operator==(const GraphKey &o) const {
       return fused == o.fused && wmode == o.wmode && npl == o.npl && neg_tol == o.neg_tol &&
              ctl_mode == o.ctl_mode && ctl_cap == o.ctl_cap && cut32 == o.cut32 && switch_tol == o.switch_tol &&
-             ncp3 == o.ncp3;
+             ncp3 == o.ncp3 && sel_rounds == o.sel_rounds && sel_limit == o.sel_limit;
     }
   } gkey[2];
   StepState st_init{};  // the loop state of the current problem before its first step
@@ -61,6 +62,8 @@ struct jkg_ctx {
   void *wm_M = nullptr, *wm_T = nullptr, *wm_P = nullptr;
   double *wm_part = nullptr;
   int ncp3_n = 0;  // ID_NCP=3: per-offset totals allocated for this N
+  // selection schedule: walk rounds per step (0: by size) and entries walked before each filter
+  int sel_rounds = 0, sel_limit = 49152;
   // step controller and its training (allocated on first use, kept across problems)
   float *ctl_master = nullptr, *ctl_table = nullptr, *ctl_trace = nullptr;
   uint16_t *ctl_q = nullptr;
@@ -118,6 +121,14 @@ static int dalloc(jkg_ctx *c, T **p, size_t count) {
     }                           \
   } while (0)
 
+// Walk rounds per step by list length: about one round (49152 entries walked, then a
+// whole-GPU filter) per 512K candidate pairs, 2 to 12 (measured on MI355X: N=1024 best
+// with 2, N=4096 with 12, 4.97 against 7.45 ms per mixed step).
+static int select_rounds_for(size_t nn) {
+  const size_t r = nn / 524288 + 1;
+  return (int)(r < 2 ? 2 : (r > 12 ? 12 : r));
+}
+
 // (Re)allocates the buffers of an N x N problem (real or complex).
 static int ensure_buffers(jkg_ctx *c, int n, int cplx, int trace_cap) {
   if (trace_cap < 1) trace_cap = 1;
@@ -167,7 +178,8 @@ static int ensure_buffers(jkg_ctx *c, int n, int cplx, int trace_cap) {
   ACHK(dalloc(c, &d.st, 1));
   ACHK(dalloc(c, &d.trace, trace_cap));
   ACHK(dalloc(c, &c->sigma, n));
-  d.select_limit = 65536;  // walked before the first filter of the list
+  d.select_limit = c->sel_limit;  // walked before each filter of the list
+  d.sel_rounds = c->sel_rounds > 0 ? c->sel_rounds : select_rounds_for(nn);
   if (step_prepare(d) != 0) {
     free_all(c);
     return JKG_ENOTSUP;
@@ -460,6 +472,8 @@ static int capture_step(jkg_ctx *c, int fused) {
   key.ctl_cap = c->d.ctl_trace_cap;
   key.cut32 = slot ? c->d.cut32 : 0;
   key.ncp3 = c->d.ncp3;
+  key.sel_rounds = c->d.sel_rounds;
+  key.sel_limit = c->d.select_limit;
   key.switch_tol = slot ? c->d.switch_tol : 0.0;
   if (c->exec[slot] && c->gkey[slot] == key) return JKG_OK;
   if (c->exec[slot]) (void)hipGraphExecDestroy(c->exec[slot]);
@@ -1207,6 +1221,21 @@ extern "C" int jkg_ncp(jkg_ctx *c, int n, int nn, int nm, jkg_aw *dz, int n2, in
   HIPCHK(hipStreamSynchronize(c->stream));
   *sl = s.sl;
   if (s.sl > 0) HIPCHK(hipMemcpy(step, d.piv_pos, sizeof(int) * s.sl, hipMemcpyDeviceToHost));
+  return JKG_OK;
+}
+
+// The selection schedule of later solves on this context: `rounds` walks per step (0: by size),
+// each but the last stopping after `limit` entries of its list, followed by a
+// whole-GPU pass that drops the entries with a used index.  The pivots chosen do
+// not depend on it (the filter removes only entries the walk would skip).
+extern "C" int jkg_set_select(jkg_ctx *c, int rounds, int limit) {
+  if (!c) return -1;
+  if (rounds < 0 || rounds > 16) return -2;
+  if (limit < 2048) return -3;
+  c->sel_rounds = rounds;
+  c->sel_limit = limit;
+  c->d.sel_rounds = rounds > 0 ? rounds : select_rounds_for((size_t)c->d.nn);
+  c->d.select_limit = limit;
   return JKG_OK;
 }
 

@@ -9,8 +9,10 @@
 //              counts, plus the digit total; a digit holding every key marks the
 //              pass as the identity (skipped: stability makes it a no-op)
 //   scatter -- stable rank of each key inside its tile (wave ballots over the 8
-//              digit bits, then a cross-wave prefix in LDS), write to
-//              base[digit] + tile offset + rank
+//              digit bits, then a cross-wave prefix in LDS), the tile staged in
+//              LDS in digit order, then written out in that order so that each
+//              digit's run leaves as consecutive addresses (base[digit] + tile
+//              offset + rank)
 // Buffers ping-pong; which one holds the current data is derived on the device
 // from the per-pass skip flags, so the launch sequence is fixed (graph-safe).
 // The first executed pass takes the values implicitly as the key's index.
@@ -98,9 +100,13 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter(uint64_t *__restr
                                                                const int *n_ptr, const int *__restrict__ ghist,
                                                                const int *__restrict__ dtotal, const int *skip,
                                                                int pass, int ntiles) {
-  __shared__ int base[RADIX];       // global output offset per digit for this tile
+  __shared__ int gbase[RADIX];      // global output position of this tile's first key of each digit
+  __shared__ int lstart[RADIX];     // first slot of each digit in the tile staged in digit order
+  __shared__ int run[RADIX];        // running slot of each digit while staging
   __shared__ int wcnt[4][RADIX];    // per-wave digit counts of the current round
   __shared__ int scan_tmp[RADIX];
+  __shared__ uint64_t skey[SORT_TILE];
+  __shared__ uint32_t sval[SORT_TILE];
   const int n = *n_ptr;
   const int tiles = (n + SORT_TILE - 1) / SORT_TILE;
   const int t = blockIdx.x;
@@ -112,10 +118,14 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter(uint64_t *__restr
   uint32_t *vout = pi.cur ? v0 : v1;
   const int shift = pass * 8;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tlen = (n - t * SORT_TILE < SORT_TILE) ? n - t * SORT_TILE : SORT_TILE;
 
-  // digit bases: exclusive scan of the digit totals + this tile's offset
+  // digit starts: exclusive scan of the digit totals; this tile's count of each digit from the
+  // per-tile offsets (ghist holds, per digit, the exclusive scan over the tiles)
   {
-    int d = threadIdx.x;
+    const int d = threadIdx.x;
+    const int tile_off = ghist[d * ntiles + t];
+    const int tile_cnt = ((t + 1 < tiles) ? ghist[d * ntiles + t + 1] : dtotal[d]) - tile_off;
     scan_tmp[d] = dtotal[d];
     __syncthreads();
     for (int off = 1; off < RADIX; off <<= 1) {
@@ -124,14 +134,27 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter(uint64_t *__restr
       scan_tmp[d] += v;
       __syncthreads();
     }
-    base[d] = scan_tmp[d] - dtotal[d] + ghist[d * ntiles + t];
+    gbase[d] = scan_tmp[d] - dtotal[d] + tile_off;
+    __syncthreads();
+    scan_tmp[d] = tile_cnt;
+    __syncthreads();
+    for (int off = 1; off < RADIX; off <<= 1) {
+      int v = (d >= off) ? scan_tmp[d - off] : 0;
+      __syncthreads();
+      scan_tmp[d] += v;
+      __syncthreads();
+    }
+    lstart[d] = scan_tmp[d] - tile_cnt;
+    run[d] = lstart[d];
   }
+  // stage the tile in LDS in stable digit order (16 rounds of 256 keys)
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   for (int r = 0; r < SORT_ITEMS; ++r) {
     for (int i = threadIdx.x; i < 4 * RADIX; i += SORT_THREADS) (&wcnt[0][0])[i] = 0;
     __syncthreads();
-    const int idx = t * SORT_TILE + r * SORT_THREADS + threadIdx.x;
-    const bool valid = idx < n;
+    const int li = r * SORT_THREADS + threadIdx.x;
+    const int idx = t * SORT_TILE + li;
+    const bool valid = li < tlen;
     uint64_t key = valid ? kin[idx] : 0;
     uint32_t val = valid ? (pi.vals_iota ? (uint32_t)idx : vin[idx]) : 0;
     int d = (int)((key >> shift) & 0xFF);
@@ -148,17 +171,25 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter(uint64_t *__restr
     if (valid) {
       int before = 0;
       for (int w = 0; w < wave; ++w) before += wcnt[w][d];
-      int pos = base[d] + before + wrank;
-      kout[pos] = key;
-      vout[pos] = val;
+      const int slot = run[d] + before + wrank;
+      skey[slot] = key;
+      sval[slot] = val;
     }
     __syncthreads();
-    // advance the per-digit running offsets by this round's counts
     {
-      int dd = threadIdx.x;
-      base[dd] += wcnt[0][dd] + wcnt[1][dd] + wcnt[2][dd] + wcnt[3][dd];
+      const int dd = threadIdx.x;
+      run[dd] += wcnt[0][dd] + wcnt[1][dd] + wcnt[2][dd] + wcnt[3][dd];
     }
     __syncthreads();
+  }
+  // write out in staged order: neighbouring lanes hold neighbouring slots, so each digit's
+  // run of this tile goes out as consecutive addresses
+  for (int i = threadIdx.x; i < tlen; i += SORT_THREADS) {
+    const uint64_t key = skey[i];
+    const int d = (int)((key >> shift) & 0xFF);
+    const int pos = gbase[d] + (i - lstart[d]);
+    kout[pos] = key;
+    vout[pos] = sval[i];
   }
 }
 

@@ -838,9 +838,8 @@ void weights_launch(DeviceStep &d, hipStream_t s) {
   }
 }
 
-constexpr int SEL_ROUNDS = 2;
-
 void select_launch(DeviceStep &d, hipStream_t s) {
+  const int SEL_ROUNDS = d.sel_rounds > 0 ? d.sel_rounds : 2;
   if (d.ncp3) {
     ncp3_launch(d, s);
     return;

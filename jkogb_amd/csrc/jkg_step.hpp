@@ -96,6 +96,7 @@ struct DeviceStep {
   StepState *st;
   StepRec *trace;
   int select_limit;       // entries walked per selection round before filtering
+  int sel_rounds;         // walk rounds per step (each but the last followed by a filter pass)
   // mixed solver
   int wmode;              // hyperbolic weights: 0 per pair (ABODNDF2), 1 Gram form
   double *gram;           // ks planes of npl x nmi: (O^T O)(plus, minus), O = A with a zero diagonal

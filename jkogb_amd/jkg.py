@@ -117,6 +117,7 @@ _srt = _sig("jkg_srt", _i, [_vp, _i, _i, _vp, C.POINTER(C.c_int64)])
 _ncp = _sig("jkg_ncp", _i, [_vp, _i, _i, _i, _vp, _i, C.POINTER(C.c_int32), _vp, C.POINTER(C.c_int64)])
 _ncp3 = _sig("jkg_ncp3", _i, [_vp, _i, _i, _i, _vp, _i, C.POINTER(C.c_int32), _vp, C.POINTER(C.c_int64)])
 _dgen = _sig("jkg_dgen", _i, [_vp, _i, _vp, _i, _d, C.c_uint64, _vp, _vp])
+_set_select = _sig("jkg_set_select", _i, [_vp, _i, _i])
 _tru = _sig("jkg_tru", _i, [_i, _i, _vp, _i, _vp, _vp, _vp])
 _jstep_len = _sig("jkg_jstep_len", _i, [_i, _i])
 _dhsvd2 = _sig("jkg_dhsvd2", _i, [_vp, _i, _vp, _vp, _d, _i, _vp, _vp, _vp, _vp])
@@ -151,7 +152,7 @@ EXPORTS = ["jkg_create", "jkg_destroy", "jkg_strerror", "jkg_last_hip_error", "j
            "jkg_dweights", "jkg_zweights", "jkg_dbuild", "jkg_dstep", "jkg_dprofile", "jkg_pair_index",
            "jkg_key_weight", "jkg_mixed_opts_default", "jkg_dsolve_mixed", "jkg_dweights_mixed",
            "jkg_dprofile_mixed", "jkg_ctl_grad", "jkg_ctl_fit", "jkg_batch_opts_default", "jkg_dsolve_batched",
-           "jkg_zsolve_mixed", "jkg_zweights_mixed", "jkg_zprofile", "jkg_zprofile_mixed", "jkg_ncp3", "jkg_dgen"]
+           "jkg_zsolve_mixed", "jkg_zweights_mixed", "jkg_zprofile", "jkg_zprofile_mixed", "jkg_ncp3", "jkg_dgen", "jkg_set_select"]
 
 
 def mixed_opts(**kw):
@@ -418,6 +419,10 @@ class Context:
                _p(ms), C.byref(r)), "jkg_zprofile" if cplx else "jkg_dprofile")
         return dict(stage_ms=dict(zip(STAGES, ms.tolist())), steps=r.steps, n2=r.n2, seconds=r.seconds,
                     **_counts(r))
+
+    def set_select(self, rounds=0, limit=49152):
+        """The greedy walk's schedule (jkg_set_select); the pivots do not depend on it."""
+        _chk(_set_select(self._h, rounds, limit), "jkg_set_select")
 
     # ---------------------------------------------------------------- test matrices
     def dgen(self, lam, rounds=0, tmax=0.5, seed=1):

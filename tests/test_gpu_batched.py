@@ -109,3 +109,30 @@ def test_batched_convergence_test(ctx):
         rel = np.max(np.abs(o["SY"] - p["SY"]) / np.abs(p["SY"]))
         assert rel <= 4 * spread + 64 * n * eps, (n, rel, spread)
         assert 0 <= o["info"] <= p["info"]
+
+
+def test_convergence_test_on_a_stagnating_instance(ctx):
+    # an instance of the batched configuration (N=208) on which the reference's termination never
+    # triggers: its weights underflow to 0 after ~15000 steps while transforms continue, and the
+    # diagonal is then still far from the HSVD (smallest value 0.0566 against the exact 0.00632).
+    # With the convergence test both the batched and the mixed solver stop near 2000 steps at the
+    # exact values, sqrt|eig(Y J Y^T)| per class -- up to the float64 eigenvalues' own error,
+    # |d sigma| <= N eps |Y|^2 / sigma, so each value is checked to 64 N eps (sigma + |Y|^2 / sigma).
+    from jkogb_amd import inputs
+    eps = np.finfo(np.float64).eps
+    n = 208
+    y, j = inputs.make("c4", n=n, seed=12370)
+    ev = np.linalg.eigvalsh(y @ np.diag(j.astype(float)) @ y.T)
+    y2 = np.linalg.norm(y, 2) ** 2
+    b, _ = ctx.dsolve_batched([y], [j], tol=eps)
+    m = ctx.dsolve_mixed(y, j, n // 2)
+    for out in (b[0], m):
+        assert 0 < out["info"] < 4000
+        for cls in (1, -1):
+            a = np.sort(np.abs(out["SY"][j == cls]))
+            e = np.sort(np.sqrt(np.abs(ev[ev > 0] if cls > 0 else ev[ev < 0])))
+            assert np.all(np.abs(a - e) <= 64 * n * eps * (e + y2 / e))
+    # the two agree with each other to rounding
+    assert np.max(np.abs(b[0]["SY"] - m["SY"]) / np.abs(m["SY"])) <= 64 * n * eps
+    p = ctx.dsolve(y, j, n // 2, max_steps=5000)
+    assert p["info"] == 5000  # still running where the convergence test has long ended
