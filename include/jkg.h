@@ -180,7 +180,9 @@ typedef struct {
                          0: the reference's test only */
   float *ctl_trace;   /* per-step controller records (JKG_CTL_REC floats each), or NULL */
   int32_t ctl_trace_cap;
-  int32_t reserved0;
+  int32_t ordering;   /* 0: dynamic (the reference's, default); 1 (real): cyclic round-robin sweeps, N/2
+                         disjoint pairs per step, no weights, sort or walk, ended by a sweep without a
+                         transform -- the classic baseline (no controller or threshold) */
   struct jkg_warm_info *warm; /* out (fp32_warm): what the warm start did, or NULL */
   int32_t reserved[4];
 } jkg_mixed_opts;

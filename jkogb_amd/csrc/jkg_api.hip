@@ -45,12 +45,12 @@ struct jkg_ctx {
   // host values the captured graph holds as launch shapes or kernel arguments
   struct GraphKey {
     int fused = -1, wmode = -1, npl = -1, ctl_mode = -1, ctl_cap = -1, cut32 = -1, ncp3 = -1, sel_rounds = -1,
-        sel_limit = -1;
+        sel_limit = -1, cyclic = -1;
     double neg_tol = -1.0, switch_tol = -1.0;
     bool operator==(const GraphKey &o) const {
       return fused == o.fused && wmode == o.wmode && npl == o.npl && neg_tol == o.neg_tol &&
              ctl_mode == o.ctl_mode && ctl_cap == o.ctl_cap && cut32 == o.cut32 && switch_tol == o.switch_tol &&
-             ncp3 == o.ncp3 && sel_rounds == o.sel_rounds && sel_limit == o.sel_limit;
+             ncp3 == o.ncp3 && sel_rounds == o.sel_rounds && sel_limit == o.sel_limit && cyclic == o.cyclic;
     }
   } gkey[2];
   StepState st_init{};  // the loop state of the current problem before its first step
@@ -474,6 +474,7 @@ static int capture_step(jkg_ctx *c, int fused) {
   key.ncp3 = c->d.ncp3;
   key.sel_rounds = c->d.sel_rounds;
   key.sel_limit = c->d.select_limit;
+  key.cyclic = c->d.cyclic;
   key.switch_tol = slot ? c->d.switch_tol : 0.0;
   if (c->exec[slot] && c->gkey[slot] == key) return JKG_OK;
   if (c->exec[slot]) (void)hipGraphExecDestroy(c->exec[slot]);
@@ -608,6 +609,7 @@ struct MixedCfg {
   const float *ctl = nullptr;
   float *ctl_trace = nullptr;
   int ctl_trace_cap = 0;
+  int cyclic = 0;  // round-robin sweeps in place of the dynamic ordering
   // FP32 warm start
   int warm = 0, cut32 = 0;
   double switch_tol = 0.0;
@@ -730,6 +732,7 @@ static int solve_common(jkg_ctx *c, int cplx, int n, int n2, const void *y, cons
     rc = ensure_warm(c);
     if (rc) return rc;
   }
+  c->d.cyclic = mc.cyclic;
   c->d.ncp3 = 0;
   if (o.id_ncp == 3) {
     rc = ensure_ncp3(c);
@@ -808,9 +811,12 @@ extern "C" int jkg_dsolve_mixed(jkg_ctx *c, int n, int n2, const double *y, cons
   if (m.weights != JKG_W_PAIR && m.weights != JKG_W_GRAM) return -10;
   if (!(m.tol >= 0.0) || !(m.threshold >= 0.0 && m.threshold < 1.0)) return -10;
   if (m.controller < 0 || m.controller > 2 || (m.controller && !m.ctl) || m.ctl_trace_cap < 0) return -10;
+  if (m.ordering < 0 || m.ordering > 1) return -10;
   if (m.fp32_warm != 0 || m.key_bits != 64) return JKG_ENOTSUP;
   if (m.weights == JKG_W_PAIR && (m.tol != 0.0 || m.controller)) return JKG_ENOTSUP;
+  if (m.ordering == 1 && (m.controller || m.threshold != 0.0)) return JKG_ENOTSUP;
   MixedCfg mc;
+  mc.cyclic = m.ordering;
   mc.wmode = m.weights == JKG_W_GRAM ? 1 : 0;
   mc.tol = m.tol;
   mc.tau = m.controller ? 0.0 : m.threshold;
@@ -837,7 +843,7 @@ extern "C" int jkg_zsolve_mixed(jkg_ctx *c, int n, int n2, const double *y, cons
   if (m.controller < 0 || m.controller > 2 || m.ctl_trace_cap < 0) return -10;
   if (m.fp32_warm < 0 || m.fp32_warm > 1 || (m.key_bits != 32 && m.key_bits != 64)) return -10;
   if (m.fp32_warm && !(m.switch_tol > 0.0 && m.switch_tol < 1.0)) return -10;
-  if (m.controller) return JKG_ENOTSUP;
+  if (m.controller || m.ordering != 0) return JKG_ENOTSUP;  // real-only options
   if (m.key_bits == 32 && !m.fp32_warm) return JKG_ENOTSUP;  // 32-bit keys: the warm start only
   if (m.weights == JKG_W_PAIR && (m.tol != 0.0 || m.fp32_warm)) return JKG_ENOTSUP;
   MixedCfg mc;
@@ -953,6 +959,7 @@ static int prepare_single(jkg_ctx *c, int n, int n2, const void *a, const int64_
   c->d.neg_tol = 0.0;
   c->d.ctl_mode = 0;
   c->d.ncp3 = 0;
+  c->d.cyclic = 0;
   int npos, nneg;
   return upload_problem(c, n, jkg_jstep_len(n, n2), a, j, &npos, &nneg, -1, c->trace_cap);
 }
@@ -1388,6 +1395,7 @@ static int profile_common(jkg_ctx *c, int n, int n2, const double *y, const int6
   c->d.neg_tol = tol;
   c->d.ctl_mode = 0;
   c->d.ncp3 = 0;
+  c->d.cyclic = 0;
   if (wmode) {
     rc = ensure_mixed(c);
     if (rc) return rc;

@@ -481,22 +481,37 @@ __global__ void __launch_bounds__(256) k_pivots(int n, const double *__restrict_
                                                  const uint32_t *v1, const int *sort_skip, double *piv_w,
                                                  int *piv_info, double *piv_V,
                                                  double *piv_W, double *piv_d, Rot *rot_row, Rot *rot_col,
-                                                 int *colpiv, StepState *st) {
+                                                 int *colpiv, StepState *st, double tol) {
   if (!st->active) return;
   const int sl = st->sl;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= sl) return;
   const int p = piv_p[i], q = piv_q[i];
   const size_t P = p, Q = q;
-  {
-    // the pivot's weight (for NORMF2D) from its position in the sorted list
-    SortResult sr = sort_result_dev(k0, k1, v0, v1, sort_skip);
-    piv_w[i] = key_weight(sr.keys[piv_pos[i] - 1]);
-  }
   double b[4] = {A[P * lda + P], A[P * lda + Q], A[Q * lda + P], A[Q * lda + Q]};
   int k[2] = {J[p], J[q]};
+  const int pos = piv_pos[i];
+  if (pos > 0) {
+    // the pivot's weight (for NORMF2D) from its position in the sorted list
+    SortResult sr = sort_result_dev(k0, k1, v0, v1, sort_skip);
+    piv_w[i] = key_weight(sr.keys[pos - 1]);
+  } else {
+    // cyclic ordering (no list): the pair's off-diagonal mass; a negligible pair whose
+    // transform would otherwise be the identity is left alone (INFO 8: no transform, no
+    // new diagonal), as the dynamic ordering drops it
+    piv_w[i] = dasum2<F>(b[1], b[2]);
+    const bool order = sign_neg(b[0]) || sign_neg(b[3]) ||
+                       (k[0] == k[1] && ((k[0] + k[1] == -2) ? (b[3] < b[0]) : (b[0] < b[3])));
+    if (tol > 0.0 && !order && negligible(b[1], b[2], b[0], b[3], tol)) {
+      piv_info[i] = 8;
+      colpiv[p] = i;
+      colpiv[q] = -(i + 2);
+      return;
+    }
+  }
   double v[4], w[4];
   int info = dhsvd2<F>(b, k, v, w, TH1FIX);
+  if (info < 0 && pos <= 0) info = 8;  // cyclic: a pair without a transform is skipped
   piv_info[i] = info;
   colpiv[p] = i;
   colpiv[q] = -(i + 2);
@@ -504,6 +519,7 @@ __global__ void __launch_bounds__(256) k_pivots(int n, const double *__restrict_
     atomicMin(&st->err, info);
     return;
   }
+  if (info == 8) return;
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
     piv_V[4 * i + t] = v[t];
@@ -669,7 +685,7 @@ __global__ void __launch_bounds__(256) k_step_end(const int *__restrict__ J, con
                                                    const int *__restrict__ piv_q, const int *__restrict__ piv_info,
                                                    const double *__restrict__ piv_w, StepState *st, StepRec *trace,
                                                    int n, int nn, const int *__restrict__ piv_pos, int ctl_on,
-                                                   float *ctl_trace, int ctl_cap) {
+                                                   float *ctl_trace, int ctl_cap, int sweep) {
   __shared__ int cnt[5];
   if (!st->active) return;
   const int sl = st->sl, s = st->step;
@@ -757,7 +773,16 @@ __global__ void __launch_bounds__(256) k_step_end(const int *__restrict__ J, con
     st->nl = nl;
     if (ctl_on) controller_features(st, s, n, nn, sl, nl, rec.trig, rec.hyp, tw, piv_pos, ctl_trace, ctl_cap);
     const int sl2 = (nl < sl) ? nl : sl;
-    if (sl2 <= 0) {
+    if (sweep > 0 && nl >= 0) {
+      // cyclic ordering: the run ends after a whole sweep without a transform
+      st->quiet = (nl > 0) ? 0 : st->quiet + 1;
+      if (st->quiet >= sweep) {
+        st->done = 1;
+        st->info = s;
+      } else {
+        st->step = s + 1;
+      }
+    } else if (sl2 <= 0) {
       st->done = 1;
       st->info = (sl2 < 0) ? sl2 : s;
     } else {
@@ -796,6 +821,31 @@ __global__ void k_finalize(double *A, double *Z, int n, int lda, const int *J, d
       *zrc = a;
       *zcr = b;
     }
+  }
+}
+
+// ------------------------------------------------------------------ cyclic ordering
+// The classic sweep baseline: step s takes round s mod (M - 1) of the round-robin
+// schedule over M = N (+1 if N is odd: a dummy index) positions -- pair k pairs
+// (r + k) and (r - k) mod (M - 1), pair 0 pairs M - 1 with r -- so every round is
+// N/2 disjoint pairs and M - 1 rounds are one sweep over all pairs.  No weights,
+// sort or walk; positions 0 mark the pivots as cyclic for k_pivots.
+__global__ void k_cyc_select(int n, int *piv_p, int *piv_q, int *piv_pos, StepState *st) {
+  if (!st->active) return;
+  const int m = n + (n & 1), r = st->step % (m - 1);
+  for (int k = threadIdx.x; k < m / 2; k += blockDim.x) {
+    const int x = (k == 0) ? m - 1 : (r + k) % (m - 1);
+    const int y = (r - k + (m - 1)) % (m - 1);
+    if ((n & 1) && k == 0) continue;  // the dummy index n sits in pair 0
+    const int slot = (n & 1) ? k - 1 : k;
+    piv_p[slot] = x < y ? x : y;
+    piv_q[slot] = x < y ? y : x;
+    piv_pos[slot] = 0;
+  }
+  if (threadIdx.x == 0) {
+    st->sl = n / 2;
+    st->it = 0;
+    st->sel_done = 1;
   }
 }
 
@@ -876,12 +926,12 @@ void pivots_launch(DeviceStep &d, hipStream_t s) {
     hipLaunchKernelGGL(k_pivots<true>, dim3(g), dim3(256), 0, s, d.n, (const double *)d.A, d.lda, d.J, d.piv_p,
                        d.piv_q, d.piv_pos, d.sort.k0, d.sort.k1, d.sort.v0, d.sort.v1, d.sort_skip, d.piv_w,
                        d.piv_info, (double *)d.piv_V, (double *)d.piv_W, d.piv_d, d.rot_row, d.rot_col,
-                       d.colpiv, d.st);
+                       d.colpiv, d.st, d.neg_tol);
   else
     hipLaunchKernelGGL(k_pivots<false>, dim3(g), dim3(256), 0, s, d.n, (const double *)d.A, d.lda, d.J, d.piv_p,
                        d.piv_q, d.piv_pos, d.sort.k0, d.sort.k1, d.sort.v0, d.sort.v1, d.sort_skip, d.piv_w,
                        d.piv_info, (double *)d.piv_V, (double *)d.piv_W, d.piv_d, d.rot_row, d.rot_col,
-                       d.colpiv, d.st);
+                       d.colpiv, d.st, d.neg_tol);
 }
 
 void rotate_launch(DeviceStep &d, hipStream_t s) {
@@ -899,8 +949,9 @@ void rotate_launch(DeviceStep &d, hipStream_t s) {
 }
 
 void step_end_launch(DeviceStep &d, hipStream_t s) {
+  const int sweep = d.cyclic ? d.n + (d.n & 1) - 1 : 0;
   hipLaunchKernelGGL(k_step_end, dim3(1), dim3(256), 0, s, d.J, d.piv_p, d.piv_q, d.piv_info, d.piv_w, d.st,
-                     d.trace, d.n, d.nn, d.piv_pos, d.ctl_mode != 0 ? 1 : 0, d.ctl_trace, d.ctl_trace_cap);
+                     d.trace, d.n, d.nn, d.piv_pos, d.ctl_mode != 0 ? 1 : 0, d.ctl_trace, d.ctl_trace_cap, sweep);
 }
 
 int step_prepare(DeviceStep &d) {
@@ -917,6 +968,13 @@ int step_prepare(DeviceStep &d) {
 
 void step_launch(DeviceStep &d, hipStream_t s) {
   step_begin_launch(d, s);
+  if (d.cyclic) {
+    hipLaunchKernelGGL(k_cyc_select, dim3(1), dim3(256), 0, s, d.n, d.piv_p, d.piv_q, d.piv_pos, d.st);
+    pivots_launch(d, s);
+    rotate_launch(d, s);
+    step_end_launch(d, s);
+    return;
+  }
   if (d.ctl_mode) controller_launch(d, s);
   weights_launch(d, s);
   sort_pairs_launch(d.sort, &d.st->nn_sort, d.nn, s, 1);

@@ -67,6 +67,7 @@ struct StepState {
   // FP32 warm start: relative off-norm of A (this step's), the best seen and its step, why the phase ended
   double off_rel, off_best;
   int off_best_step, warm_end;
+  int quiet;  // cyclic ordering: consecutive steps without a transform
   // work counters summed over the steps (bytes and flops of the run)
   unsigned long long acc_sl, acc_it, acc_b2, acc_b4, acc_abod;
 };
@@ -119,6 +120,8 @@ struct DeviceStep {
   // ID_NCP=3 (jkg_ncp3.hip): the best-of-all-offsets selection replaces the greedy walk
   int ncp3;
   double *ncp3_wsum;      // NN: total weight of the walk from each start offset
+  // cyclic ordering (the sweep baseline of the mixed solver): round-robin pivots, no weights
+  int cyclic;
 };
 
 // test-matrix generator (jkg_gen.hip)

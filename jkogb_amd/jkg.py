@@ -65,7 +65,7 @@ class MixedOpts(C.Structure):
     _fields_ = [("weights", C.c_int32), ("fp32_warm", C.c_int32), ("switch_tol", C.c_double),
                 ("key_bits", C.c_int32), ("controller", C.c_int32), ("ctl", C.c_void_p),
                 ("threshold", C.c_double), ("tol", C.c_double), ("ctl_trace", C.c_void_p),
-                ("ctl_trace_cap", C.c_int32), ("reserved0", C.c_int32), ("warm", C.c_void_p),
+                ("ctl_trace_cap", C.c_int32), ("ordering", C.c_int32), ("warm", C.c_void_p),
                 ("reserved", C.c_int32 * 4)]
 
 

@@ -89,7 +89,10 @@ int32_t PFX##zstep_loop(int n, jk_z *U, int ldu, jk_z *A, int lda, jk_z *Z,     
                         int ldz, const int32_t *J, double *sigma, int nn,                \
                         const int32_t *P, const int32_t *Q, int id_ncp, int nm,          \
                         jk_aw *dz, int n2, int32_t *step, int tt, jk_z *W,               \
-                        jk_steprec *trace, int max_trace, int max_steps);
+                        jk_steprec *trace, int max_trace, int max_steps);                \
+int32_t PFX##dcyc_loop(int n, double *U, int ldu, double *A, int lda, double *Z, int ldz, \
+                       const int32_t *J, double *sigma, double *W, jk_steprec *trace,    \
+                       int max_trace, int max_steps);
 
 JK_ORACLE_DECL(orc_)
 JK_ORACLE_DECL(orcf_)

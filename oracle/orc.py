@@ -163,6 +163,28 @@ def solve(Y, J, n2=0, id_ncp=0, id_tru=1, fused=False, max_steps=-1, max_trace=N
                 trace=trace[:min(nsteps if nsteps else max_trace, max_trace)], tt=tt, pn=PN, A=A)
 
 
+def solve_cyclic(Y, J, fused=False, max_steps=-1, max_trace=None):
+    """The mixed solver's cyclic sweep baseline on the CPU (dcyc_loop, no convergence
+    tolerance).  Returns dict(YU, ZZ, SY, info, trace)."""
+    A = np.array(Y, dtype=np.float64, order="F", copy=True)
+    n = A.shape[0]
+    U = np.zeros((n, n), order="F")
+    Z = np.zeros((n, n), order="F")
+    S = np.zeros(max(n, 3))
+    W = np.zeros(6 * max(n // 2, 1))
+    if max_trace is None:
+        max_trace = 64 * n + 64
+    trace = np.zeros(max_trace, STEPREC_DTYPE)
+    J32 = np.ascontiguousarray(J, dtype=np.int32)
+    f = getattr(lib(), pfx(fused) + "dcyc_loop")
+    f.restype = C.c_int32
+    f.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
+                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int]
+    info = f(n, _p(U), n, _p(A), n, _p(Z), n, _p(J32), _p(S), _p(W), _p(trace), max_trace, max_steps)
+    nsteps = info + 1 if info >= 0 else max_trace
+    return dict(YU=U, ZZ=Z, SY=S[:n].copy(), info=info, trace=trace[:min(nsteps, max_trace)])
+
+
 def pair_weights(A, J, fused=False):
     """Weight of every pair (DMAGF2T for J_p == J_q, DMAGF2H otherwise, with
     the hyperbolic -inf dropped to NaN as DSTEP_BUILD does), indexed by the
