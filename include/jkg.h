@@ -118,6 +118,16 @@ int jkg_ncp3(jkg_ctx *ctx, int n, int nn, int nm, jkg_aw *dz, int n2, int32_t *s
  * whole-GPU pass that keeps only the entries with both indices free.  The
  * pivots do not depend on it.  Default: rounds by size, 49152 entries. */
 int jkg_set_select(jkg_ctx *ctx, int rounds, int limit);
+/* jkg_set_frames: pictures of |A| during later solves on ctx (the role of the
+ * reference's ANIMATE frames, src/D/dstep.F90:485-495 and 549-592, written there
+ * by libpvn's PVN_RVIS_*; no file format is shared): at the start of every
+ * `every`-th step, before its transforms, a size x size map (row-major bytes)
+ * whose pixel holds the largest |a_ij| of the rows and columns mapped to it on a
+ * fixed log scale -- 0 for zero, else 1 + floor(8 (log10|a| + 16)) clamped to
+ * 1..255 (8 levels per decade from 1e-16).  After each solve the first
+ * min(frames recorded, max_frames) frames are in `frames` (max_frames * size^2
+ * bytes, kept valid by the caller) and their number in *count.  every <= 0: off. */
+int jkg_set_frames(jkg_ctx *ctx, int every, int size, int max_frames, uint8_t *frames, int32_t *count);
 int jkg_tru(int id_tru, int n, const int32_t *j, int nn, int32_t *p, int32_t *q, int32_t pn[2]);
 int jkg_jstep_len(int n, int n2);
 
@@ -184,7 +194,8 @@ typedef struct {
                          disjoint pairs per step, no weights, sort or walk, ended by a sweep without a
                          transform -- the classic baseline (no controller or threshold) */
   struct jkg_warm_info *warm; /* out (fp32_warm): what the warm start did, or NULL */
-  int32_t reserved[4];
+  double ctl_tau_max; /* largest threshold a controller may set (0: 1e-6, the network's range) */
+  int32_t reserved[2];
 } jkg_mixed_opts;
 
 /* The FP32 warm start of jkg_zsolve_mixed: steps on FP32 copies of A, U and Z

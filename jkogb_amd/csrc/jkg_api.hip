@@ -45,12 +45,13 @@ struct jkg_ctx {
   // host values the captured graph holds as launch shapes or kernel arguments
   struct GraphKey {
     int fused = -1, wmode = -1, npl = -1, ctl_mode = -1, ctl_cap = -1, cut32 = -1, ncp3 = -1, sel_rounds = -1,
-        sel_limit = -1, cyclic = -1;
+        sel_limit = -1, cyclic = -1, frame_every = -1, frame_size = -1, frame_max = -1;
     double neg_tol = -1.0, switch_tol = -1.0;
     bool operator==(const GraphKey &o) const {
       return fused == o.fused && wmode == o.wmode && npl == o.npl && neg_tol == o.neg_tol &&
              ctl_mode == o.ctl_mode && ctl_cap == o.ctl_cap && cut32 == o.cut32 && switch_tol == o.switch_tol &&
-             ncp3 == o.ncp3 && sel_rounds == o.sel_rounds && sel_limit == o.sel_limit && cyclic == o.cyclic;
+             ncp3 == o.ncp3 && sel_rounds == o.sel_rounds && sel_limit == o.sel_limit && cyclic == o.cyclic &&
+             frame_every == o.frame_every && frame_size == o.frame_size && frame_max == o.frame_max;
     }
   } gkey[2];
   StepState st_init{};  // the loop state of the current problem before its first step
@@ -64,6 +65,10 @@ struct jkg_ctx {
   int ncp3_n = 0;  // ID_NCP=3: per-offset totals allocated for this N
   // selection schedule: walk rounds per step (0: by size) and entries walked before each filter
   int sel_rounds = 0, sel_limit = 49152;
+  // per-step frames of |A| (jkg_set_frames): device buffer, schedule, caller's buffer and count
+  uint8_t *frames_dev = nullptr, *frames_host = nullptr;
+  int32_t *frames_count = nullptr;
+  int frame_every = 0, frame_size = 0, frame_max = 0;
   // step controller and its training (allocated on first use, kept across problems)
   float *ctl_master = nullptr, *ctl_table = nullptr, *ctl_trace = nullptr;
   uint16_t *ctl_q = nullptr;
@@ -358,7 +363,7 @@ void jkg_destroy(jkg_ctx *c) {
   free_all(c);
   if (c->tmp) (void)hipFree(c->tmp);
   for (void *p : {(void *)c->ctl_master, (void *)c->ctl_q, (void *)c->ctl_table, (void *)c->ctl_trace,
-                  (void *)c->tr_buf})
+                  (void *)c->tr_buf, (void *)c->frames_dev})
     if (p) (void)hipFree(p);
   (void)hipEventDestroy(c->ev0);
   (void)hipEventDestroy(c->ev1);
@@ -420,7 +425,7 @@ double jkg_key_weight(uint64_t key) {
 
 // ---------------------------------------------------------------- problem setup
 static int upload_problem(jkg_ctx *c, int n, int n2, const void *y, const int64_t *j, int *npos, int *nneg,
-                          int max_steps, int trace_cap, double tau = 0.0) {
+                          int max_steps, int trace_cap, double tau = 0.0, double tau_cap = 1e-6) {
   // (the element size follows the buffers: c->d.cplx)
   DeviceStep &d = c->d;
   std::vector<int> J32(n), pl, mi;
@@ -447,6 +452,7 @@ static int upload_problem(jkg_ctx *c, int n, int n2, const void *y, const int64_
   s.n2f = 1.f;
   s.ltau = (tau > 0.0) ? (float)std::log10(tau) : -16.f;
   s.off_best = HUGE_VAL;
+  s.ltau_cap = (float)std::log10(tau_cap);
   {
     double y2 = 0.0;  // ||Y||_F^2 (the controller's scale)
     const double *yd = (const double *)y;
@@ -475,6 +481,9 @@ static int capture_step(jkg_ctx *c, int fused) {
   key.sel_rounds = c->d.sel_rounds;
   key.sel_limit = c->d.select_limit;
   key.cyclic = c->d.cyclic;
+  key.frame_every = c->d.frames ? c->d.frame_every : 0;
+  key.frame_size = c->d.frames ? c->d.frame_size : 0;
+  key.frame_max = c->d.frames ? c->d.frame_max : 0;
   key.switch_tol = slot ? c->d.switch_tol : 0.0;
   if (c->exec[slot] && c->gkey[slot] == key) return JKG_OK;
   if (c->exec[slot]) (void)hipGraphExecDestroy(c->exec[slot]);
@@ -609,6 +618,7 @@ struct MixedCfg {
   const float *ctl = nullptr;
   float *ctl_trace = nullptr;
   int ctl_trace_cap = 0;
+  double tau_cap = 1e-6;
   int cyclic = 0;  // round-robin sweeps in place of the dynamic ordering
   // FP32 warm start
   int warm = 0, cut32 = 0;
@@ -733,6 +743,10 @@ static int solve_common(jkg_ctx *c, int cplx, int n, int n2, const void *y, cons
     if (rc) return rc;
   }
   c->d.cyclic = mc.cyclic;
+  c->d.frames = (c->frame_every > 0) ? c->frames_dev : nullptr;
+  c->d.frame_every = c->frame_every;
+  c->d.frame_size = c->frame_size;
+  c->d.frame_max = c->frame_max;
   c->d.ncp3 = 0;
   if (o.id_ncp == 3) {
     rc = ensure_ncp3(c);
@@ -763,7 +777,7 @@ static int solve_common(jkg_ctx *c, int cplx, int n, int n2, const void *y, cons
   }
   // TT: the number of trigonometric pairs (the reference's R%TRU result)
   int npos = 0, nneg = 0;
-  rc = upload_problem(c, n, n2, y, j, &npos, &nneg, o.max_steps, c->trace_cap, tau);
+  rc = upload_problem(c, n, n2, y, j, &npos, &nneg, o.max_steps, c->trace_cap, tau, mc.tau_cap);
   if (rc) return rc;
   res->npos = npos;
   res->nneg = nneg;
@@ -774,6 +788,13 @@ static int solve_common(jkg_ctx *c, int cplx, int n, int n2, const void *y, cons
   if (o.trace && o.trace_cap > 0) {
     int cnt = std::min(res->steps, o.trace_cap);
     if (cnt > 0) HIPCHK(hipMemcpy(o.trace, d.trace, sizeof(jkg_steprec) * cnt, hipMemcpyDeviceToHost));
+  }
+  if (c->d.frames && c->frames_host) {
+    const int cnt = res->steps > 0 ? std::min((res->steps - 1) / c->frame_every + 1, c->frame_max) : 0;
+    if (cnt > 0)
+      HIPCHK(hipMemcpy(c->frames_host, c->frames_dev, (size_t)cnt * c->frame_size * c->frame_size,
+                       hipMemcpyDeviceToHost));
+    if (c->frames_count) *c->frames_count = cnt;
   }
   if (mc.ctl_mode && mc.ctl_trace && mc.ctl_trace_cap > 0) {
     int cnt = std::min(res->steps, mc.ctl_trace_cap);
@@ -811,6 +832,7 @@ extern "C" int jkg_dsolve_mixed(jkg_ctx *c, int n, int n2, const double *y, cons
   if (m.weights != JKG_W_PAIR && m.weights != JKG_W_GRAM) return -10;
   if (!(m.tol >= 0.0) || !(m.threshold >= 0.0 && m.threshold < 1.0)) return -10;
   if (m.controller < 0 || m.controller > 2 || (m.controller && !m.ctl) || m.ctl_trace_cap < 0) return -10;
+  if (!(m.ctl_tau_max >= 0.0 && m.ctl_tau_max <= 1e-6)) return -10;
   if (m.ordering < 0 || m.ordering > 1) return -10;
   if (m.fp32_warm != 0 || m.key_bits != 64) return JKG_ENOTSUP;
   if (m.weights == JKG_W_PAIR && (m.tol != 0.0 || m.controller)) return JKG_ENOTSUP;
@@ -824,6 +846,7 @@ extern "C" int jkg_dsolve_mixed(jkg_ctx *c, int n, int n2, const double *y, cons
   mc.ctl = m.ctl;
   mc.ctl_trace = m.ctl_trace;
   mc.ctl_trace_cap = m.ctl_trace ? m.ctl_trace_cap : 0;
+  mc.tau_cap = m.ctl_tau_max > 0.0 ? m.ctl_tau_max : 1e-6;
   return solve_common(c, 0, n, n2, y, j, yu, zz, sy, opts, res, mc);
 }
 
@@ -960,6 +983,7 @@ static int prepare_single(jkg_ctx *c, int n, int n2, const void *a, const int64_
   c->d.ctl_mode = 0;
   c->d.ncp3 = 0;
   c->d.cyclic = 0;
+  c->d.frames = nullptr;
   int npos, nneg;
   return upload_problem(c, n, jkg_jstep_len(n, n2), a, j, &npos, &nneg, -1, c->trace_cap);
 }
@@ -1231,6 +1255,35 @@ extern "C" int jkg_ncp(jkg_ctx *c, int n, int nn, int nm, jkg_aw *dz, int n2, in
   return JKG_OK;
 }
 
+// Frames of |A| for later solves on this context (see k_frame): every `every`-th step
+// (<= 0: off), size x size bytes each, at most max_frames; after each solve they are
+// copied to frames (max_frames * size^2 bytes) and their number to *count.
+extern "C" int jkg_set_frames(jkg_ctx *c, int every, int size, int max_frames, uint8_t *frames,
+                              int32_t *count) {
+  if (!c) return -1;
+  HIPCHK(hipSetDevice(c->device));
+  if (c->frames_dev) (void)hipFree(c->frames_dev);
+  c->frames_dev = nullptr;
+  c->frame_every = 0;
+  c->frames_host = nullptr;
+  c->frames_count = nullptr;
+  if (every <= 0) return JKG_OK;
+  if (size < 1 || size > 1024) return -3;
+  if (max_frames < 1) return -4;
+  if (!frames) return -5;
+  if (hipMalloc(&c->frames_dev, (size_t)max_frames * size * size) != hipSuccess) {
+    (void)hipGetLastError();
+    c->frames_dev = nullptr;
+    return JKG_ENOMEM;
+  }
+  c->frame_every = every;
+  c->frame_size = size;
+  c->frame_max = max_frames;
+  c->frames_host = frames;
+  c->frames_count = count;
+  return JKG_OK;
+}
+
 // The selection schedule of later solves on this context: `rounds` walks per step (0: by size),
 // each but the last stopping after `limit` entries of its list, followed by a
 // whole-GPU pass that drops the entries with a used index.  The pivots chosen do
@@ -1396,6 +1449,7 @@ static int profile_common(jkg_ctx *c, int n, int n2, const double *y, const int6
   c->d.ctl_mode = 0;
   c->d.ncp3 = 0;
   c->d.cyclic = 0;
+  c->d.frames = nullptr;
   if (wmode) {
     rc = ensure_mixed(c);
     if (rc) return rc;

@@ -157,6 +157,7 @@ __device__ __forceinline__ double absz(dz a) { return hypot_ref(a.re, a.im); }  
 struct fz {
   float re, im;
 };
+__device__ __forceinline__ dz ldz(const double &a) { return dz{a, 0.0}; }
 __device__ __forceinline__ dz ldz(const dz &a) { return a; }
 __device__ __forceinline__ dz ldz(const fz &a) { return dz{(double)a.re, (double)a.im}; }
 template <class E>

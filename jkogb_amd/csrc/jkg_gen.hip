@@ -91,3 +91,53 @@ void gen_rows_launch(int n, int npairs, double *G, const int *pa, const int *pb,
 }
 
 }  // namespace jkg
+
+namespace jkg {
+
+// ------------------------------------------------------------------ frames
+// Per-step pictures of |A| (the role of the reference's ANIMATE frames,
+// src/D/dstep.F90:485-495, 549-592, which libpvn's PVN_RVIS_* write): at the
+// start of every `every`-th step (A before its transforms, as the reference
+// records it) an S x S map, pixel (u, v) = the largest |a_ij| over the rows
+// and columns that map to it, on a fixed logarithmic scale: 8 levels per
+// decade from 1e-16 (level 1) to 1e+15.875 (255); 0 for exact zeros.  One
+// workgroup per pixel row; frame f goes to frames[f * S * S], row-major.
+template <class E>
+__global__ void __launch_bounds__(256) k_frame(int n, const E *__restrict__ A, int lda, int size, int every,
+                                               int max_frames, uint8_t *__restrict__ frames, const StepState *st) {
+  if (!st->active) return;
+  const int s = st->step;
+  if (s % every != 0 || s / every >= max_frames) return;
+  uint8_t *f = frames + (size_t)(s / every) * size * size + (size_t)blockIdx.x * size;
+  const int u = blockIdx.x;
+  const int r0 = (int)((long)u * n / size), r1 = (int)((long)(u + 1) * n / size);
+  for (int v = threadIdx.x; v < size; v += blockDim.x) {
+    const int c0 = (int)((long)v * n / size), c1 = (int)((long)(v + 1) * n / size);
+    double m = 0.0;
+    for (int c = c0; c < c1; ++c)
+      for (int r = r0; r < r1; ++r) {
+        const dz a = ldz(A[(size_t)c * lda + r]);
+        const double x = __builtin_sqrt(a.re * a.re + a.im * a.im);
+        m = x > m ? x : m;
+      }
+    int lev = 0;
+    if (m > 0.0) {
+      lev = 1 + (int)__builtin_floor((log10(m) + 16.0) * 8.0);
+      lev = lev < 1 ? 1 : (lev > 255 ? 255 : lev);
+    }
+    f[v] = (uint8_t)lev;
+  }
+}
+
+void frame_launch(DeviceStep &d, hipStream_t s) {
+  if (!d.frames) return;
+  auto go = [&](auto kern, const auto *A) {
+    hipLaunchKernelGGL(kern, dim3(d.frame_size), dim3(256), 0, s, d.n, A, d.lda, d.frame_size, d.frame_every,
+                       d.frame_max, d.frames, d.st);
+  };
+  if (d.cplx && d.prec32) go(k_frame<fz>, (const fz *)d.A32);
+  else if (d.cplx) go(k_frame<dz>, (const dz *)d.A);
+  else go(k_frame<double>, (const double *)d.A);
+}
+
+}  // namespace jkg

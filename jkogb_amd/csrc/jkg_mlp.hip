@@ -246,6 +246,7 @@ __global__ void __launch_bounds__(64) k_controller(StepState *st, int mode, cons
     const int m = st->n2max;
     int n2 = (int)(n2f * (float)m + 0.5f);
     st->n2 = n2 < 1 ? 1 : (n2 > m ? m : n2);
+    if (ltau > st->ltau_cap) ltau = st->ltau_cap;  // the caller's bound on the threshold
     st->tau = (ltau <= -15.5f) ? 0.0 : exp10((double)ltau);
     st->n2f = n2f;
     st->ltau = ltau;

@@ -968,6 +968,7 @@ int step_prepare(DeviceStep &d) {
 
 void step_launch(DeviceStep &d, hipStream_t s) {
   step_begin_launch(d, s);
+  frame_launch(d, s);
   if (d.cyclic) {
     hipLaunchKernelGGL(k_cyc_select, dim3(1), dim3(256), 0, s, d.n, d.piv_p, d.piv_q, d.piv_pos, d.st);
     pivots_launch(d, s);

@@ -63,6 +63,7 @@ struct StepState {
   double tw0, twprev, ynorm2;  // TW of the first and of the previous step; ||Y||_F^2
   unsigned long long t_begin;  // wall clock (100 MHz) at the start of this step
   float ema, n2f, ltau;        // smoothed TW rate; the action of this step (N_2 / n2max, log10 tau)
+  float ltau_cap;              // largest log10 tau a controller may set
   float feat[16];              // controller features after the last step
   // FP32 warm start: relative off-norm of A (this step's), the best seen and its step, why the phase ended
   double off_rel, off_best;
@@ -122,6 +123,9 @@ struct DeviceStep {
   double *ncp3_wsum;      // NN: total weight of the walk from each start offset
   // cyclic ordering (the sweep baseline of the mixed solver): round-robin pivots, no weights
   int cyclic;
+  // per-step frames of |A| (jkg_gen.hip): every frame_every-th step, frame_size^2 bytes each
+  uint8_t *frames;
+  int frame_size, frame_every, frame_max;
 };
 
 // test-matrix generator (jkg_gen.hip)
@@ -129,6 +133,7 @@ void gen_diag_launch(int n, const double *d, double *G, hipStream_t s);
 void gen_cols_launch(int n, int npairs, double *G, const int *pa, const int *pb, const double *cs, const int *J,
                      hipStream_t s);
 int gen_rows_prepare(int n);
+void frame_launch(DeviceStep &d, hipStream_t s);
 void gen_rows_launch(int n, int npairs, double *G, const int *pa, const int *pb, const double *cs, hipStream_t s);
 
 constexpr int NCP3_NMAX = 8192;

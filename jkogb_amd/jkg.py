@@ -66,7 +66,7 @@ class MixedOpts(C.Structure):
                 ("key_bits", C.c_int32), ("controller", C.c_int32), ("ctl", C.c_void_p),
                 ("threshold", C.c_double), ("tol", C.c_double), ("ctl_trace", C.c_void_p),
                 ("ctl_trace_cap", C.c_int32), ("ordering", C.c_int32), ("warm", C.c_void_p),
-                ("reserved", C.c_int32 * 4)]
+                ("ctl_tau_max", C.c_double), ("reserved", C.c_int32 * 2)]
 
 
 class WarmInfo(C.Structure):
@@ -118,6 +118,7 @@ _ncp = _sig("jkg_ncp", _i, [_vp, _i, _i, _i, _vp, _i, C.POINTER(C.c_int32), _vp,
 _ncp3 = _sig("jkg_ncp3", _i, [_vp, _i, _i, _i, _vp, _i, C.POINTER(C.c_int32), _vp, C.POINTER(C.c_int64)])
 _dgen = _sig("jkg_dgen", _i, [_vp, _i, _vp, _i, _d, C.c_uint64, _vp, _vp])
 _set_select = _sig("jkg_set_select", _i, [_vp, _i, _i])
+_set_frames = _sig("jkg_set_frames", _i, [_vp, _i, _i, _i, _vp, _vp])
 _tru = _sig("jkg_tru", _i, [_i, _i, _vp, _i, _vp, _vp, _vp])
 _jstep_len = _sig("jkg_jstep_len", _i, [_i, _i])
 _dhsvd2 = _sig("jkg_dhsvd2", _i, [_vp, _i, _vp, _vp, _d, _i, _vp, _vp, _vp, _vp])
@@ -152,7 +153,7 @@ EXPORTS = ["jkg_create", "jkg_destroy", "jkg_strerror", "jkg_last_hip_error", "j
            "jkg_dweights", "jkg_zweights", "jkg_dbuild", "jkg_dstep", "jkg_dprofile", "jkg_pair_index",
            "jkg_key_weight", "jkg_mixed_opts_default", "jkg_dsolve_mixed", "jkg_dweights_mixed",
            "jkg_dprofile_mixed", "jkg_ctl_grad", "jkg_ctl_fit", "jkg_batch_opts_default", "jkg_dsolve_batched",
-           "jkg_zsolve_mixed", "jkg_zweights_mixed", "jkg_zprofile", "jkg_zprofile_mixed", "jkg_ncp3", "jkg_dgen", "jkg_set_select"]
+           "jkg_zsolve_mixed", "jkg_zweights_mixed", "jkg_zprofile", "jkg_zprofile_mixed", "jkg_ncp3", "jkg_dgen", "jkg_set_select", "jkg_set_frames"]
 
 
 def mixed_opts(**kw):
@@ -419,6 +420,23 @@ class Context:
                _p(ms), C.byref(r)), "jkg_zprofile" if cplx else "jkg_dprofile")
         return dict(stage_ms=dict(zip(STAGES, ms.tolist())), steps=r.steps, n2=r.n2, seconds=r.seconds,
                     **_counts(r))
+
+    def set_frames(self, every=1, size=64, max_frames=256):
+        """Record size x size pictures of |A| every `every` steps of later solves (jkg_set_frames);
+        after a solve, frames() returns them as (count, size, size) uint8.  every=0 turns it off."""
+        if every <= 0:
+            _chk(_set_frames(self._h, 0, 0, 0, None, None), "jkg_set_frames")
+            self._frames = None
+            return
+        buf = np.zeros((max_frames, size, size), np.uint8)
+        cnt = np.zeros(1, np.int32)
+        _chk(_set_frames(self._h, every, size, max_frames, _p(buf), _p(cnt)), "jkg_set_frames")
+        self._frames = (buf, cnt)
+
+    def frames(self):
+        """The frames of the last solve (see set_frames)."""
+        buf, cnt = getattr(self, "_frames", None) or (np.zeros((0, 0, 0), np.uint8), np.zeros(1, np.int32))
+        return buf[:int(cnt[0])].copy()
 
     def set_select(self, rounds=0, limit=49152):
         """The greedy walk's schedule (jkg_set_select); the pivots do not depend on it."""
