@@ -20,7 +20,7 @@ Printed as one JSON line:
                its HBM bytes per step measured by rocprofv3 FETCH_SIZE and
                WRITE_SIZE, scaled by a plain copy's readings (the summary in
                profiles/hbm_traffic_n4096_step_stages.md)
-  rooflines    the Gram product (FP64 MFMA) and the rotations (HBM)
+  rooflines    the weights stage (FP64 MFMA) and the rotation kernel (HBM)
   full_solve   complete HSVDs at N = 1024 (C2 input): plain, mixed, and mixed
                with the trained step controller (jkogb_amd/ctl_params.npy):
                device time, steps, and the singular values' largest
@@ -126,6 +126,39 @@ def timed_steps(ctx, solve, y, j, n2, steps, warmup, **kw):
     return dict(steps=k, ms_per_step=ms, gbs=b / (ms * 1e-3) / 1e9, bytes=b)
 
 
+def c5_full_run(ctx, y, j, n2, fused, plain_cap, log):
+    """Complete solves of the N=4096 input: the mixed solver, the mixed solver with the trained
+    controller, and (plain_cap > 0) the plain solver for at most plain_cap steps; singular values
+    compared with the plain run's (a lower bound on its time if it did not end)."""
+    n = y.shape[0]
+    eps = np.finfo(float).eps
+    ctl = np.load(os.path.join(ROOT, "jkogb_amd", "ctl_params.npy"))
+    last = [0]
+
+    def prog(step, done):
+        if done or step - last[0] >= 5000:
+            last[0] = step
+            log("c5 solve: step %d%s" % (step, " (done)" if done else ""))
+
+    out = {"workload": "N=4096 real, column scales 10**U(-6,6), J 2048/2048, N_2=2048, complete solves"}
+    runs = {}
+    for name, kw in (("mixed", {}), ("learned", {"ctl_params": ctl})):
+        last[0] = 0
+        r = ctx.dsolve_mixed(y, j, n2, fused=fused, trace_cap=1, progress=prog, **kw)
+        runs[name] = r
+        out[name] = {"steps": r["steps"], "seconds": round(r["seconds"], 3)}
+    if plain_cap > 0:
+        last[0] = 0
+        p = ctx.dsolve(y, j, n2, fused=fused, max_steps=plain_cap, trace_cap=1, progress=prog)
+        done = p["info"] < plain_cap
+        out["plain"] = {"steps": p["steps"], "seconds": round(p["seconds"], 3), "converged": bool(done)}
+        for name, r in runs.items():
+            rel = np.abs(r["SY"] - p["SY"]) / np.abs(p["SY"]) / (n * eps)
+            out[name]["sy_err_vs_plain_N_eps"] = round(float(rel.max()), 3)
+            out[name]["speedup_vs_plain"] = ("%s%.2f" % ("" if done else ">=", p["seconds"] / r["seconds"]))
+    return out
+
+
 def complex_run(ctx, fused, log, steps=20, full=True):
     """The complex configuration: per-step times and complete solves of the FP64 mixed solver
     and of its FP32 warm start."""
@@ -193,17 +226,19 @@ def batched_run(ctx, count, fused, sample=8):
     eps = np.finfo(float).eps
     outs, run = ctx.dsolve_batched(ys, js, fused=fused, tol=eps)
     infos = np.array([o["info"] for o in outs])
-    t1, worst = 0.0, 0.0
+    t1, worst, vs_eig = 0.0, 0.0, 0.0
     for i in rng.choice(count, sample, replace=False):
         n, y, j = int(ns[i]), ys[i], js[i]
         r = ctx.dsolve_mixed(y, j, n // 2, fused=fused)
         t1 += r["seconds"]
-        # exact values: sqrt|eig(Y J Y^T)|, the eigenvalue signs giving the classes; compared sorted per class
+        worst = max(worst, float(np.max(np.abs(outs[i]["SY"] - r["SY"]) / np.abs(r["SY"]))) / (n * eps))
+        # sqrt|eig(Y J Y^T)| in float64, per J class; its own error is about N eps (sigma + |Y|^2 / sigma)
         ev = np.linalg.eigvalsh(y @ np.diag(j.astype(float)) @ y.T)
+        y2 = np.linalg.norm(y, 2) ** 2
         for cls in (1, -1):
             a = np.sort(np.abs(outs[i]["SY"][j == cls]))
             e = np.sort(np.sqrt(np.abs(ev[ev > 0] if cls > 0 else ev[ev < 0])))
-            worst = max(worst, float(np.max(np.abs(a - e) / e)) / (n * eps))
+            vs_eig = max(vs_eig, float(np.max(np.abs(a - e) / (n * eps * (e + y2 / e)))))
     single = t1 * count / sample
     return {"workload": "%d independent HSVDs, N uniform in [16,256] (sum N^2 = %d), Gaussian Y, random J, "
                         "N_2=N/2, one persistent launch per step, tol=eps" % (count, int(np.sum(ns.astype(np.int64) ** 2))),
@@ -211,10 +246,12 @@ def batched_run(ctx, count, fused, sample=8):
             "variant": run["variant"], "tune_seconds": round(run["tune_seconds"], 3),
             "instance_steps": {"min": int(infos.min()), "median": int(np.median(infos)), "max": int(infos.max())},
             "single_projected_seconds": round(single, 2), "speedup_vs_single": round(single / run["seconds"], 2),
-            "sy_err_vs_exact_N_eps_sampled": round(worst, 4),
+            "sy_err_vs_single_N_eps_sampled": round(worst, 4),
+            "sy_vs_eig_in_eig_error_units": round(vs_eig, 4),
             "note": "single_projected: %d sampled instances solved one at a time by jkg_dsolve_mixed (the same "
-                    "convergence test), scaled to %d; sy_err: the batch's largest relative difference from "
-                    "sqrt|eig(Y J Y^T)| (float64, per J class) on the sample" % (sample, count)}
+                    "convergence test), scaled to %d; sy_err_vs_single: the batch's largest relative difference "
+                    "from those solves; sy_vs_eig: its largest difference from sqrt|eig(Y J Y^T)| (float64, per J "
+                    "class) in units of that reference's own error N eps (sigma + |Y|^2/sigma)" % (sample, count)}
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_n4096_step_stages.md")
@@ -267,6 +304,10 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-complex", action="store_true")
     ap.add_argument("--no-complex-full", action="store_true", help="complex configuration: per-step times only")
+    ap.add_argument("--c5-full", action="store_true",
+                    help="also complete N=4096 solves (mixed, learned; minutes each) and a plain run of "
+                         "--c5-plain-cap steps, reported as c5_full")
+    ap.add_argument("--c5-plain-cap", type=int, default=0, help="steps of the plain run of --c5-full (0: none)")
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--traffic-workload", action="store_true",
                     help="only the workload of the HBM-traffic counter runs: a 512 MiB device copy (three times, "
@@ -330,6 +371,9 @@ def main():
                 "note": "sy_err: largest relative SY difference against the plain (reference-exact) solver; "
                         "reference_build_spread: the same between the reference's unfused and fused builds"}
         log("full solves at N=1024")
+    c5 = None
+    if args.c5_full:
+        c5 = c5_full_run(ctx, y, j, n2, args.fused, args.c5_plain_cap, log)
     cplx = None
     if not args.no_complex:
         cplx = complex_run(ctx, args.fused, log, full=not args.no_complex_full)
@@ -395,6 +439,7 @@ def main():
                               "plain": {s: round(v, 4) for s, v in pp.items()}},
         "cpu_baseline": cpu,
         "full_solve": full,
+        "c5_full": c5,
         "complex": cplx,
         "batched": batched,
         "host_wall_s": round(wall, 3),

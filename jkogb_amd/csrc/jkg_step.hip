@@ -173,30 +173,26 @@ __global__ void __launch_bounds__(256) k_hyp_weights(int n, const double *__rest
 }
 
 // ------------------------------------------------------------------ (c) select
-constexpr int SEL_THREADS_RES = 512;
-constexpr int SEL_WAVES = SEL_THREADS_RES / 64;
 constexpr int FILT_TILE = 2048;
 
-// One workgroup walks the current list in order, RES_CHUNK entries at a time
-// (RES_ITEMS consecutive ones per lane).  At the start of a chunk each lane
-// tests its entries against used[] (n + 1 bytes; used[n] = 1 is the sentinel
-// that past-the-end entries, 0xFFFFFFFF, map to) and keeps a bit mask of the
-// free ones.  One iteration accepts the first free entry of the chunk:
-//   - a wave finds its first free entry with one ballot (lanes hold increasing
-//     positions), reads it with readlane, and writes (index, pair[, pos]) to an
-//     LDS slot;
-//   - one barrier; every lane reads the SEL_WAVES slots (one LDS round trip)
-//     and takes the first non-empty one;
-//   - lanes clear the entries that share an index with the accepted pair
-//     (packed 16-bit compare: a half of v ^ (x * 0x10001) is zero iff that
-//     half of v equals x); waves with nothing free skip this.
-// A chunk ends with an iteration that finds nothing; its barrier also
-// publishes the last used[] writes to the next chunk's test.  The barriers
-// fence LDS only, so the next chunk's loads stay in flight.  The walk stops
-// after `limit` entries, at the end of the list, or at N_2 pivots (AW_NCP2
-// semantics, src/shared/atypes.F90:191-230).
-constexpr int RES_ITEMS = 4;
-constexpr int RES_CHUNK = SEL_THREADS_RES * RES_ITEMS;
+// One wave walks the current list in order, W_CHUNK entries at a time
+// (W_ITEMS consecutive ones per lane).  At the start of a chunk each lane
+// tests its entries against used[] (n + 1 bytes in LDS; used[n] = 1 is the
+// sentinel that past-the-end entries, 0xFFFFFFFF, map to) and keeps a bit
+// mask of the free ones.  One iteration accepts the first free entry of the
+// chunk: a ballot finds the first lane with a free entry (lanes hold
+// increasing positions), readlane takes that entry, lane 0 marks its indices
+// in used[] and writes the pivot, and every lane clears the entries that share
+// an index with it (packed 16-bit compare: a half of v ^ (x * 0x10001) is zero
+// iff that half of v equals x).  A chunk ends with an iteration that finds
+// nothing.  No barrier anywhere: the wave's own LDS accesses are ordered, so
+// the next chunk's test sees this chunk's marks, and the loads of the next
+// W_DEPTH chunks are in flight while this one resolves.  The walk stops after `limit`
+// entries, at the end of the list, or at N_2 pivots (AW_NCP2 semantics,
+// src/shared/atypes.F90:191-230).
+constexpr int W_ITEMS = 4;
+constexpr int W_CHUNK = 64 * W_ITEMS;
+constexpr int W_DEPTH = 8;  // chunks whose loads are in flight (one global round trip per chunk otherwise)
 
 __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
@@ -211,144 +207,137 @@ __device__ __forceinline__ uint32_t pair_hits(uint32_t v, uint32_t P, uint32_t Q
   return (((x - 0x10001u) & ~x) | ((y - 0x10001u) & ~y)) & 0x80008000u;
 }
 
+// First index in [lo, hi) at which the monotone predicate holds (hi if none),
+// by one wave: 64 probes per round cut the interval by a factor of 64.
+template <class Pred>
+__device__ int wave_first_true(int lo, int hi, Pred pred) {
+  const int lane = threadIdx.x & 63;
+  while (lo < hi) {
+    const int step = (hi - lo + 63) >> 6;
+    const int p = lo + (lane + 1) * step - 1;
+    const bool t = (p < hi) ? pred(p) : true;
+    const uint64_t b = __ballot(t);
+    const int k = b ? __ffsll((unsigned long long)b) - 1 : 64;
+    const int nhi = (k < 64) ? min(lo + (k + 1) * step - 1, hi) : hi;
+    lo = min(lo + k * step, hi);
+    hi = nhi;
+  }
+  return lo;
+}
+
 template <bool HAS_POS>
-__global__ void __launch_bounds__(SEL_THREADS_RES) k_sel_resolve(int n, const uint64_t *k0, const uint64_t *k1,
-                                                                 const uint32_t *v0, const uint32_t *v1,
-                                                                 const int *sort_skip, ListPtrs lp,
-                                                                 uint8_t *used_g, int *piv_p, int *piv_q,
-                                                                 int *piv_pos, StepState *st, int round, int limit,
-                                                                 int nn) {
-  extern __shared__ uint8_t used[];  // n + 1 bytes
-  __shared__ __attribute__((aligned(16))) uint32_t slot[2][3 * SEL_WAVES];  // index | pair | pos
-  __shared__ int itv;
+__global__ void __launch_bounds__(64) k_sel_resolve(int n, const uint64_t *k0, const uint64_t *k1,
+                                                    const uint32_t *v0, const uint32_t *v1, const int *sort_skip,
+                                                    ListPtrs lp, uint8_t *used_g, int *piv_p, int *piv_q,
+                                                    int *piv_pos, StepState *st, int round, int limit, int nn) {
+  extern __shared__ uint32_t usedw[];  // n + 1 flag bytes, rounded up to words
+  uint8_t *used = (uint8_t *)usedw;
   if (st->sel_done || st->sel_round != round) return;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lane = threadIdx.x;
   SortResult sr = sort_result_dev(k0, k1, v0, v1, sort_skip);
   const uint32_t *vals = (round == 0) ? sr.vals : lp.val[(round - 1) & 1];
   const uint32_t *poss = HAS_POS ? lp.pos[(round - 1) & 1] : nullptr;
+  const int nw = n >> 2;
   int len;
   if (round == 0) {
-    if (tid == 0) {
-      // IT = number of finite weights = position of the first NaN key
-      const uint64_t *keys = sr.keys;
-      int lo = 0, hi = nn;
-      while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if (key_nan(keys[mid])) hi = mid;
-        else lo = mid + 1;
-      }
-      st->it = lo;
-      st->wmax = (lo > 0) ? key_weight(keys[0]) : 0.0;
-      // the mixed solver's threshold: walk only the weights >= tau * (largest weight)
-      const double tau = st->tau;
-      if (tau > 0.0 && lo > 0) {
-        const double wmax = key_weight(keys[0]);
-        if (wmax > 0.0) {
-          const uint64_t kt = weight_key(tau * wmax);  // keys above kt stand for smaller weights
-          int a = 0, b = lo;
-          while (a < b) {
-            const int mid = (a + b) >> 1;
-            if (keys[mid] > kt) b = mid;
-            else a = mid + 1;
-          }
-          lo = a;
-        }
-      }
-      itv = lo;
-      st->list_len = lo;
+    // IT = number of finite weights = position of the first NaN key
+    const uint64_t *keys = sr.keys;
+    const int it = wave_first_true(0, nn, [&](int i) { return key_nan(keys[i]); });
+    const double wmax = (it > 0) ? key_weight(keys[0]) : 0.0;
+    len = it;
+    // the mixed solver's threshold: walk only the weights >= tau * (largest weight)
+    const double tau = st->tau;
+    if (tau > 0.0 && it > 0 && wmax > 0.0) {
+      const uint64_t kt = weight_key(tau * wmax);  // keys above kt stand for smaller weights
+      len = wave_first_true(0, it, [&](int i) { return keys[i] > kt; });
     }
-    for (int i = tid; i <= n; i += SEL_THREADS_RES) used[i] = (i == n);
-    __syncthreads();
-    len = itv;
+    if (lane == 0) {
+      st->it = it;
+      st->wmax = wmax;
+      st->list_len = len;
+    }
+    for (int i = lane; i <= nw; i += 64) usedw[i] = 0u;
   } else {
-    for (int i = tid; i <= n; i += SEL_THREADS_RES) used[i] = (i == n) ? 1 : used_g[i];
-    __syncthreads();
+    const uint32_t *ug = (const uint32_t *)used_g;  // 256-byte aligned allocation of n bytes
+    for (int i = lane; i < nw; i += 64) usedw[i] = ug[i];
+    if (lane < (n & 3)) used[4 * nw + lane] = used_g[4 * nw + lane];
     len = st->list_len;
   }
+  if (lane == 0) used[n] = 1;
+  lds_barrier();
   const int n2 = st->n2;
   int acc = st->sl;
   int c = st->consumed;
   const int stop = (limit > 0 && len - c > limit) ? c + limit : len;
-  int par = 0;
-  uint32_t nv[RES_ITEMS], np[RES_ITEMS];
-  auto load_chunk = [&](int base) {
+  // ring of W_DEPTH chunks in flight: chunk m of the walk (from `consumed`) is in registers rv[m % W_DEPTH]
+  uint32_t rv[W_DEPTH][W_ITEMS], rp[W_DEPTH][W_ITEMS];
+  auto load_chunk = [&](uint32_t(&dv)[W_ITEMS], uint32_t(&dp)[W_ITEMS], int base) {
 #pragma unroll
-    for (int i = 0; i < RES_ITEMS; ++i) {
-      const int e = base + tid * RES_ITEMS + i;
-      nv[i] = (e < stop) ? vals[e] : 0xFFFFFFFFu;
-      if constexpr (HAS_POS) np[i] = (e < stop) ? poss[e] : 0u;
+    for (int i = 0; i < W_ITEMS; ++i) {
+      const int e = base + lane * W_ITEMS + i;
+      dv[i] = (e < stop) ? vals[e] : 0xFFFFFFFFu;
+      if constexpr (HAS_POS) dp[i] = (e < stop) ? poss[e] : 0u;
     }
   };
-  if (c < stop) load_chunk(c);
-  while (c < stop && acc < n2) {
-    uint32_t vv[RES_ITEMS], pv[RES_ITEMS];
-    uint32_t alive = 0;
 #pragma unroll
-    for (int i = 0; i < RES_ITEMS; ++i) {
-      vv[i] = nv[i];
-      if constexpr (HAS_POS) pv[i] = np[i];
-      const uint32_t p = min(vv[i] & 0xFFFFu, (uint32_t)n), q = min(vv[i] >> 16, (uint32_t)n);
-      alive |= ((used[p] | used[q]) == 0 ? 1u : 0u) << i;
-    }
-    const int base = c;
-    c = (c + RES_CHUNK < stop) ? c + RES_CHUNK : stop;
-    if (c < stop) load_chunk(c);  // in flight while this chunk resolves
-    for (;;) {
-      const int fi = __ffs(alive) - 1;
-      uint32_t fv = vv[0], fp = HAS_POS ? pv[0] : 0u;
+  for (int k = 0; k < W_DEPTH; ++k) load_chunk(rv[k], rp[k], c + k * W_CHUNK);
+  bool run = c < stop && acc < n2;
+  while (run) {
 #pragma unroll
-      for (int i = 1; i < RES_ITEMS; ++i) {
-        fv = (fi == i) ? vv[i] : fv;
-        if constexpr (HAS_POS) fp = (fi == i) ? pv[i] : fp;
+    for (int k = 0; k < W_DEPTH; ++k) {
+      if (!run) break;
+      uint32_t vv[W_ITEMS], pv[W_ITEMS];
+      uint32_t alive = 0;
+#pragma unroll
+      for (int i = 0; i < W_ITEMS; ++i) {
+        vv[i] = rv[k][i];
+        if constexpr (HAS_POS) pv[i] = rp[k][i];
+        const uint32_t p = min(vv[i] & 0xFFFFu, (uint32_t)n), q = min(vv[i] >> 16, (uint32_t)n);
+        alive |= ((used[p] | used[q]) == 0 ? 1u : 0u) << i;
       }
-      const uint64_t bal = __ballot(alive != 0);
-      uint32_t widx = 0x7FFFFFFFu, wval = 0, wpos = 0;
-      if (bal != 0) {
+      const int base = c;
+      c = (c + W_CHUNK < stop) ? c + W_CHUNK : stop;
+      load_chunk(rv[k], rp[k], base + W_DEPTH * W_CHUNK);  // in flight while the next chunks resolve
+      for (;;) {
+        const uint64_t bal = __ballot(alive != 0);
+        if (bal == 0) break;
         const int L = __ffsll((unsigned long long)bal) - 1;
-        widx = (uint32_t)((wave * 64 + L) * RES_ITEMS + __builtin_amdgcn_readlane(fi, L));
-        wval = __builtin_amdgcn_readlane(fv, L);
-        if constexpr (HAS_POS) wpos = __builtin_amdgcn_readlane(fp, L);
-      }
-      if (lane == 0) {
-        slot[par][wave] = widx;
-        slot[par][SEL_WAVES + wave] = wval;
-        if constexpr (HAS_POS) slot[par][2 * SEL_WAVES + wave] = wpos;
-      }
-      lds_barrier();
-      uint32_t g = slot[par][SEL_WAVES - 1], gv = slot[par][2 * SEL_WAVES - 1];
-      uint32_t gp = HAS_POS ? slot[par][3 * SEL_WAVES - 1] : 0u;
+        const int fi = __ffs(alive) - 1;
+        uint32_t fv = vv[0], fp = HAS_POS ? pv[0] : 0u;
 #pragma unroll
-      for (int w = SEL_WAVES - 2; w >= 0; --w) {  // first non-empty wave (positions grow with the wave)
-        const uint32_t si = slot[par][w];
-        const bool ok = si != 0x7FFFFFFFu;
-        g = ok ? si : g;
-        gv = ok ? slot[par][SEL_WAVES + w] : gv;
-        if constexpr (HAS_POS) gp = ok ? slot[par][2 * SEL_WAVES + w] : gp;
-      }
-      par ^= 1;
-      if (g == 0x7FFFFFFFu) break;
-      const uint32_t pg = pq_p(gv), qg = pq_q(gv);
-      if (tid == 0) {
-        used[pg] = 1;
-        used[qg] = 1;
-      } else if (tid == 64) {
-        piv_p[acc] = (int)pg;
-        piv_q[acc] = (int)qg;
-        piv_pos[acc] = HAS_POS ? (int)gp : base + (int)g + 1;
-      }
-      ++acc;
-      if (acc >= n2) break;
-      if (bal == 0) continue;
-      const uint32_t P = pg * 0x10001u, Q = qg * 0x10001u;
-      uint32_t kill = 0;
+        for (int i = 1; i < W_ITEMS; ++i) {
+          fv = (fi == i) ? vv[i] : fv;
+          if constexpr (HAS_POS) fp = (fi == i) ? pv[i] : fp;
+        }
+        const uint32_t gv = __builtin_amdgcn_readlane(fv, L);
+        const int gpos = HAS_POS ? (int)__builtin_amdgcn_readlane(fp, L)
+                                 : base + L * W_ITEMS + __builtin_amdgcn_readlane(fi, L) + 1;
+        const uint32_t pg = pq_p(gv), qg = pq_q(gv);
+        if (lane == 0) {
+          used[pg] = 1;
+          used[qg] = 1;
+          piv_p[acc] = (int)pg;
+          piv_q[acc] = (int)qg;
+          piv_pos[acc] = gpos;
+        }
+        ++acc;
+        if (acc >= n2) break;
+        const uint32_t P = pg * 0x10001u, Q = qg * 0x10001u;
+        uint32_t kill = 0;
 #pragma unroll
-      for (int i = 0; i < RES_ITEMS; ++i) kill |= (pair_hits(vv[i], P, Q) != 0 ? 1u : 0u) << i;
-      alive &= ~kill;
+        for (int i = 0; i < W_ITEMS; ++i) kill |= (pair_hits(vv[i], P, Q) != 0 ? 1u : 0u) << i;
+        alive &= ~kill;
+      }
+      run = c < stop && acc < n2;
     }
   }
-  __syncthreads();
-  for (int i = tid; i < n; i += SEL_THREADS_RES) used_g[i] = used[i];
-  if (tid == 0) {
+  lds_barrier();
+  {
+    uint32_t *ug = (uint32_t *)used_g;
+    for (int i = lane; i < nw; i += 64) ug[i] = usedw[i];
+    if (lane < (n & 3)) used_g[4 * nw + lane] = used[4 * nw + lane];
+  }
+  if (lane == 0) {
     st->sl = acc;
     st->consumed = c;
     if (acc >= n2 || c >= len) st->sel_done = 1;
@@ -896,14 +885,15 @@ void select_launch(DeviceStep &d, hipStream_t s) {
   }
   const int ntiles = (d.nn + FILT_TILE - 1) / FILT_TILE;
   const size_t lds = (size_t)d.n + 1;
+  const size_t ldsw = (((size_t)d.n >> 2) + 1) * 4;  // the walk's flag words
   for (int r = 0; r < SEL_ROUNDS; ++r) {
     const int limit = (r == SEL_ROUNDS - 1) ? 0 : d.select_limit;
     if (r == 0)
-      hipLaunchKernelGGL(k_sel_resolve<false>, dim3(1), dim3(SEL_THREADS_RES), lds, s, d.n, d.sort.k0, d.sort.k1,
-                         d.sort.v0, d.sort.v1, d.sort_skip, d.lists, d.used, d.piv_p, d.piv_q, d.piv_pos, d.st, r,
-                         limit, d.nn);
+      hipLaunchKernelGGL(k_sel_resolve<false>, dim3(1), dim3(64), ldsw, s, d.n, d.sort.k0, d.sort.k1, d.sort.v0,
+                         d.sort.v1, d.sort_skip, d.lists, d.used, d.piv_p, d.piv_q, d.piv_pos, d.st, r, limit,
+                         d.nn);
     else
-      hipLaunchKernelGGL(k_sel_resolve<true>, dim3(1), dim3(SEL_THREADS_RES), lds, s, d.n, d.sort.k0, d.sort.k1,
+      hipLaunchKernelGGL(k_sel_resolve<true>, dim3(1), dim3(64), ldsw, s, d.n, d.sort.k0, d.sort.k1,
                          d.sort.v0, d.sort.v1, d.sort_skip, d.lists, d.used, d.piv_p, d.piv_q, d.piv_pos, d.st, r,
                          limit, d.nn);
     if (r == SEL_ROUNDS - 1) break;
