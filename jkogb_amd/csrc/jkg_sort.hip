@@ -8,11 +8,13 @@
 //   scan    -- one workgroup per digit: exclusive scan of its row of tile
 //              counts, plus the digit total; a digit holding every key marks the
 //              pass as the identity (skipped: stability makes it a no-op)
-//   scatter -- stable rank of each key inside its tile (wave ballots over the 8
-//              digit bits, then a cross-wave prefix in LDS), the tile staged in
-//              LDS in digit order, then written out in that order so that each
-//              digit's run leaves as consecutive addresses (base[digit] + tile
-//              offset + rank)
+//   scatter -- stable rank of each key inside its tile: each wave ranks its
+//              own quarter of the tile against wave-private digit counters
+//              (8 ballots per key give the lanes with the same digit; no
+//              barrier), then the per-wave counts give each wave's offset
+//              inside each digit; the tile is staged in LDS in digit order and
+//              written out in that order so that each digit's run leaves as
+//              consecutive addresses (base[digit] + tile offset + rank)
 // Buffers ping-pong; which one holds the current data is derived on the device
 // from the per-pass skip flags, so the launch sequence is fixed (graph-safe).
 // The first executed pass takes the values implicitly as the key's index.
@@ -95,16 +97,44 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scan(int *__restrict__ gh
   }
 }
 
+// Inclusive prefix sum over the 64 lanes of a wave.
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(v, o, 64);
+    if (lane >= o) v += u;
+  }
+  return v;
+}
+
+// Exclusive prefix sum of v over the 256 threads (digit = thread), total in *tot.
+__device__ __forceinline__ int block_excl_scan256(int v, int *wsum, int *tot) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int inc = wave_incl_scan(v);
+  if (lane == 63) wsum[wave] = inc;
+  __syncthreads();
+  int before = 0;
+#pragma unroll
+  for (int w = 0; w < SORT_THREADS / 64; ++w) before += (w < wave) ? wsum[w] : 0;
+  if (tot) *tot = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+  return before + inc - v;
+}
+
+// Wave w ranks the keys [1024 w, 1024 (w + 1)) of the tile, 64 at a time in lane order, against
+// counters of its own (no barrier: a wave's LDS accesses are ordered); then the per-wave counts
+// give each wave's offset inside each digit, the tile is staged in LDS in stable digit order and
+// written out in that order.
 __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter(uint64_t *__restrict__ k0, uint64_t *__restrict__ k1,
                                                                uint32_t *__restrict__ v0, uint32_t *__restrict__ v1,
                                                                const int *n_ptr, const int *__restrict__ ghist,
                                                                const int *__restrict__ dtotal, const int *skip,
                                                                int pass, int ntiles) {
-  __shared__ int gbase[RADIX];      // global output position of this tile's first key of each digit
-  __shared__ int lstart[RADIX];     // first slot of each digit in the tile staged in digit order
-  __shared__ int run[RADIX];        // running slot of each digit while staging
-  __shared__ int wcnt[4][RADIX];    // per-wave digit counts of the current round
-  __shared__ int scan_tmp[RADIX];
+  constexpr int NW = SORT_THREADS / 64;
+  // 53 KB in all, so that three workgroups share a CU
+  __shared__ int gdelta[RADIX];    // global position of a staged slot of digit d, minus the slot
+  __shared__ int wcnt[NW][RADIX];  // per-wave digit counters, then the first slot of wave w's keys of digit d
+  __shared__ int wsum[NW];
   __shared__ uint64_t skey[SORT_TILE];
   __shared__ uint32_t sval[SORT_TILE];
   const int n = *n_ptr;
@@ -119,76 +149,75 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter(uint64_t *__restr
   const int shift = pass * 8;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int tlen = (n - t * SORT_TILE < SORT_TILE) ? n - t * SORT_TILE : SORT_TILE;
+  const int wbase = wave * (SORT_TILE / NW);
 
-  // digit starts: exclusive scan of the digit totals; this tile's count of each digit from the
-  // per-tile offsets (ghist holds, per digit, the exclusive scan over the tiles)
-  {
-    const int d = threadIdx.x;
-    const int tile_off = ghist[d * ntiles + t];
-    const int tile_cnt = ((t + 1 < tiles) ? ghist[d * ntiles + t + 1] : dtotal[d]) - tile_off;
-    scan_tmp[d] = dtotal[d];
-    __syncthreads();
-    for (int off = 1; off < RADIX; off <<= 1) {
-      int v = (d >= off) ? scan_tmp[d - off] : 0;
-      __syncthreads();
-      scan_tmp[d] += v;
-      __syncthreads();
-    }
-    gbase[d] = scan_tmp[d] - dtotal[d] + tile_off;
-    __syncthreads();
-    scan_tmp[d] = tile_cnt;
-    __syncthreads();
-    for (int off = 1; off < RADIX; off <<= 1) {
-      int v = (d >= off) ? scan_tmp[d - off] : 0;
-      __syncthreads();
-      scan_tmp[d] += v;
-      __syncthreads();
-    }
-    lstart[d] = scan_tmp[d] - tile_cnt;
-    run[d] = lstart[d];
-  }
-  // stage the tile in LDS in stable digit order (16 rounds of 256 keys)
-  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  uint64_t key[SORT_ITEMS];
+  uint32_t val[SORT_ITEMS];
+#pragma unroll
   for (int r = 0; r < SORT_ITEMS; ++r) {
-    for (int i = threadIdx.x; i < 4 * RADIX; i += SORT_THREADS) (&wcnt[0][0])[i] = 0;
-    __syncthreads();
-    const int li = r * SORT_THREADS + threadIdx.x;
+    const int li = wbase + r * 64 + lane;
     const int idx = t * SORT_TILE + li;
     const bool valid = li < tlen;
-    uint64_t key = valid ? kin[idx] : 0;
-    uint32_t val = valid ? (pi.vals_iota ? (uint32_t)idx : vin[idx]) : 0;
-    int d = (int)((key >> shift) & 0xFF);
-    // lanes of this wave with the same digit
+    key[r] = valid ? kin[idx] : ~0ull;
+    val[r] = valid ? (pi.vals_iota ? (uint32_t)idx : vin[idx]) : 0u;
+  }
+#pragma unroll
+  for (int i = 0; i < RADIX / 64; ++i) wcnt[wave][lane + 64 * i] = 0;
+
+  // stable rank of each key among the wave's keys of the same digit
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  int rank[SORT_ITEMS];
+#pragma unroll
+  for (int r = 0; r < SORT_ITEMS; ++r) {
+    const bool valid = wbase + r * 64 + lane < tlen;
+    const int d = (int)((key[r] >> shift) & 0xFF);
     uint64_t match = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
-      uint64_t bal = __ballot(valid && ((d >> b) & 1));
+      const uint64_t bal = __ballot((d >> b) & 1);
       match &= ((d >> b) & 1) ? bal : ~bal;
     }
-    int wrank = __popcll(match & lt_mask);
-    if (valid && wrank == 0) wcnt[wave][d] = __popcll(match);
-    __syncthreads();
-    if (valid) {
-      int before = 0;
-      for (int w = 0; w < wave; ++w) before += wcnt[w][d];
-      const int slot = run[d] + before + wrank;
-      skey[slot] = key;
-      sval[slot] = val;
-    }
-    __syncthreads();
-    {
-      const int dd = threadIdx.x;
-      run[dd] += wcnt[0][dd] + wcnt[1][dd] + wcnt[2][dd] + wcnt[3][dd];
-    }
-    __syncthreads();
+    const int before = valid ? wcnt[wave][d] : 0;
+    const int g = __popcll(match & lt_mask);
+    rank[r] = before + g;
+    if (valid && g == 0) wcnt[wave][d] = before + __popcll(match);
   }
+  __syncthreads();
+  // digit d = thread: the waves' offsets inside the digit, the tile's digit starts, the global starts
+  {
+    const int d = threadIdx.x;
+    int c = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      const int x = wcnt[w][d];
+      wcnt[w][d] = c;
+      c += x;
+    }
+    const int ls = block_excl_scan256(c, wsum, nullptr);  // first slot of digit d in the staged tile
+#pragma unroll
+    for (int w = 0; w < NW; ++w) wcnt[w][d] += ls;
+    __syncthreads();
+    const int gs = block_excl_scan256(dtotal[d], wsum, nullptr);
+    gdelta[d] = gs + ghist[d * ntiles + t] - ls;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < SORT_ITEMS; ++r) {
+    if (wbase + r * 64 + lane < tlen) {
+      const int d = (int)((key[r] >> shift) & 0xFF);
+      const int slot = wcnt[wave][d] + rank[r];
+      skey[slot] = key[r];
+      sval[slot] = val[r];
+    }
+  }
+  __syncthreads();
   // write out in staged order: neighbouring lanes hold neighbouring slots, so each digit's
   // run of this tile goes out as consecutive addresses
   for (int i = threadIdx.x; i < tlen; i += SORT_THREADS) {
-    const uint64_t key = skey[i];
-    const int d = (int)((key >> shift) & 0xFF);
-    const int pos = gbase[d] + (i - lstart[d]);
-    kout[pos] = key;
+    const uint64_t k = skey[i];
+    const int d = (int)((k >> shift) & 0xFF);
+    const int pos = gdelta[d] + i;
+    kout[pos] = k;
     vout[pos] = sval[i];
   }
 }
