@@ -186,13 +186,13 @@ constexpr int FILT_TILE = 2048;
 // an index with it (packed 16-bit compare: a half of v ^ (x * 0x10001) is zero
 // iff that half of v equals x).  A chunk ends with an iteration that finds
 // nothing.  No barrier anywhere: the wave's own LDS accesses are ordered, so
-// the next chunk's test sees this chunk's marks, and the loads of the next
-// W_DEPTH chunks are in flight while this one resolves.  The walk stops after `limit`
+// the next chunk's test sees this chunk's marks; the loads of the next batch
+// of W_BATCH chunks are in flight while this batch resolves.  The walk stops after `limit`
 // entries, at the end of the list, or at N_2 pivots (AW_NCP2 semantics,
 // src/shared/atypes.F90:191-230).
 constexpr int W_ITEMS = 4;
 constexpr int W_CHUNK = 64 * W_ITEMS;
-constexpr int W_DEPTH = 8;  // chunks whose loads are in flight (one global round trip per chunk otherwise)
+constexpr int W_BATCH = 7;  // chunks per batch of loads: 7 x 8 loads in flight fit the 6-bit vmcnt
 
 __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
@@ -229,8 +229,12 @@ template <bool HAS_POS>
 __global__ void __launch_bounds__(64) k_sel_resolve(int n, const uint64_t *k0, const uint64_t *k1,
                                                     const uint32_t *v0, const uint32_t *v1, const int *sort_skip,
                                                     ListPtrs lp, uint8_t *used_g, int *piv_p, int *piv_q,
-                                                    int *piv_pos, StepState *st, int round, int limit, int nn) {
-  extern __shared__ uint32_t usedw[];  // n + 1 flag bytes, rounded up to words
+                                                    int *piv_pos, StepState *st, int round, int limit, int nn,
+                                                    int n2max) {
+  // n + 1 flag bytes rounded up to words, then this round's pivots (p, q, position; n2max each),
+  // written out at the end: no store to global memory inside the walk, so that the wait for a
+  // chunk's loads counts exactly the loads issued after it
+  extern __shared__ uint32_t usedw[];
   uint8_t *used = (uint8_t *)usedw;
   if (st->sel_done || st->sel_round != round) return;
   const int lane = threadIdx.x;
@@ -265,70 +269,94 @@ __global__ void __launch_bounds__(64) k_sel_resolve(int n, const uint64_t *k0, c
   }
   if (lane == 0) used[n] = 1;
   lds_barrier();
+  int *bp = (int *)(usedw + nw + 1), *bq = bp + n2max, *bpos = bq + n2max;
   const int n2 = st->n2;
-  int acc = st->sl;
+  const int acc0 = st->sl;
+  int acc = acc0;
   int c = st->consumed;
   const int stop = (limit > 0 && len - c > limit) ? c + limit : len;
-  // ring of W_DEPTH chunks in flight: chunk m of the walk (from `consumed`) is in registers rv[m % W_DEPTH]
-  uint32_t rv[W_DEPTH][W_ITEMS], rp[W_DEPTH][W_ITEMS];
+  // every lane loads (an index clamped into the list; the buffers hold at least one entry), so the
+  // number of loads in flight does not depend on the data; entries past the end are replaced by the
+  // sentinel only when their chunk is used, which is where the wait for the load belongs
   auto load_chunk = [&](uint32_t(&dv)[W_ITEMS], uint32_t(&dp)[W_ITEMS], int base) {
 #pragma unroll
     for (int i = 0; i < W_ITEMS; ++i) {
-      const int e = base + lane * W_ITEMS + i;
-      dv[i] = (e < stop) ? vals[e] : 0xFFFFFFFFu;
-      if constexpr (HAS_POS) dp[i] = (e < stop) ? poss[e] : 0u;
+      const int ec = max(min(base + lane * W_ITEMS + i, stop - 1), 0);
+      dv[i] = vals[ec];
+      if constexpr (HAS_POS) dp[i] = poss[ec];
     }
   };
-#pragma unroll
-  for (int k = 0; k < W_DEPTH; ++k) load_chunk(rv[k], rp[k], c + k * W_CHUNK);
+  // one chunk: the free-entry test against used[], then the acceptances
   bool run = c < stop && acc < n2;
+  auto walk_chunk = [&](const uint32_t(&cv)[W_ITEMS], const uint32_t(&cp)[W_ITEMS]) {
+    uint32_t vv[W_ITEMS], pv[W_ITEMS];
+    uint32_t alive = 0;
+#pragma unroll
+    for (int i = 0; i < W_ITEMS; ++i) {
+      const bool in = c + lane * W_ITEMS + i < stop;
+      vv[i] = in ? cv[i] : 0xFFFFFFFFu;
+      if constexpr (HAS_POS) pv[i] = in ? cp[i] : 0u;
+      const uint32_t p = min(vv[i] & 0xFFFFu, (uint32_t)n), q = min(vv[i] >> 16, (uint32_t)n);
+      alive |= ((used[p] | used[q]) == 0 ? 1u : 0u) << i;
+    }
+    const int base = c;
+    c = (c + W_CHUNK < stop) ? c + W_CHUNK : stop;
+    for (;;) {
+      const uint64_t bal = __ballot(alive != 0);
+      if (bal == 0) break;
+      const int L = __ffsll((unsigned long long)bal) - 1;
+      const int fi = __ffs(alive) - 1;
+      uint32_t fv = vv[0], fp = HAS_POS ? pv[0] : 0u;
+#pragma unroll
+      for (int i = 1; i < W_ITEMS; ++i) {
+        fv = (fi == i) ? vv[i] : fv;
+        if constexpr (HAS_POS) fp = (fi == i) ? pv[i] : fp;
+      }
+      const uint32_t gv = __builtin_amdgcn_readlane(fv, L);
+      const int gpos = HAS_POS ? (int)__builtin_amdgcn_readlane(fp, L)
+                               : base + L * W_ITEMS + __builtin_amdgcn_readlane(fi, L) + 1;
+      const uint32_t pg = pq_p(gv), qg = pq_q(gv);
+      if (lane == 0) {
+        used[pg] = 1;
+        used[qg] = 1;
+        bp[acc - acc0] = (int)pg;
+        bq[acc - acc0] = (int)qg;
+        bpos[acc - acc0] = gpos;
+      }
+      ++acc;
+      if (acc >= n2) break;
+      const uint32_t P = pg * 0x10001u, Q = qg * 0x10001u;
+      uint32_t kill = 0;
+#pragma unroll
+      for (int i = 0; i < W_ITEMS; ++i) kill |= (pair_hits(vv[i], P, Q) != 0 ? 1u : 0u) << i;
+      alive &= ~kill;
+    }
+    run = c < stop && acc < n2;
+  };
+  // two batches of W_BATCH chunks, ping-pong: all loads of the next batch are issued before the
+  // current one is walked, so a batch's loads have a whole batch of work to arrive in
+  uint32_t va[W_BATCH][W_ITEMS], pa[W_BATCH][W_ITEMS], vb[W_BATCH][W_ITEMS], pb[W_BATCH][W_ITEMS];
+  int nb = c;
+#pragma unroll
+  for (int k = 0; k < W_BATCH; ++k) load_chunk(va[k], pa[k], nb + k * W_CHUNK);
+  nb += W_BATCH * W_CHUNK;
   while (run) {
 #pragma unroll
-    for (int k = 0; k < W_DEPTH; ++k) {
+    for (int k = 0; k < W_BATCH; ++k) load_chunk(vb[k], pb[k], nb + k * W_CHUNK);
+    nb += W_BATCH * W_CHUNK;
+#pragma unroll
+    for (int k = 0; k < W_BATCH; ++k) {
       if (!run) break;
-      uint32_t vv[W_ITEMS], pv[W_ITEMS];
-      uint32_t alive = 0;
+      walk_chunk(va[k], pa[k]);
+    }
+    if (!run) break;
 #pragma unroll
-      for (int i = 0; i < W_ITEMS; ++i) {
-        vv[i] = rv[k][i];
-        if constexpr (HAS_POS) pv[i] = rp[k][i];
-        const uint32_t p = min(vv[i] & 0xFFFFu, (uint32_t)n), q = min(vv[i] >> 16, (uint32_t)n);
-        alive |= ((used[p] | used[q]) == 0 ? 1u : 0u) << i;
-      }
-      const int base = c;
-      c = (c + W_CHUNK < stop) ? c + W_CHUNK : stop;
-      load_chunk(rv[k], rp[k], base + W_DEPTH * W_CHUNK);  // in flight while the next chunks resolve
-      for (;;) {
-        const uint64_t bal = __ballot(alive != 0);
-        if (bal == 0) break;
-        const int L = __ffsll((unsigned long long)bal) - 1;
-        const int fi = __ffs(alive) - 1;
-        uint32_t fv = vv[0], fp = HAS_POS ? pv[0] : 0u;
+    for (int k = 0; k < W_BATCH; ++k) load_chunk(va[k], pa[k], nb + k * W_CHUNK);
+    nb += W_BATCH * W_CHUNK;
 #pragma unroll
-        for (int i = 1; i < W_ITEMS; ++i) {
-          fv = (fi == i) ? vv[i] : fv;
-          if constexpr (HAS_POS) fp = (fi == i) ? pv[i] : fp;
-        }
-        const uint32_t gv = __builtin_amdgcn_readlane(fv, L);
-        const int gpos = HAS_POS ? (int)__builtin_amdgcn_readlane(fp, L)
-                                 : base + L * W_ITEMS + __builtin_amdgcn_readlane(fi, L) + 1;
-        const uint32_t pg = pq_p(gv), qg = pq_q(gv);
-        if (lane == 0) {
-          used[pg] = 1;
-          used[qg] = 1;
-          piv_p[acc] = (int)pg;
-          piv_q[acc] = (int)qg;
-          piv_pos[acc] = gpos;
-        }
-        ++acc;
-        if (acc >= n2) break;
-        const uint32_t P = pg * 0x10001u, Q = qg * 0x10001u;
-        uint32_t kill = 0;
-#pragma unroll
-        for (int i = 0; i < W_ITEMS; ++i) kill |= (pair_hits(vv[i], P, Q) != 0 ? 1u : 0u) << i;
-        alive &= ~kill;
-      }
-      run = c < stop && acc < n2;
+    for (int k = 0; k < W_BATCH; ++k) {
+      if (!run) break;
+      walk_chunk(vb[k], pb[k]);
     }
   }
   lds_barrier();
@@ -336,6 +364,11 @@ __global__ void __launch_bounds__(64) k_sel_resolve(int n, const uint64_t *k0, c
     uint32_t *ug = (uint32_t *)used_g;
     for (int i = lane; i < nw; i += 64) ug[i] = usedw[i];
     if (lane < (n & 3)) used_g[4 * nw + lane] = used[4 * nw + lane];
+    for (int i = lane; i < acc - acc0; i += 64) {
+      piv_p[acc0 + i] = bp[i];
+      piv_q[acc0 + i] = bq[i];
+      piv_pos[acc0 + i] = bpos[i];
+    }
   }
   if (lane == 0) {
     st->sl = acc;
@@ -877,6 +910,21 @@ void weights_launch(DeviceStep &d, hipStream_t s) {
   }
 }
 
+static size_t select_lds_bytes(const DeviceStep &d) {
+  return (((size_t)d.n >> 2) + 1) * 4 + (size_t)3 * d.n2max * sizeof(int);
+}
+
+int select_prepare(DeviceStep &d) {
+  const size_t lds = select_lds_bytes(d);
+  if (lds > 160 * 1024) return -1;
+  if (hipFuncSetAttribute((const void *)k_sel_resolve<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)lds) != hipSuccess ||
+      hipFuncSetAttribute((const void *)k_sel_resolve<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)lds) != hipSuccess)
+    return -2;
+  return 0;
+}
+
 void select_launch(DeviceStep &d, hipStream_t s) {
   const int SEL_ROUNDS = d.sel_rounds > 0 ? d.sel_rounds : 2;
   if (d.ncp3) {
@@ -885,17 +933,17 @@ void select_launch(DeviceStep &d, hipStream_t s) {
   }
   const int ntiles = (d.nn + FILT_TILE - 1) / FILT_TILE;
   const size_t lds = (size_t)d.n + 1;
-  const size_t ldsw = (((size_t)d.n >> 2) + 1) * 4;  // the walk's flag words
+  const size_t ldsw = select_lds_bytes(d);
   for (int r = 0; r < SEL_ROUNDS; ++r) {
     const int limit = (r == SEL_ROUNDS - 1) ? 0 : d.select_limit;
     if (r == 0)
       hipLaunchKernelGGL(k_sel_resolve<false>, dim3(1), dim3(64), ldsw, s, d.n, d.sort.k0, d.sort.k1, d.sort.v0,
                          d.sort.v1, d.sort_skip, d.lists, d.used, d.piv_p, d.piv_q, d.piv_pos, d.st, r, limit,
-                         d.nn);
+                         d.nn, d.n2max);
     else
       hipLaunchKernelGGL(k_sel_resolve<true>, dim3(1), dim3(64), ldsw, s, d.n, d.sort.k0, d.sort.k1,
                          d.sort.v0, d.sort.v1, d.sort_skip, d.lists, d.used, d.piv_p, d.piv_q, d.piv_pos, d.st, r,
-                         limit, d.nn);
+                         limit, d.nn, d.n2max);
     if (r == SEL_ROUNDS - 1) break;
     hipLaunchKernelGGL(k_sel_filter_count, dim3(ntiles), dim3(256), lds, s, d.n, d.sort.k0, d.sort.k1, d.sort.v0,
                        d.sort.v1, d.sort_skip, d.lists, d.used, d.fcount, d.st, r);
@@ -945,6 +993,7 @@ void step_end_launch(DeviceStep &d, hipStream_t s) {
 }
 
 int step_prepare(DeviceStep &d) {
+  if (select_prepare(d) != 0) return -3;
   if (d.cplx) return zstep_prepare(d);
   const size_t lds = (size_t)2 * d.n * sizeof(double);
   if (lds > 160 * 1024) return -1;  // two columns must fit the LDS of one CU

@@ -162,6 +162,7 @@ int zstep_prepare(DeviceStep &d);
 void zinit_identity_launch(DeviceStep &d, hipStream_t s);
 void zfinalize_launch(DeviceStep &d, hipStream_t s, double *sigma);
 void select_launch(DeviceStep &d, hipStream_t s);
+int select_prepare(DeviceStep &d);  // the walk's LDS (flags and pivots of a round) for the problem size
 void pivots_launch(DeviceStep &d, hipStream_t s);
 void rotate_launch(DeviceStep &d, hipStream_t s);
 void step_end_launch(DeviceStep &d, hipStream_t s);
