@@ -64,7 +64,7 @@ struct jkg_ctx {
   double *wm_part = nullptr;
   int ncp3_n = 0;  // ID_NCP=3: per-offset totals allocated for this N
   // selection schedule: walk rounds per step (0: by size) and entries walked before each filter
-  int sel_rounds = 0, sel_limit = 32768;
+  int sel_rounds = 0, sel_limit = 49152;
   // per-step frames of |A| (jkg_set_frames): device buffer, schedule, caller's buffer and count
   uint8_t *frames_dev = nullptr, *frames_host = nullptr;
   int32_t *frames_count = nullptr;
@@ -126,13 +126,13 @@ static int dalloc(jkg_ctx *c, T **p, size_t count) {
     }                           \
   } while (0)
 
-// Walk rounds per step by list length: about one round (32768 entries walked by one
-// wave, then a whole-GPU filter) per 384K candidate pairs, 2 to 16 (measured on MI355X:
-// N=1024 best with 2; N=4096, select stage per step: 16 x 32768 1.82 ms, 12 x 49152
-// 1.91 ms, 8 x 98304 2.52 ms, 4 x 262144 4.67 ms).
+// Walk rounds per step by list length: about one round (49152 entries walked by one
+// wave, then a whole-GPU filter) per 512K candidate pairs, 2 to 12 (measured on MI355X:
+// N=1024 best with 2; N=4096, select stage per step: 12 x 49152 1.51 ms, 16 x 32768
+// 1.53 ms, 8 x 98304 1.72 ms, 4 x 262144 2.68 ms).
 static int select_rounds_for(size_t nn) {
-  const size_t r = nn / 393216 + 1;
-  return (int)(r < 2 ? 2 : (r > 16 ? 16 : r));
+  const size_t r = nn / 524288 + 1;
+  return (int)(r < 2 ? 2 : (r > 12 ? 12 : r));
 }
 
 // (Re)allocates the buffers of an N x N problem (real or complex).
@@ -158,14 +158,14 @@ static int ensure_buffers(jkg_ctx *c, int n, int cplx, int trace_cap) {
   ACHK(dalloc(c, &d.minus, n));
   ACHK(dalloc(c, &d.sort.k0, nn));
   ACHK(dalloc(c, &d.sort.k1, nn));
-  ACHK(dalloc(c, &d.sort.v0, nn));
-  ACHK(dalloc(c, &d.sort.v1, nn));
+  ACHK(dalloc(c, &d.sort.v0, nn + LIST_PAD));
+  ACHK(dalloc(c, &d.sort.v1, nn + LIST_PAD));
   ACHK(dalloc(c, (char **)&d.sort.scratch, sort_scratch_bytes((int)nn)));
   d.sort_skip = sort_skip_flags(d.sort, (int)nn);
   for (int k = 0; k < 2; ++k) {
     ACHK(dalloc(c, &d.lists.key[k], nn));
-    ACHK(dalloc(c, &d.lists.val[k], nn));
-    ACHK(dalloc(c, &d.lists.pos[k], nn));
+    ACHK(dalloc(c, &d.lists.val[k], nn + LIST_PAD));
+    ACHK(dalloc(c, &d.lists.pos[k], nn + LIST_PAD));
   }
   ACHK(dalloc(c, &d.fcount, nn / 2048 + 2));
   ACHK(dalloc(c, &d.used, n));

@@ -6,9 +6,9 @@
 //              hyperbolic pairs: the 2x2 HSVD per pair, then the off-norm
 //              change of columns p,q (ABODNDF2) with the rows staged in LDS
 //   sort    -- (b) stable LSD radix sort of (key(W), pair index)
-//   select  -- (c) exact greedy walk of the sorted list (AW_NCP2): one
-//              workgroup resolves chunks in order with wave ballots against an
-//              LDS bitmap; between rounds every CU filters the rest of the list
+//   select  -- (c) exact greedy walk of the sorted list (AW_NCP2): one wave
+//              resolves chunks in order with ballots against an LDS flag
+//              array; between rounds every CU filters the rest of the list
 //              against the bitmap (order-preserving compaction), so the walk
 //              only ever visits candidates whose indices were free
 //   pivots  -- (d) batched 2x2 HSVD of the chosen pivots, rotation descriptors
@@ -178,8 +178,8 @@ constexpr int FILT_TILE = 2048;
 // One wave walks the current list in order, W_CHUNK entries at a time
 // (W_ITEMS consecutive ones per lane).  At the start of a chunk each lane
 // tests its entries against used[] (n + 1 bytes in LDS; used[n] = 1 is the
-// sentinel that past-the-end entries, 0xFFFFFFFF, map to) and keeps a bit
-// mask of the free ones.  One iteration accepts the first free entry of the
+// sentinel that past-the-end entries, replaced by the pair (n, n), map to) and
+// keeps a bit mask of the free ones.  One iteration accepts the first free entry of the
 // chunk: a ballot finds the first lane with a free entry (lanes hold
 // increasing positions), readlane takes that entry, lane 0 marks its indices
 // in used[] and writes the pivot, and every lane clears the entries that share
@@ -187,12 +187,13 @@ constexpr int FILT_TILE = 2048;
 // iff that half of v equals x).  A chunk ends with an iteration that finds
 // nothing.  No barrier anywhere: the wave's own LDS accesses are ordered, so
 // the next chunk's test sees this chunk's marks; the loads of the next batch
-// of W_BATCH chunks are in flight while this batch resolves.  The walk stops after `limit`
-// entries, at the end of the list, or at N_2 pivots (AW_NCP2 semantics,
-// src/shared/atypes.F90:191-230).
+// of W_BATCH chunks are in flight while this batch resolves.  The walk stops
+// after `limit` entries, at the end of the list, or at N_2 pivots (AW_NCP2
+// semantics, src/shared/atypes.F90:191-230).
 constexpr int W_ITEMS = 4;
 constexpr int W_CHUNK = 64 * W_ITEMS;
-constexpr int W_BATCH = 7;  // chunks per batch of loads: 7 x 8 loads in flight fit the 6-bit vmcnt
+static_assert(W_CHUNK <= LIST_PAD, "a clamped chunk load stays inside the padded lists");
+constexpr int W_BATCH = 8;  // chunks per batch of loads (2 loads per chunk in flight; vmcnt counts 63)
 
 __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
@@ -275,30 +276,42 @@ __global__ void __launch_bounds__(64) k_sel_resolve(int n, const uint64_t *k0, c
   int acc = acc0;
   int c = st->consumed;
   const int stop = (limit > 0 && len - c > limit) ? c + limit : len;
-  // every lane loads (an index clamped into the list; the buffers hold at least one entry), so the
-  // number of loads in flight does not depend on the data; entries past the end are replaced by the
-  // sentinel only when their chunk is used, which is where the wait for the load belongs
-  auto load_chunk = [&](uint32_t(&dv)[W_ITEMS], uint32_t(&dp)[W_ITEMS], int base) {
-#pragma unroll
-    for (int i = 0; i < W_ITEMS; ++i) {
-      const int ec = max(min(base + lane * W_ITEMS + i, stop - 1), 0);
-      dv[i] = vals[ec];
-      if constexpr (HAS_POS) dp[i] = poss[ec];
-    }
+  // Chunk loads: one 16-byte buffer load per array per lane (a round starts at entry 0 of its list, so
+  // chunk bases are multiples of W_CHUNK and the loads are aligned).  The buffer range ends at the end of
+  // the list rounded up to whole chunks (the allocations hold LIST_PAD entries past their capacity), so a
+  // load never straddles it and the loads of batches past the end read zeros without touching memory:
+  // every lane loads, and the number of loads in flight does not depend on the data.  Entries past the
+  // end become the pair (n, n), dead through the sentinel used[n] = 1, when their chunk is walked.
+  typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+  static_assert(W_ITEMS == 4, "one 16-byte load per lane and array");
+  const uint32_t sent = (uint32_t)n * 0x10001u;
+  const int range = ((stop + W_CHUNK - 1) / W_CHUNK) * W_CHUNK * 4;  // bytes
+  const __amdgpu_buffer_rsrc_t rsv = __builtin_amdgcn_make_buffer_rsrc((void *)vals, (short)0, range, 0x00020000);
+  const __amdgpu_buffer_rsrc_t rsp =
+      __builtin_amdgcn_make_buffer_rsrc((void *)(HAS_POS ? poss : vals), (short)0, range, 0x00020000);
+  const int lane_off = lane * W_ITEMS * 4;
+  auto load_chunk = [&](u4 &dv, u4 &dp, int base) {
+    dv = __builtin_amdgcn_raw_buffer_load_b128(rsv, lane_off + base * 4, 0, 0);
+    if constexpr (HAS_POS) dp = __builtin_amdgcn_raw_buffer_load_b128(rsp, lane_off + base * 4, 0, 0);
   };
   // one chunk: the free-entry test against used[], then the acceptances
   bool run = c < stop && acc < n2;
-  auto walk_chunk = [&](const uint32_t(&cv)[W_ITEMS], const uint32_t(&cp)[W_ITEMS]) {
-    uint32_t vv[W_ITEMS], pv[W_ITEMS];
+  auto walk_chunk = [&](const u4 &cv, const u4 &cp) {
+    uint32_t vv[W_ITEMS] = {cv.x, cv.y, cv.z, cv.w};
+    uint32_t pv[W_ITEMS] = {0u, 0u, 0u, 0u};
+    if constexpr (HAS_POS) {
+      pv[0] = cp.x;
+      pv[1] = cp.y;
+      pv[2] = cp.z;
+      pv[3] = cp.w;
+    }
+    if (c + W_CHUNK > stop) {  // the last chunk of the walk
+#pragma unroll
+      for (int i = 0; i < W_ITEMS; ++i) vv[i] = (c + lane * W_ITEMS + i < stop) ? vv[i] : sent;
+    }
     uint32_t alive = 0;
 #pragma unroll
-    for (int i = 0; i < W_ITEMS; ++i) {
-      const bool in = c + lane * W_ITEMS + i < stop;
-      vv[i] = in ? cv[i] : 0xFFFFFFFFu;
-      if constexpr (HAS_POS) pv[i] = in ? cp[i] : 0u;
-      const uint32_t p = min(vv[i] & 0xFFFFu, (uint32_t)n), q = min(vv[i] >> 16, (uint32_t)n);
-      alive |= ((used[p] | used[q]) == 0 ? 1u : 0u) << i;
-    }
+    for (int i = 0; i < W_ITEMS; ++i) alive |= ((used[vv[i] & 0xFFFFu] | used[vv[i] >> 16]) == 0 ? 1u : 0u) << i;
     const int base = c;
     c = (c + W_CHUNK < stop) ? c + W_CHUNK : stop;
     for (;;) {
@@ -306,7 +319,7 @@ __global__ void __launch_bounds__(64) k_sel_resolve(int n, const uint64_t *k0, c
       if (bal == 0) break;
       const int L = __ffsll((unsigned long long)bal) - 1;
       const int fi = __ffs(alive) - 1;
-      uint32_t fv = vv[0], fp = HAS_POS ? pv[0] : 0u;
+      uint32_t fv = vv[0], fp = pv[0];
 #pragma unroll
       for (int i = 1; i < W_ITEMS; ++i) {
         fv = (fi == i) ? vv[i] : fv;
@@ -335,7 +348,7 @@ __global__ void __launch_bounds__(64) k_sel_resolve(int n, const uint64_t *k0, c
   };
   // two batches of W_BATCH chunks, ping-pong: all loads of the next batch are issued before the
   // current one is walked, so a batch's loads have a whole batch of work to arrive in
-  uint32_t va[W_BATCH][W_ITEMS], pa[W_BATCH][W_ITEMS], vb[W_BATCH][W_ITEMS], pb[W_BATCH][W_ITEMS];
+  u4 va[W_BATCH], pa[W_BATCH], vb[W_BATCH], pb[W_BATCH];
   int nb = c;
 #pragma unroll
   for (int k = 0; k < W_BATCH; ++k) load_chunk(va[k], pa[k], nb + k * W_CHUNK);

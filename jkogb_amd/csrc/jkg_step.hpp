@@ -10,6 +10,10 @@
 
 namespace jkg {
 
+// Entries allocated past the capacity of the sort's value buffers and of the
+// candidate lists: the greedy walk loads whole chunks of 256 entries.
+constexpr int LIST_PAD = 256;
+
 // Candidate lists of the selection rounds (ping-pong), passed by value.
 struct ListPtrs {
   uint64_t *key[2];
