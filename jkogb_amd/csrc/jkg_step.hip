@@ -59,24 +59,53 @@ __global__ void k_step_begin(StepState *st, int nn, int n, int *colpiv) {
 // tol > 0 (mixed solver only): a pair whose transform would be the identity
 // if its off-diagonal pair were zero and whose off-diagonal pair is negligible
 // (see negligible()) is dropped like the reference drops a zero pair.
+// One workgroup per 32 x 32 tile of pairs (p in [P0, P0 + 32), q in [Q0, Q0 +
+// 32), p < q): the blocks A(q, p) and A(p, q) are staged in LDS with coalesced
+// column loads, and the pairs are handed to the lanes along the tile's
+// diagonals (q - p fixed within 32 lanes), where consecutive lanes have
+// consecutive pair indices, so the keys and values leave as contiguous runs.
+constexpr int TWT = 32;
 template <bool F>
 __global__ void __launch_bounds__(256) k_trig_weights(int n, const double *__restrict__ A, int lda,
                                                        const int *__restrict__ J, uint64_t *__restrict__ keys,
-                                                       uint32_t *__restrict__ vals, const StepState *st, int nn,
+                                                       uint32_t *__restrict__ vals, const StepState *st,
                                                        double tol) {
-  if (!st->active) return;
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < (uint32_t)nn; k += gridDim.x * blockDim.x) {
-    int p, q;
-    pair_of(n, k, p, q);
-    int jp = J[p - 1], jq = J[q - 1];
-    if (jp != jq) continue;  // hyperbolic: k_hyp_weights
-    const size_t P = p - 1, Q = q - 1;
-    double aqp = A[P * lda + Q], apq = A[Q * lda + P], app = A[P * lda + P], aqq = A[Q * lda + Q];
-    const bool order = sign_neg(app) || sign_neg(aqq) || ((jp + jq == -2) ? (aqq < app) : (app < aqq));
+  const int bp = blockIdx.y, bq = blockIdx.x;
+  if (bp > bq || !st->active) return;
+  __shared__ double X[TWT][TWT + 1];  // X[qi][pi] = A(Q0 + qi, P0 + pi)
+  __shared__ double Y[TWT][TWT + 1];  // Y[pi][qi] = A(P0 + pi, Q0 + qi)
+  __shared__ double dgp[TWT], dgq[TWT];
+  __shared__ int jp[TWT], jq[TWT];
+  const int P0 = bp * TWT, Q0 = bq * TWT, t = threadIdx.x;
+  for (int e = t; e < TWT * TWT; e += 256) {
+    const int r = e % TWT, c = e / TWT;
+    X[r][c] = (P0 + c < n && Q0 + r < n) ? A[(size_t)(P0 + c) * lda + Q0 + r] : 0.0;
+    Y[r][c] = (P0 + r < n && Q0 + c < n) ? A[(size_t)(Q0 + c) * lda + P0 + r] : 0.0;
+  }
+  if (t < TWT) {
+    const int P = P0 + t;
+    dgp[t] = (P < n) ? A[(size_t)P * lda + P] : 0.0;
+    jp[t] = (P < n) ? J[P] : 0;
+  } else if (t < 2 * TWT) {
+    const int Q = Q0 + t - TWT;
+    dgq[t - TWT] = (Q < n) ? A[(size_t)Q * lda + Q] : 0.0;
+    jq[t - TWT] = (Q < n) ? J[Q] : 0;
+  }
+  __syncthreads();
+  // slot s: diagonal dl = s / 32 - 31 (q - Q0 = p - P0 + dl), a = p - P0 = s % 32
+  for (int sl = t; sl < (2 * TWT - 1) * TWT; sl += 256) {
+    const int a = sl % TWT, qi = a + sl / TWT - (TWT - 1);
+    if (qi < 0 || qi >= TWT) continue;
+    const int P = P0 + a, Q = Q0 + qi;
+    if (P >= Q || Q >= n || jp[a] != jq[qi]) continue;  // hyperbolic pairs: k_hyp_weights
+    const int jj = jp[a];
+    const double aqp = X[qi][a], apq = Y[a][qi], app = dgp[a], aqq = dgq[qi];
+    const bool order = sign_neg(app) || sign_neg(aqq) || ((jj + jj == -2) ? (aqq < app) : (app < aqq));
     bool go = (aqp != 0.0) || (apq != 0.0) || order;
     if (tol > 0.0 && go && !order && negligible(aqp, apq, app, aqq, tol)) go = false;
+    const uint32_t k = pair_index(n, P + 1, Q + 1);
     keys[k] = go ? weight_key(dasum2<F>(aqp, apq)) : KEY_NAN;
-    vals[k] = pack_pq(p - 1, q - 1);
+    vals[k] = pack_pq(P, Q);
   }
 }
 
@@ -901,12 +930,13 @@ void weights_launch(DeviceStep &d, hipStream_t s) {
     return;
   }
   const double *A = (const double *)d.A;
+  const int nt = (d.n + TWT - 1) / TWT;
   if (d.fused) {
-    hipLaunchKernelGGL(k_trig_weights<true>, dim3(grid_for(d.nn, 256, 65536)), dim3(256), 0, s, d.n, A, d.lda, d.J,
-                       d.sort.k0, d.sort.v0, d.st, d.nn, d.wmode ? d.neg_tol : 0.0);
+    hipLaunchKernelGGL(k_trig_weights<true>, dim3(nt, nt), dim3(256), 0, s, d.n, A, d.lda, d.J, d.sort.k0,
+                       d.sort.v0, d.st, d.wmode ? d.neg_tol : 0.0);
   } else {
-    hipLaunchKernelGGL(k_trig_weights<false>, dim3(grid_for(d.nn, 256, 65536)), dim3(256), 0, s, d.n, A, d.lda, d.J,
-                       d.sort.k0, d.sort.v0, d.st, d.nn, d.wmode ? d.neg_tol : 0.0);
+    hipLaunchKernelGGL(k_trig_weights<false>, dim3(nt, nt), dim3(256), 0, s, d.n, A, d.lda, d.J, d.sort.k0,
+                       d.sort.v0, d.st, d.wmode ? d.neg_tol : 0.0);
   }
   if (d.wmode == 1) {
     gram_hyp_weights_launch(d, s);
