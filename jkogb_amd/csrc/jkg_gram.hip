@@ -63,13 +63,49 @@ __global__ void __launch_bounds__(256) k_gemm_tn(int k, int kchunk, long plane, 
   const T *xp = X + (size_t)(xcol < 0 ? 0 : xcol) * ldx;
   const T *yp = Y + (size_t)(ycol < 0 ? 0 : ycol) * ldy;
   double xr[8], yr[8];
+  // 16-byte column starts (even leading dimension): a lane's 8 rows are four 16-byte loads
+  // (rows kb + r0 + lh are a multiple of 8), issued unconditionally from the clamped column; the
+  // entries to zero (no column, the diagonal of O) are cleared when the stage is written to LDS,
+  // after the products, so nothing waits for the loads before then.  The last group of a chunk
+  // that ends inside it is loaded row by row.
+  const bool vec = sizeof(T) == 8 && (ldx & 1) == 0 && (ldy & 1) == 0;
+  bool raw = false;  // xr, yr hold unmasked vector loads
   auto gload = [&](int r0) {
+    const int rb = kb + r0 + lh;
+    raw = vec && rb + 8 <= ke;
+    if (raw) {
+      typedef double d2v __attribute__((ext_vector_type(2)));
+      const d2v *px = (const d2v *)(xp + rb), *py = (const d2v *)(yp + rb);
+#pragma unroll
+      for (int h = 0; h < 4; ++h) {
+        const d2v a = px[h], b = py[h];
+        xr[2 * h] = a.x;
+        xr[2 * h + 1] = a.y;
+        yr[2 * h] = b.x;
+        yr[2 * h + 1] = b.y;
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int r = rb + u;
+        const int rd = CX ? (r >> 1) : r;
+        xr[u] = (xcol >= 0 && r < ke && !(ZD && rd == xcol)) ? (double)xp[r] : 0.0;
+        yr[u] = (ycol >= 0 && r < ke && !(ZD && rd == ycol)) ? (double)yp[r] : 0.0;
+      }
+    }
+  };
+  auto sstore = [&](int buf, int r0) {
+    const int rb = kb + r0 + lh;
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
-      const int r = kb + r0 + lh + u;
-      const int rd = CX ? (r >> 1) : r;
-      xr[u] = (xcol >= 0 && r < ke && !(ZD && rd == xcol)) ? (double)xp[r] : 0.0;
-      yr[u] = (ycol >= 0 && r < ke && !(ZD && rd == ycol)) ? (double)yp[r] : 0.0;
+      const int rd = CX ? ((rb + u) >> 1) : rb + u;
+      double x = xr[u], y = yr[u];
+      if (raw) {
+        x = (xcol < 0 || (ZD && rd == xcol)) ? 0.0 : x;
+        y = (ycol < 0 || (ZD && rd == ycol)) ? 0.0 : y;
+      }
+      yr[u] = y;
+      Xs[buf][lh + u][lc] = x;
     }
     if (CX == 2) {  // rows r0 + lh + u are even for even u: (re, im) -> (im, -re)
 #pragma unroll
@@ -79,13 +115,8 @@ __global__ void __launch_bounds__(256) k_gemm_tn(int k, int kchunk, long plane, 
         yr[u + 1] = -re;
       }
     }
-  };
-  auto sstore = [&](int buf) {
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      Xs[buf][lh + u][lc] = xr[u];
-      Ys[buf][lh + u][lc] = yr[u];
-    }
+    for (int u = 0; u < 8; ++u) Ys[buf][lh + u][lc] = yr[u];
   };
   d4 acc[4][4];
 #pragma unroll
@@ -93,7 +124,7 @@ __global__ void __launch_bounds__(256) k_gemm_tn(int k, int kchunk, long plane, 
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[a][b] = (d4){0.0, 0.0, 0.0, 0.0};
   gload(0);
-  sstore(0);
+  sstore(0, 0);
   __syncthreads();
   const int nst = (ke - kb + GB_K - 1) / GB_K;
   for (int s = 0; s < nst; ++s) {
@@ -111,7 +142,7 @@ __global__ void __launch_bounds__(256) k_gemm_tn(int k, int kchunk, long plane, 
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[a], bf[b], acc[a][b], 0, 0, 0);
     }
-    if (s + 1 < nst) sstore(buf ^ 1);
+    if (s + 1 < nst) sstore(buf ^ 1, (s + 1) * GB_K);
     __syncthreads();
   }
 #pragma unroll
