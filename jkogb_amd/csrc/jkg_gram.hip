@@ -219,14 +219,50 @@ __global__ void __launch_bounds__(256) k_zgram_f32(int k, int kchunk, long plane
   const float *xp = X + (size_t)(xcol < 0 ? 0 : xcol) * ldx;
   const float *yp = X + (size_t)(ycol < 0 ? 0 : ycol) * ldx;
   float xr[8], yr[8];
+  // as in k_gemm_tn: two unconditional 16-byte loads per lane when the column starts are 16-byte
+  // aligned (ldx a multiple of 4) and the 8 rows lie inside the chunk, the diagonal pair and absent
+  // columns zeroed when the stage is written to LDS, after the products
+  const bool vec = (ldx & 3) == 0;
+  bool raw = false;
   auto gload = [&](int r0) {
+    const int rb = kb + r0 + lh;
+    raw = vec && rb + 8 <= ke;
+    if (raw) {
+      const f4v *px = (const f4v *)(xp + rb), *py = (const f4v *)(yp + rb);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const f4v a = px[h], b = py[h];
+        xr[4 * h] = a.x;
+        xr[4 * h + 1] = a.y;
+        xr[4 * h + 2] = a.z;
+        xr[4 * h + 3] = a.w;
+        yr[4 * h] = b.x;
+        yr[4 * h + 1] = b.y;
+        yr[4 * h + 2] = b.z;
+        yr[4 * h + 3] = b.w;
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int r = rb + u;
+        xr[u] = (xcol >= 0 && r < ke && (r >> 1) != xcol) ? xp[r] : 0.0f;
+        yr[u] = (ycol >= 0 && r < ke && (r >> 1) != ycol) ? yp[r] : 0.0f;
+      }
+    }
+  };
+  auto sstore = [&](int buf, int r0) {
+    const int rb = kb + r0 + lh;
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
-      const int r = kb + r0 + lh + u;
-      xr[u] = (xcol >= 0 && r < ke && (r >> 1) != xcol) ? xp[r] : 0.0f;
-      yr[u] = (ycol >= 0 && r < ke && (r >> 1) != ycol) ? yp[r] : 0.0f;
+      float x = xr[u], y = yr[u];
+      if (raw) {
+        x = (xcol < 0 || ((rb + u) >> 1) == xcol) ? 0.0f : x;
+        y = (ycol < 0 || ((rb + u) >> 1) == ycol) ? 0.0f : y;
+      }
+      yr[u] = y;
+      Xs[buf][lh + u][lc] = x;
     }
-    if (CX == 2) {
+    if (CX == 2) {  // rows rb + u are even for even u: (re, im) -> (im, -re)
 #pragma unroll
       for (int u = 0; u < 8; u += 2) {
         const float re = yr[u];
@@ -234,13 +270,8 @@ __global__ void __launch_bounds__(256) k_zgram_f32(int k, int kchunk, long plane
         yr[u + 1] = -re;
       }
     }
-  };
-  auto sstore = [&](int buf) {
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      Xs[buf][lh + u][lc] = xr[u];
-      Ys[buf][lh + u][lc] = yr[u];
-    }
+    for (int u = 0; u < 8; ++u) Ys[buf][lh + u][lc] = yr[u];
   };
   f4v acc[4][4];
 #pragma unroll
@@ -248,7 +279,7 @@ __global__ void __launch_bounds__(256) k_zgram_f32(int k, int kchunk, long plane
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[a][b] = (f4v){0.f, 0.f, 0.f, 0.f};
   gload(0);
-  sstore(0);
+  sstore(0, 0);
   __syncthreads();
   const int nst = (ke - kb + GB_K - 1) / GB_K;
   for (int s = 0; s < nst; ++s) {
@@ -266,7 +297,7 @@ __global__ void __launch_bounds__(256) k_zgram_f32(int k, int kchunk, long plane
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[a], bf[b], acc[a][b], 0, 0, 0);
     }
-    if (s + 1 < nst) sstore(buf ^ 1);
+    if (s + 1 < nst) sstore(buf ^ 1, (s + 1) * GB_K);
     __syncthreads();
   }
 #pragma unroll
