@@ -128,10 +128,37 @@ def test_build_bit_exact_vs_oracle(ctx, orc, cfg, n, seed, n2):
 
 
 def test_build_deep_walk_uses_filter_rounds(ctx, orc):
-    """N = 700: NN = 244650 candidates, more than one walk round (4N = 2800
+    """N = 700: NN = 244650 candidates, more than one walk round (49152
     entries), so the filter/compaction rounds decide the tail of the walk."""
     y, j = inputs("c2", 700, 9)
     assert check_build(ctx, orc, y, j, 350) == 350
+
+
+@pytest.mark.parametrize("rounds,limit", [(2, 2048), (5, 2049), (16, 3001), (3, 100000)])
+def test_build_any_walk_schedule(ctx, orc, rounds, limit):
+    """The pivots do not depend on the walk's schedule: rounds that stop after
+    `limit` entries (not a multiple of the walk's 256-entry chunks), each
+    followed by a filter pass, and a last unlimited round give the oracle's
+    selection."""
+    y, j = inputs("c2", 700, 9)
+    try:
+        ctx.set_select(rounds, limit)
+        assert check_build(ctx, orc, y, j, 350) == 350
+    finally:
+        ctx.set_select()
+
+
+def test_solve_same_under_any_walk_schedule(ctx):
+    y, j = inputs("c5", 300, 31)
+    base = ctx.dsolve(y, j, 150, max_steps=40)
+    try:
+        ctx.set_select(7, 2051)
+        other = ctx.dsolve(y, j, 150, max_steps=40)
+    finally:
+        ctx.set_select()
+    for key in ("SY", "YU", "ZZ"):
+        assert same_bits(base[key], other[key]), key
+    assert base["info"] == other["info"]
 
 
 def test_build_vs_reference_debug_dumps(ctx):
