@@ -207,22 +207,22 @@ constexpr int FILT_TILE = 2048;
 // One wave walks the current list in order, W_CHUNK entries at a time
 // (W_ITEMS consecutive ones per lane).  At the start of a chunk each lane
 // tests its entries against used[] (n + 1 bytes in LDS; used[n] = 1 is the
-// sentinel that past-the-end entries, replaced by the pair (n, n), map to) and
-// keeps a bit mask of the free ones.  One iteration accepts the first free entry of the
-// chunk: a ballot finds the first lane with a free entry (lanes hold
-// increasing positions), readlane takes that entry, lane 0 marks its indices
-// in used[] and writes the pivot, and every lane clears the entries that share
-// an index with it (packed 16-bit compare: a half of v ^ (x * 0x10001) is zero
-// iff that half of v equals x).  A chunk ends with an iteration that finds
-// nothing.  No barrier anywhere: the wave's own LDS accesses are ordered, so
-// the next chunk's test sees this chunk's marks; the loads of the next batch
-// of W_BATCH chunks are in flight while this batch resolves.  The walk stops
-// after `limit` entries, at the end of the list, or at N_2 pivots (AW_NCP2
-// semantics, src/shared/atypes.F90:191-230).
+// sentinel that past-the-end entries, replaced by the pair (n, n), map to)
+// and keeps a bit mask of the free ones.  One iteration accepts the first free
+// entry of the chunk: a ballot finds the first lane with a free entry (lanes
+// hold increasing positions), readlane takes that entry, lane 0 marks its
+// indices in used[] and records the pivot in LDS, and every lane clears the
+// entries that share an index with it (packed 16-bit compare: a half of
+// v ^ (x * 0x10001) is zero iff that half of v equals x).  A chunk ends with an
+// iteration that finds nothing.  No barrier anywhere: the wave's own LDS
+// accesses are ordered, so the next chunk's test sees this chunk's marks; the
+// loads of the next batch of W_BATCH chunks are in flight while this batch
+// resolves.  The walk stops after `limit` entries, at the end of the list, or
+// at N_2 pivots (AW_NCP2 semantics, src/shared/atypes.F90:191-230).
 constexpr int W_ITEMS = 4;
 constexpr int W_CHUNK = 64 * W_ITEMS;
-static_assert(W_CHUNK <= LIST_PAD, "a clamped chunk load stays inside the padded lists");
-constexpr int W_BATCH = 8;  // chunks per batch of loads (2 loads per chunk in flight; vmcnt counts 63)
+static_assert(W_CHUNK <= LIST_PAD, "the last chunk of a list stays inside the padded allocation");
+constexpr int W_BATCH = 8;  // chunks per batch of loads: 2 x 8 loads, two batches in flight (vmcnt counts 63)
 
 __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
