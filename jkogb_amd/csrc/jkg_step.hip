@@ -14,7 +14,9 @@
 //   pivots  -- (d) batched 2x2 HSVD of the chosen pivots, rotation descriptors
 //   rotate  -- (e) one workgroup per column pair: both columns of A staged in
 //              LDS, all row transforms (BA), then the column transform (AB)
-//              and the new diagonal; columns of U and Z updated in place
+//              and the new diagonal; columns of U and Z updated in place.
+//              1024 lanes per workgroup: the LDS of two columns leaves two
+//              workgroups per CU, and 32 waves keep more loads in flight
 //   end     -- NEWLEN, NORMF2D, TRIG, HYP trace line; loop control  (1 WG)
 //
 // Everything follows the reference's operation order, so with identical
@@ -598,7 +600,7 @@ __global__ void __launch_bounds__(256) k_pivots(int n, const double *__restrict_
 
 // ------------------------------------------------------------------ (e) rotate
 template <bool F>
-__global__ void __launch_bounds__(512) k_rotate(int n, double *__restrict__ A, int lda, double *__restrict__ U,
+__global__ void __launch_bounds__(1024) k_rotate(int n, double *__restrict__ A, int lda, double *__restrict__ U,
                                                 double *__restrict__ Z, const int *__restrict__ piv_p,
                                                 const int *__restrict__ piv_q, const int *__restrict__ piv_info,
                                                 const double *__restrict__ piv_d, const Rot *__restrict__ rot_row,
@@ -1022,10 +1024,10 @@ void rotate_launch(DeviceStep &d, hipStream_t s) {
   }
   const size_t lds = (size_t)2 * d.n * sizeof(double);
   if (d.fused)
-    hipLaunchKernelGGL(k_rotate<true>, dim3(d.n), dim3(512), lds, s, d.n, (double *)d.A, d.lda, (double *)d.U,
+    hipLaunchKernelGGL(k_rotate<true>, dim3(d.n), dim3(1024), lds, s, d.n, (double *)d.A, d.lda, (double *)d.U,
                        (double *)d.Z, d.piv_p, d.piv_q, d.piv_info, d.piv_d, d.rot_row, d.rot_col, d.colpiv, d.st);
   else
-    hipLaunchKernelGGL(k_rotate<false>, dim3(d.n), dim3(512), lds, s, d.n, (double *)d.A, d.lda, (double *)d.U,
+    hipLaunchKernelGGL(k_rotate<false>, dim3(d.n), dim3(1024), lds, s, d.n, (double *)d.A, d.lda, (double *)d.U,
                        (double *)d.Z, d.piv_p, d.piv_q, d.piv_info, d.piv_d, d.rot_row, d.rot_col, d.colpiv, d.st);
 }
 
