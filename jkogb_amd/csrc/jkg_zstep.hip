@@ -272,7 +272,7 @@ __global__ void __launch_bounds__(256) k_zpivots(int n, const E *__restrict__ A,
 
 // ---------------------------------------------------------------- rotate
 template <bool F, class E>
-__global__ void __launch_bounds__(512) k_zrotate(int n, E *__restrict__ A, int lda, E *__restrict__ U,
+__global__ void __launch_bounds__(1024) k_zrotate(int n, E *__restrict__ A, int lda, E *__restrict__ U,
                                                  E *__restrict__ Z, const int *__restrict__ piv_p,
                                                  const int *__restrict__ piv_q, const int *__restrict__ piv_info,
                                                  const double *__restrict__ piv_d, const ZRot *__restrict__ zrot,
@@ -454,7 +454,7 @@ void zpivots_launch(DeviceStep &d, hipStream_t s) {
 
 void zrotate_launch(DeviceStep &d, hipStream_t s) {
   auto go = [&](auto kern, auto *A, auto *U, auto *Z, size_t esz) {
-    hipLaunchKernelGGL(kern, dim3(d.n), dim3(512), 2 * d.n * esz, s, d.n, A, d.lda, U, Z, d.piv_p, d.piv_q,
+    hipLaunchKernelGGL(kern, dim3(d.n), dim3(1024), 2 * d.n * esz, s, d.n, A, d.lda, U, Z, d.piv_p, d.piv_q,
                        d.piv_info, d.piv_d, (const ZRot *)d.zrot, d.colpiv, d.st);
   };
   if (d.prec32) {
