@@ -314,8 +314,8 @@ def main():
                          "the counters' calibration) and two profiled steps of each solver at N=4096")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only (the step is single-GPU; see DESIGN.md for how it "
-              "would shard); --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        print("bench.py: the step runs on one GPU (its sharding over several GPUs is a design, not built); "
+              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     from jkogb_amd import inputs, jkg

@@ -120,8 +120,8 @@ def test_controlled_solve_accuracy(ctx, cfg, n):
 def test_trained_controller_gain(ctx):
     """What the trained network (jkogb_amd/ctl_params.npy) gains over the mixed solver without it, on an
     input of the learned-controller family that training and validation did not see (seed 7, N=512);
-    the plain full-sort ordering judges: equal accuracy (within the spread of the reference's two FMA
-    builds + 64 N eps) and fewer steps in less device time."""
+    measured against the plain full-sort ordering: equal accuracy (within the spread of the reference's
+    two FMA builds + 64 N eps) and fewer steps in less device time."""
     import os
     from jkogb_amd import ctl_train
     p = np.load(os.path.join(os.path.dirname(ctl_train.__file__), "ctl_params.npy"))
