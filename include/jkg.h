@@ -29,6 +29,14 @@ extern "C" {
 
 typedef struct jkg_ctx jkg_ctx;
 
+/* Size limits of the device step (JKG_ENOTSUP beyond them).  The rotation kernel
+ * stages the two columns of A of a pivot in LDS (160 KiB per workgroup on gfx950):
+ * N <= 10240 for real and N <= 5120 for complex problems.  Pair values are two
+ * 16-bit indices.  ID_NCP=3 keeps a used-index map per wave: N <= 8192. */
+#define JKG_NMAX_REAL 10240
+#define JKG_NMAX_COMPLEX 5120
+#define JKG_NMAX_NCP3 8192
+
 /* The AW candidate record of the reference (src/shared/atypes.F90:18-24):
  * weight, row P < column Q (1-based), band B = Q - P, info. 24 bytes. */
 typedef struct {
@@ -295,7 +303,7 @@ int jkg_dgen(jkg_ctx *ctx, int n, const double *lam, int rounds, double tmax, ui
 typedef struct {
   int32_t fused;     /* FMAD policy: 0 unfused (the reference's default build), 1 fma */
   int32_t max_steps; /* < 0: to convergence; else at most this many steps per instance */
-  int32_t grid;      /* workgroups of the persistent launch; 0: tuned */
+  int32_t grid;      /* workgroups of the persistent launch, 1..JKG_BATCH_GRID_MAX; 0: tuned */
   int32_t variant;   /* kernel variant 0..3; -1: tuned */
   double tol;        /* 0: djk's termination (bit-exact per instance); > 0: the mixed solver's
                         convergence test (see jkg_mixed_opts.tol), results agree to rounding */
@@ -307,6 +315,7 @@ typedef struct {
   double seconds;      /* device time of the step loop */
   double tune_seconds; /* host time spent tuning */
 } jkg_batch_opts;
+#define JKG_BATCH_GRID_MAX 4096
 void jkg_batch_opts_default(jkg_batch_opts *o);
 int jkg_dsolve_batched(jkg_ctx *ctx, int count, const int32_t *ns, const int32_t *n2s, const double *y,
                        const int64_t *j, double *yu, double *zz, double *sy, int32_t *info, jkg_batch_opts *opts);

@@ -42,17 +42,12 @@ struct jkg_ctx {
   // captured steps: [0] FP64, [1] the FP32 warm start (each for the options it was captured with)
   hipGraph_t graph[2] = {nullptr, nullptr};
   hipGraphExec_t exec[2] = {nullptr, nullptr};
-  // host values the captured graph holds as launch shapes or kernel arguments
+  // What a captured graph was captured from: every launch shape and kernel argument of the step
+  // comes from the DeviceStep (sizes, options and every device pointer, such as the frame buffer of
+  // jkg_set_frames), so a graph is reused only while the DeviceStep is byte for byte the same.
   struct GraphKey {
-    int fused = -1, wmode = -1, npl = -1, ctl_mode = -1, ctl_cap = -1, cut32 = -1, ncp3 = -1, sel_rounds = -1,
-        sel_limit = -1, cyclic = -1, frame_every = -1, frame_size = -1, frame_max = -1;
-    double neg_tol = -1.0, switch_tol = -1.0;
-    bool operator==(const GraphKey &o) const {
-      return fused == o.fused && wmode == o.wmode && npl == o.npl && neg_tol == o.neg_tol &&
-             ctl_mode == o.ctl_mode && ctl_cap == o.ctl_cap && cut32 == o.cut32 && switch_tol == o.switch_tol &&
-             ncp3 == o.ncp3 && sel_rounds == o.sel_rounds && sel_limit == o.sel_limit && cyclic == o.cyclic &&
-             frame_every == o.frame_every && frame_size == o.frame_size && frame_max == o.frame_max;
-    }
+    bool valid = false;
+    DeviceStep d{};
   } gkey[2];
   StepState st_init{};  // the loop state of the current problem before its first step
   // mixed solver buffers (allocated on first use)
@@ -470,33 +465,20 @@ static int upload_problem(jkg_ctx *c, int n, int n2, const void *y, const int64_
 // Captures (or reuses) the step graph of the current precision: slot 1 while d.prec32.
 static int capture_step(jkg_ctx *c, int fused) {
   const int slot = c->d.prec32 ? 1 : 0;
-  jkg_ctx::GraphKey key;
-  key.fused = fused;
-  key.wmode = c->d.wmode;
-  key.npl = c->d.npl;  // grids and arguments follow this problem's J split (npl, nmi)
-  key.neg_tol = c->d.neg_tol;
-  key.ctl_mode = c->d.ctl_mode;
-  key.ctl_cap = c->d.ctl_trace_cap;
-  key.cut32 = slot ? c->d.cut32 : 0;
-  key.ncp3 = c->d.ncp3;
-  key.sel_rounds = c->d.sel_rounds;
-  key.sel_limit = c->d.select_limit;
-  key.cyclic = c->d.cyclic;
-  key.frame_every = c->d.frames ? c->d.frame_every : 0;
-  key.frame_size = c->d.frames ? c->d.frame_size : 0;
-  key.frame_max = c->d.frames ? c->d.frame_max : 0;
-  key.switch_tol = slot ? c->d.switch_tol : 0.0;
-  if (c->exec[slot] && c->gkey[slot] == key) return JKG_OK;
+  c->d.fused = fused;
+  if (c->exec[slot] && c->gkey[slot].valid && memcmp(&c->gkey[slot].d, &c->d, sizeof(DeviceStep)) == 0)
+    return JKG_OK;
   if (c->exec[slot]) (void)hipGraphExecDestroy(c->exec[slot]);
   if (c->graph[slot]) (void)hipGraphDestroy(c->graph[slot]);
   c->exec[slot] = nullptr;
   c->graph[slot] = nullptr;
-  c->d.fused = fused;
+  c->gkey[slot].valid = false;
   HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   step_launch(c->d, c->stream);
   HIPCHK(hipStreamEndCapture(c->stream, &c->graph[slot]));
   HIPCHK(hipGraphInstantiate(&c->exec[slot], c->graph[slot], nullptr, nullptr, 0));
-  c->gkey[slot] = key;
+  memcpy(&c->gkey[slot].d, &c->d, sizeof(DeviceStep));
+  c->gkey[slot].valid = true;
   return JKG_OK;
 }
 
@@ -1729,7 +1711,9 @@ extern "C" int jkg_dsolve_batched(jkg_ctx *c, int count, const int32_t *ns, cons
   jkg_batch_opts o;
   if (opts) o = *opts;
   else jkg_batch_opts_default(&o);
-  if (o.grid < 0 || o.variant < -1 || o.variant >= batch_variants() || !(o.tol >= 0.0)) return -11;
+  if (o.grid < 0 || o.grid > JKG_BATCH_GRID_MAX || o.variant < -1 || o.variant >= batch_variants() ||
+      !(o.tol >= 0.0))
+    return -11;
   HIPCHK(hipSetDevice(c->device));
   // layout of the pools
   std::vector<BInst> inst(count);
@@ -1812,7 +1796,7 @@ extern "C" int jkg_dsolve_batched(jkg_ctx *c, int count, const int32_t *ns, cons
   BInst *dinst;
   int rc = JKG_OK;
   int *dsched, *doff;
-  const int maxgrid = 4096;
+  const int maxgrid = JKG_BATCH_GRID_MAX;
   if ((rc = m.get(&dinst, count)) || (rc = m.get(&a.state, count)) || (rc = m.get(&a.mat, mat)) ||
       (rc = m.get(&a.dbl, dbl)) || (rc = m.get(&a.ints, ints)) || (rc = m.get(&a.keys, 2 * pairs)) ||
       (rc = m.get(&a.vals, 2 * pairs)) || (rc = m.get(&a.rot, rot)) || (rc = m.get(&dsched, count)) ||

@@ -94,6 +94,19 @@ def test_batched_rejects_bad_j(ctx):
     assert e.value.code == jkg.JKG_EJ
 
 
+def test_batched_rejects_a_grid_beyond_the_maximum(ctx):
+    """grid above JKG_BATCH_GRID_MAX (4096) is argument 11 (the options), before any device work."""
+    from jkogb_amd import jkg
+    Ys, Js = batch(7, 3, 5, 10)
+    for grid in (4097, 1 << 20, -1):
+        with pytest.raises(jkg.JKGError) as e:
+            ctx.dsolve_batched(Ys, Js, grid=grid)
+        assert e.value.code == -11
+    outs, run = ctx.dsolve_batched(Ys, Js, grid=4096, variant=0)
+    assert run["grid"] == 4096
+    check(ctx, Ys, Js, outs)
+
+
 def test_batched_convergence_test(ctx):
     # with the mixed solver's convergence test: each instance within the spread of the reference's two
     # FMA builds (+64 N eps) of its plain solve, in no more steps

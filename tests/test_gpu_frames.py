@@ -58,3 +58,28 @@ def test_frames_of_complex_and_warm_start_solves(ctx):
         assert f.shape[0] == (out["steps"] - 1) // 2 + 1
         assert np.max(np.abs(f[0].astype(np.int64) - quantise(y, 12))) <= 1
     ctx.set_frames(0)
+
+
+def test_frames_set_twice_with_the_same_arguments(ctx):
+    """jkg_set_frames replaces its device buffer on every call.  A step graph captured with the
+    old buffer must not be replayed with the new one: same schedule and size, other buffer (the
+    old address is taken by another context's frames in between, so the new one differs)."""
+    from jkogb_amd import inputs, jkg
+    y, j = inputs.make("c2", n=40, seed=4)
+    ctx.set_frames(every=1, size=16, max_frames=4096)
+    first = ctx.dsolve(y, j, 20)
+    f1 = ctx.frames()
+    other = jkg.Context(0)
+    try:
+        ctx.set_frames(every=1, size=8, max_frames=16)
+        other.set_frames(every=1, size=16, max_frames=4096)
+        ctx.set_frames(every=1, size=16, max_frames=4096)
+        second = ctx.dsolve(y, j, 20)
+        f2 = ctx.frames()
+    finally:
+        other.set_frames(0)
+        other.close()
+    assert second["steps"] == first["steps"] and f2.shape == f1.shape
+    assert np.array_equal(f1, f2)
+    assert np.max(np.abs(f2[0].astype(np.int64) - quantise(y, 16))) <= 1
+    ctx.set_frames(0)
