@@ -121,10 +121,11 @@ int jkg_ncp(jkg_ctx *ctx, int n, int nn, int nm, jkg_aw *dz, int n2, int32_t *sl
 int jkg_ncp3(jkg_ctx *ctx, int n, int nn, int nm, jkg_aw *dz, int n2, int32_t *sl, int32_t *step, int64_t *info);
 /* jkg_set_select: the greedy walk's schedule for later solves on ctx (no
  * counterpart in the reference; a tuning knob): `rounds` walks per step (1..16;
- * 0: by the list length, about one per 512K pairs, 2 to 12), each but the last
- * stopping after `limit` (>= 2048) entries of its list and followed by a
- * whole-GPU pass that keeps only the entries with both indices free.  The
- * pivots do not depend on it.  Default: rounds by size, 49152 entries. */
+ * 0: by the list length, about one per 768K pairs, 2 to 12), each but the last
+ * stopping after `limit` (>= 2048, rounded up to whole super-chunks of 4096)
+ * entries of its list and followed by a whole-GPU pass that keeps only the
+ * entries with both indices free.  The pivots do not depend on it.  Default:
+ * rounds by size, 65536 entries. */
 int jkg_set_select(jkg_ctx *ctx, int rounds, int limit);
 /* jkg_set_frames: pictures of |A| during later solves on ctx (the role of the
  * reference's ANIMATE frames, src/D/dstep.F90:485-495 and 549-592, written there

@@ -59,7 +59,7 @@ struct jkg_ctx {
   double *wm_part = nullptr;
   int ncp3_n = 0;  // ID_NCP=3: per-offset totals allocated for this N
   // selection schedule: walk rounds per step (0: by size) and entries walked before each filter
-  int sel_rounds = 0, sel_limit = 49152;
+  int sel_rounds = 0, sel_limit = 65536;
   // per-step frames of |A| (jkg_set_frames): device buffer, schedule, caller's buffer and count
   uint8_t *frames_dev = nullptr, *frames_host = nullptr;
   int32_t *frames_count = nullptr;
@@ -121,12 +121,12 @@ static int dalloc(jkg_ctx *c, T **p, size_t count) {
     }                           \
   } while (0)
 
-// Walk rounds per step by list length: about one round (49152 entries walked by one
-// wave, then a whole-GPU filter) per 512K candidate pairs, 2 to 12 (measured on MI355X:
-// N=1024 best with 2; N=4096, select stage per step: 12 x 49152 1.51 ms, 16 x 32768
-// 1.53 ms, 8 x 98304 1.72 ms, 4 x 262144 2.68 ms).
+// Walk rounds per step by list length: about one round (65536 entries walked by one
+// workgroup, then a whole-GPU filter) per 768K candidate pairs, 2 to 12 (measured on
+// MI355X at N=4096, select stage per step: 10 x 65536 0.86 ms, 12 x 49152 0.89 ms,
+// 8 x 98304 0.91 ms, 16 x 32768 0.95 ms, 4 x 262144 1.26 ms, 2 x 1048576 1.73 ms).
 static int select_rounds_for(size_t nn) {
-  const size_t r = nn / 524288 + 1;
+  const size_t r = nn / 786432 + 1;
   return (int)(r < 2 ? 2 : (r > 12 ? 12 : r));
 }
 
@@ -158,11 +158,10 @@ static int ensure_buffers(jkg_ctx *c, int n, int cplx, int trace_cap) {
   ACHK(dalloc(c, (char **)&d.sort.scratch, sort_scratch_bytes((int)nn)));
   d.sort_skip = sort_skip_flags(d.sort, (int)nn);
   for (int k = 0; k < 2; ++k) {
-    ACHK(dalloc(c, &d.lists.key[k], nn));
     ACHK(dalloc(c, &d.lists.val[k], nn + LIST_PAD));
     ACHK(dalloc(c, &d.lists.pos[k], nn + LIST_PAD));
   }
-  ACHK(dalloc(c, &d.fcount, nn / 2048 + 2));
+  ACHK(dalloc(c, &d.fcount, select_count_ints((int)nn)));
   ACHK(dalloc(c, &d.used, n));
   ACHK(dalloc(c, &d.piv_p, n2max));
   ACHK(dalloc(c, &d.piv_q, n2max));
@@ -1207,26 +1206,24 @@ extern "C" int jkg_ncp(jkg_ctx *c, int n, int nn, int nm, jkg_aw *dz, int n2, in
     return -3;
   }
   DeviceStep &d = c->d;
-  std::vector<uint64_t> key(nn);
   std::vector<uint32_t> val(nn), pos(nn);
   for (int i = 0; i < nn; ++i) {
-    key[i] = host_weight_key(dz[i].w);
     val[i] = pack_pq(dz[i].p - 1, dz[i].q - 1);
     pos[i] = (uint32_t)(i + 1);
   }
-  HIPCHK(hipMemcpyAsync(d.lists.key[1], key.data(), 8 * (size_t)nn, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(d.lists.val[1], val.data(), 4 * (size_t)nn, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(d.lists.pos[1], pos.data(), 4 * (size_t)nn, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemsetAsync(d.used, 0, nb, c->stream));
   StepState s{};
   s.n2 = n2;
-  s.sel_round = 1;
-  s.list_len = nn;
+  s.sel_round = 1;  // the walk starts with round 1 over the given list
+  s.next_len = nn;
   s.active = 1;
   HIPCHK(hipMemcpyAsync(d.st, &s, sizeof(s), hipMemcpyHostToDevice, c->stream));
-  // the lists of round 1 are read from index (1-1)&1 = 0: swap the pair
+  // the lists of round 1 are read from index (1-1)&1 = 0: swap the pair; round 0 (the sorted
+  // list) is skipped, so at least two rounds
   DeviceStep dd = d;
-  std::swap(dd.lists.key[0], dd.lists.key[1]);
+  if (dd.sel_rounds < 2) dd.sel_rounds = 2;
   std::swap(dd.lists.val[0], dd.lists.val[1]);
   std::swap(dd.lists.pos[0], dd.lists.pos[1]);
   select_launch(dd, c->stream);

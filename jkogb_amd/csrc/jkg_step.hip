@@ -6,11 +6,13 @@
 //              hyperbolic pairs: the 2x2 HSVD per pair, then the off-norm
 //              change of columns p,q (ABODNDF2) with the rows staged in LDS
 //   sort    -- (b) stable LSD radix sort of (key(W), pair index)
-//   select  -- (c) exact greedy walk of the sorted list (AW_NCP2): one wave
-//              resolves chunks in order with ballots against an LDS flag
-//              array; between rounds every CU filters the rest of the list
-//              against the bitmap (order-preserving compaction), so the walk
-//              only ever visits candidates whose indices were free
+//   select  -- (c) exact greedy walk of the sorted list (AW_NCP2): one
+//              workgroup tests super-chunks of the list against an LDS flag
+//              array with all its lanes and one wave resolves the entries
+//              left alive, in order, with ballots; between rounds every CU
+//              filters the rest of the list against the flags (order-
+//              preserving compaction), so later rounds visit only candidates
+//              whose indices were free
 //   pivots  -- (d) batched 2x2 HSVD of the chosen pivots, rotation descriptors
 //   rotate  -- (e) one workgroup per column pair: both columns of A staged in
 //              LDS, all row transforms (BA), then the column transform (AB)
@@ -49,6 +51,7 @@ __global__ void k_step_begin(StepState *st, int nn, int n, int *colpiv) {
     st->sel_round = 0;
     st->consumed = 0;
     st->list_len = 0;
+    st->next_len = 0;
     act = a;
   }
   __syncthreads();
@@ -204,27 +207,38 @@ __global__ void __launch_bounds__(256) k_hyp_weights(int n, const double *__rest
 }
 
 // ------------------------------------------------------------------ (c) select
-constexpr int FILT_TILE = 2048;
-
-// One wave walks the current list in order, W_CHUNK entries at a time
-// (W_ITEMS consecutive ones per lane).  At the start of a chunk each lane
-// tests its entries against used[] (n + 1 bytes in LDS; used[n] = 1 is the
-// sentinel that past-the-end entries, replaced by the pair (n, n), map to)
-// and keeps a bit mask of the free ones.  One iteration accepts the first free
-// entry of the chunk: a ballot finds the first lane with a free entry (lanes
-// hold increasing positions), readlane takes that entry, lane 0 marks its
-// indices in used[] and records the pivot in LDS, and every lane clears the
-// entries that share an index with it (packed 16-bit compare: a half of
-// v ^ (x * 0x10001) is zero iff that half of v equals x).  A chunk ends with an
-// iteration that finds nothing.  No barrier anywhere: the wave's own LDS
-// accesses are ordered, so the next chunk's test sees this chunk's marks; the
-// loads of the next batch of W_BATCH chunks are in flight while this batch
-// resolves.  The walk stops after `limit` entries, at the end of the list, or
-// at N_2 pivots (AW_NCP2 semantics, src/shared/atypes.F90:191-230).
-constexpr int W_ITEMS = 4;
-constexpr int W_CHUNK = 64 * W_ITEMS;
-static_assert(W_CHUNK <= LIST_PAD, "the last chunk of a list stays inside the padded allocation");
-constexpr int W_BATCH = 8;  // chunks per batch of loads: 2 x 8 loads, two batches in flight (vmcnt counts 63)
+// The greedy walk (AW_NCP2, src/shared/atypes.F90:430-484): take the first
+// entry of the sorted list whose two indices are both free, mark them used,
+// repeat, until N_2 pivots or the end of the list.  It is a chain of
+// dependent acceptances, so one workgroup walks; what it makes parallel is
+// the test of the entries that are already dead.
+//
+// k_sel_walk: one workgroup of SW_WAVES waves walks the list in super-chunks
+// of SW_CHUNK entries (SW_ITEMS consecutive entries per lane, one 16-byte
+// buffer load per array, the next super-chunk's loads in flight).  All lanes
+// test their entries against used[] (n + 1 flag bytes in LDS; used[n] = 1) and
+// the entries still alive are compacted, in list order, into an LDS queue;
+// then wave 0 alone resolves the queue 64 entries at a time: it tests them
+// again (earlier acceptances of this super-chunk may have killed them), and
+// each acceptance is a ballot for the first alive lane, a readlane, two LDS
+// byte stores by lane 0 and a packed 16-bit compare on every lane that drops
+// the entries sharing an index with the accepted pair.  used[] only grows, so
+// an entry dead at the start of its super-chunk stays dead: the queue holds
+// every entry the sequential walk could take, in its order.
+//
+// Between walk rounds the rest of the list is compacted, in order, to the
+// entries whose two indices are both free (k_sel_count: survivors per wave
+// sub-tile; k_sel_write: each tile sums the counts of the tiles before it and
+// writes its survivors with their 1-based positions in the sorted list).  The
+// walk of the next round visits only those.  The pivots do not depend on the
+// schedule: the filter drops only entries the walk would skip.
+constexpr int SW_WAVES = 16;
+constexpr int SW_ITEMS = 4;
+constexpr int SW_CHUNK = SW_WAVES * 64 * SW_ITEMS;  // entries per super-chunk
+static_assert(SW_ITEMS == 4, "one 16-byte load per lane and array");
+constexpr int FT_WAVES = 4;                          // waves of a filter workgroup
+constexpr int FT_SUB = 2048;                         // entries per wave sub-tile
+constexpr int FT_TILE = FT_WAVES * FT_SUB;           // entries per filter workgroup
 
 __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
@@ -257,285 +271,329 @@ __device__ int wave_first_true(int lo, int hi, Pred pred) {
   return lo;
 }
 
+// Exclusive prefix over the wave of a per-lane count in 0..7 (three ballots),
+// and the wave's total.
+__device__ __forceinline__ int wave_prefix_small(int cnt, int &total) {
+  const uint64_t b0 = __ballot(cnt & 1), b1 = __ballot(cnt & 2), b2 = __ballot(cnt & 4);
+  const int lane = threadIdx.x & 63;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  total = __popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2);
+  return __popcll(b0 & lt) + 2 * __popcll(b1 & lt) + 4 * __popcll(b2 & lt);
+}
+
+// Round r of the walk.  Round 0 walks the sorted list (positions implicit);
+// round r > 0 the list the filter of round r - 1 wrote (values and 1-based
+// positions in lp.*[(r - 1) & 1], st->next_len entries).  A round walks at
+// most `limit` entries (0: all) from the start of its list.
 template <bool HAS_POS>
-__global__ void __launch_bounds__(64) k_sel_resolve(int n, const uint64_t *k0, const uint64_t *k1,
-                                                    const uint32_t *v0, const uint32_t *v1, const int *sort_skip,
-                                                    ListPtrs lp, uint8_t *used_g, int *piv_p, int *piv_q,
-                                                    int *piv_pos, StepState *st, int round, int limit, int nn,
-                                                    int n2max) {
-  // n + 1 flag bytes rounded up to words, then this round's pivots (p, q, position; n2max each),
-  // written out at the end: no store to global memory inside the walk, so that the wait for a
-  // chunk's loads counts exactly the loads issued after it
-  extern __shared__ uint32_t usedw[];
-  uint8_t *used = (uint8_t *)usedw;
-  if (st->sel_done || st->sel_round != round) return;
-  const int lane = threadIdx.x;
-  SortResult sr = sort_result_dev(k0, k1, v0, v1, sort_skip);
-  const uint32_t *vals = (round == 0) ? sr.vals : lp.val[(round - 1) & 1];
-  const uint32_t *poss = HAS_POS ? lp.pos[(round - 1) & 1] : nullptr;
+__global__ void __launch_bounds__(SW_WAVES * 64) k_sel_walk(int n, const uint64_t *k0, const uint64_t *k1,
+                                                            const uint32_t *v0, const uint32_t *v1,
+                                                            const int *sort_skip, ListPtrs lp, uint8_t *used_g,
+                                                            int *piv_p, int *piv_q, int *piv_pos, StepState *st,
+                                                            int round, int limit, int nn, int n2max) {
+  // LDS: the queue of alive entries (values, positions; SW_CHUNK each, 16-byte aligned) | n + 1
+  // flag bytes rounded up to words | this round's pivots (p, q, position; n2max each)
+  extern __shared__ __attribute__((aligned(16))) uint32_t lds_raw[];
+  __shared__ int wtot[2][SW_WAVES];
+  __shared__ int sh_len, sh_acc;
+  if (st->sel_done || st->sel_round > round) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nw = n >> 2;
-  int len;
-  if (round == 0) {
-    // IT = number of finite weights = position of the first NaN key
-    const uint64_t *keys = sr.keys;
-    const int it = wave_first_true(0, nn, [&](int i) { return key_nan(keys[i]); });
-    const double wmax = (it > 0) ? key_weight(keys[0]) : 0.0;
-    len = it;
-    // the mixed solver's threshold: walk only the weights >= tau * (largest weight)
-    const double tau = st->tau;
-    if (tau > 0.0 && it > 0 && wmax > 0.0) {
-      const uint64_t kt = weight_key(tau * wmax);  // keys above kt stand for smaller weights
-      len = wave_first_true(0, it, [&](int i) { return keys[i] > kt; });
+  uint32_t *qv = lds_raw, *qp = lds_raw + SW_CHUNK;
+  uint32_t *lds = lds_raw + 2 * SW_CHUNK;  // the flag words
+  uint8_t *used = (uint8_t *)lds;
+  int *bp = (int *)(lds + nw + 1), *bq = bp + n2max, *bpos = bq + n2max;
+  SortResult sr = sort_result_dev(k0, k1, v0, v1, sort_skip);
+  const uint32_t *vals = HAS_POS ? lp.val[(round - 1) & 1] : sr.vals;
+  const uint32_t *poss = HAS_POS ? lp.pos[(round - 1) & 1] : vals;
+  if (!HAS_POS) {
+    if (wave == 0) {
+      // IT = number of finite weights = position of the first NaN key
+      const uint64_t *keys = sr.keys;
+      const int it = wave_first_true(0, nn, [&](int i) { return key_nan(keys[i]); });
+      const double wmax = (it > 0) ? key_weight(keys[0]) : 0.0;
+      int len = it;
+      // the mixed solver's threshold: walk only the weights >= tau * (largest weight)
+      const double tau = st->tau;
+      if (tau > 0.0 && it > 0 && wmax > 0.0) {
+        const uint64_t kt = weight_key(tau * wmax);  // keys above kt stand for smaller weights
+        len = wave_first_true(0, it, [&](int i) { return keys[i] > kt; });
+      }
+      if (lane == 0) {
+        st->it = it;
+        st->wmax = wmax;
+        sh_len = len;
+      }
     }
-    if (lane == 0) {
-      st->it = it;
-      st->wmax = wmax;
-      st->list_len = len;
-    }
-    for (int i = lane; i <= nw; i += 64) usedw[i] = 0u;
+    for (int i = tid; i <= nw; i += SW_WAVES * 64) lds[i] = 0u;
   } else {
     const uint32_t *ug = (const uint32_t *)used_g;  // 256-byte aligned allocation of n bytes
-    for (int i = lane; i < nw; i += 64) usedw[i] = ug[i];
-    if (lane < (n & 3)) used[4 * nw + lane] = used_g[4 * nw + lane];
-    len = st->list_len;
+    for (int i = tid; i < nw; i += SW_WAVES * 64) lds[i] = ug[i];
+    if (tid == 0) {
+      lds[nw] = 0u;
+      sh_len = st->next_len;
+    }
   }
-  if (lane == 0) used[n] = 1;
   lds_barrier();
-  int *bp = (int *)(usedw + nw + 1), *bq = bp + n2max, *bpos = bq + n2max;
+  if (HAS_POS && tid < (n & 3)) used[4 * nw + tid] = used_g[4 * nw + tid];
+  if (tid == 0) used[n] = 1;
+  lds_barrier();
+  const int len = sh_len;
   const int n2 = st->n2;
   const int acc0 = st->sl;
   int acc = acc0;
-  int c = st->consumed;
-  const int stop = (limit > 0 && len - c > limit) ? c + limit : len;
-  // Chunk loads: one 16-byte buffer load per array per lane (a round starts at entry 0 of its list, so
-  // chunk bases are multiples of W_CHUNK and the loads are aligned).  The buffer range ends at the end of
-  // the list rounded up to whole chunks (the allocations hold LIST_PAD entries past their capacity), so a
-  // load never straddles it and the loads of batches past the end read zeros without touching memory:
-  // every lane loads, and the number of loads in flight does not depend on the data.  Entries past the
-  // end become the pair (n, n), dead through the sentinel used[n] = 1, when their chunk is walked.
+  int c = 0;
+  const int stop = (limit > 0 && len > limit) ? limit : len;
+  // Buffer loads whose range ends at `stop` rounded up to whole 16-byte groups (inside the
+  // allocations): loads past it read zeros without touching memory, so every lane loads, and
+  // entries at or past `stop` are masked by position.
   typedef uint32_t u4 __attribute__((ext_vector_type(4)));
-  static_assert(W_ITEMS == 4, "one 16-byte load per lane and array");
-  const uint32_t sent = (uint32_t)n * 0x10001u;
-  const int range = ((stop + W_CHUNK - 1) / W_CHUNK) * W_CHUNK * 4;  // bytes
+  const int range = ((stop + 3) >> 2) << 4;
   const __amdgpu_buffer_rsrc_t rsv = __builtin_amdgcn_make_buffer_rsrc((void *)vals, (short)0, range, 0x00020000);
-  const __amdgpu_buffer_rsrc_t rsp =
-      __builtin_amdgcn_make_buffer_rsrc((void *)(HAS_POS ? poss : vals), (short)0, range, 0x00020000);
-  const int lane_off = lane * W_ITEMS * 4;
-  auto load_chunk = [&](u4 &dv, u4 &dp, int base) {
-    dv = __builtin_amdgcn_raw_buffer_load_b128(rsv, lane_off + base * 4, 0, 0);
-    if constexpr (HAS_POS) dp = __builtin_amdgcn_raw_buffer_load_b128(rsp, lane_off + base * 4, 0, 0);
+  const __amdgpu_buffer_rsrc_t rsp = __builtin_amdgcn_make_buffer_rsrc((void *)poss, (short)0, range, 0x00020000);
+  const int loff = tid * SW_ITEMS * 4;
+  auto load = [&](u4 &dv, u4 &dp, int base) {
+    dv = __builtin_amdgcn_raw_buffer_load_b128(rsv, loff + base * 4, 0, 0);
+    if constexpr (HAS_POS) dp = __builtin_amdgcn_raw_buffer_load_b128(rsp, loff + base * 4, 0, 0);
   };
-  // one chunk: the free-entry test against used[], then the acceptances
-  bool run = c < stop && acc < n2;
-  auto walk_chunk = [&](const u4 &cv, const u4 &cp) {
-    uint32_t vv[W_ITEMS] = {cv.x, cv.y, cv.z, cv.w};
-    uint32_t pv[W_ITEMS] = {0u, 0u, 0u, 0u};
+  bool run = stop > 0 && acc < n2;
+  int par = 0;
+  auto chunk = [&](const u4 &cv, const u4 &cp) {
+    const uint32_t vv[SW_ITEMS] = {cv.x, cv.y, cv.z, cv.w};
+    uint32_t pv[SW_ITEMS];
+    const int e0 = c + tid * SW_ITEMS;
     if constexpr (HAS_POS) {
       pv[0] = cp.x;
       pv[1] = cp.y;
       pv[2] = cp.z;
       pv[3] = cp.w;
-    }
-    if (c + W_CHUNK > stop) {  // the last chunk of the walk
+    } else {
 #pragma unroll
-      for (int i = 0; i < W_ITEMS; ++i) vv[i] = (c + lane * W_ITEMS + i < stop) ? vv[i] : sent;
+      for (int i = 0; i < SW_ITEMS; ++i) pv[i] = (uint32_t)(e0 + i + 1);
     }
     uint32_t alive = 0;
 #pragma unroll
-    for (int i = 0; i < W_ITEMS; ++i) alive |= ((used[vv[i] & 0xFFFFu] | used[vv[i] >> 16]) == 0 ? 1u : 0u) << i;
-    const int base = c;
-    c = (c + W_CHUNK < stop) ? c + W_CHUNK : stop;
-    for (;;) {
-      const uint64_t bal = __ballot(alive != 0);
-      if (bal == 0) break;
-      const int L = __ffsll((unsigned long long)bal) - 1;
-      const int fi = __ffs(alive) - 1;
-      uint32_t fv = vv[0], fp = pv[0];
+    for (int i = 0; i < SW_ITEMS; ++i)
+      alive |= ((e0 + i < stop) && (used[vv[i] & 0xFFFFu] | used[vv[i] >> 16]) == 0 ? 1u : 0u) << i;
+    int wsum;
+    const int pre = wave_prefix_small(__popc(alive), wsum);
+    if (lane == 0) wtot[par][wave] = wsum;
+    lds_barrier();
+    int base = 0, total = 0;
 #pragma unroll
-      for (int i = 1; i < W_ITEMS; ++i) {
-        fv = (fi == i) ? vv[i] : fv;
-        if constexpr (HAS_POS) fp = (fi == i) ? pv[i] : fp;
-      }
-      const uint32_t gv = __builtin_amdgcn_readlane(fv, L);
-      const int gpos = HAS_POS ? (int)__builtin_amdgcn_readlane(fp, L)
-                               : base + L * W_ITEMS + __builtin_amdgcn_readlane(fi, L) + 1;
-      const uint32_t pg = pq_p(gv), qg = pq_q(gv);
-      if (lane == 0) {
-        used[pg] = 1;
-        used[qg] = 1;
-        bp[acc - acc0] = (int)pg;
-        bq[acc - acc0] = (int)qg;
-        bpos[acc - acc0] = gpos;
-      }
-      ++acc;
-      if (acc >= n2) break;
-      const uint32_t P = pg * 0x10001u, Q = qg * 0x10001u;
-      uint32_t kill = 0;
+    for (int w = 0; w < SW_WAVES; ++w) {
+      const int t = wtot[par][w];
+      base += (w < wave) ? t : 0;
+      total += t;
+    }
+    par ^= 1;
+    int o = base + pre;
 #pragma unroll
-      for (int i = 0; i < W_ITEMS; ++i) kill |= (pair_hits(vv[i], P, Q) != 0 ? 1u : 0u) << i;
-      alive &= ~kill;
+    for (int i = 0; i < SW_ITEMS; ++i)
+      if ((alive >> i) & 1u) {
+        qv[o] = vv[i];
+        qp[o] = pv[i];
+        ++o;
+      }
+    c = (c + SW_CHUNK < stop) ? c + SW_CHUNK : stop;
+    if (total > 0) {
+      lds_barrier();
+      if (wave == 0) {
+        // the queue 64 entries at a time, one per lane: the free-entry test against used[] gives a
+        // wave-uniform mask of alive lanes; each acceptance is the first set bit, a readlane of its
+        // pair and a ballot of the lanes that share an index with it (cleared from the mask), all in
+        // scalar registers; then the accepted lanes record their pivots (in lane order, which is
+        // list order) and mark their indices in used[] together
+        const uint32_t sent = (uint32_t)n * 0x10001u;  // the pair (n, n): dead through used[n] = 1
+        const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+        for (int b = 0; b < total && acc < n2; b += 64) {
+          const int k = b + lane;
+          const uint32_t x = (k < total) ? qv[k] : sent;
+          const uint32_t xp = (k < total) ? qp[k] : 0u;
+          uint64_t A = __ballot((used[x & 0xFFFFu] | used[x >> 16]) == 0);
+          uint64_t M = 0;
+          int room = n2 - acc;
+          while (A != 0 && room > 0) {
+            const int L = __ffsll((unsigned long long)A) - 1;
+            const uint32_t gv = __builtin_amdgcn_readlane(x, L);
+            M |= 1ull << L;
+            --room;
+            A &= ~__ballot(pair_hits(x, (gv & 0xFFFFu) * 0x10001u, (gv >> 16) * 0x10001u) != 0);
+          }
+          if ((M >> lane) & 1ull) {
+            const int r = acc - acc0 + __popcll(M & lt);
+            bp[r] = (int)(x & 0xFFFFu);
+            bq[r] = (int)(x >> 16);
+            bpos[r] = (int)xp;
+            used[x & 0xFFFFu] = 1;
+            used[x >> 16] = 1;
+          }
+          acc += __popcll(M);
+        }
+        if (lane == 0) sh_acc = acc;
+      }
+      lds_barrier();
+      acc = sh_acc;
     }
     run = c < stop && acc < n2;
   };
-  // two batches of W_BATCH chunks, ping-pong: all loads of the next batch are issued before the
-  // current one is walked, so a batch's loads have a whole batch of work to arrive in
-  u4 va[W_BATCH], pa[W_BATCH], vb[W_BATCH], pb[W_BATCH];
-  int nb = c;
-#pragma unroll
-  for (int k = 0; k < W_BATCH; ++k) load_chunk(va[k], pa[k], nb + k * W_CHUNK);
-  nb += W_BATCH * W_CHUNK;
+  // two super-chunks in registers, ping-pong: the next one's loads are issued before the current
+  // one is tested, so they have a whole super-chunk of work to arrive in
+  u4 va, pa, vb, pb;
+  int nb = 0;
+  load(va, pa, nb);
+  nb += SW_CHUNK;
   while (run) {
-#pragma unroll
-    for (int k = 0; k < W_BATCH; ++k) load_chunk(vb[k], pb[k], nb + k * W_CHUNK);
-    nb += W_BATCH * W_CHUNK;
-#pragma unroll
-    for (int k = 0; k < W_BATCH; ++k) {
-      if (!run) break;
-      walk_chunk(va[k], pa[k]);
-    }
+    load(vb, pb, nb);
+    nb += SW_CHUNK;
+    chunk(va, pa);
     if (!run) break;
-#pragma unroll
-    for (int k = 0; k < W_BATCH; ++k) load_chunk(va[k], pa[k], nb + k * W_CHUNK);
-    nb += W_BATCH * W_CHUNK;
-#pragma unroll
-    for (int k = 0; k < W_BATCH; ++k) {
-      if (!run) break;
-      walk_chunk(vb[k], pb[k]);
-    }
+    load(va, pa, nb);
+    nb += SW_CHUNK;
+    chunk(vb, pb);
   }
   lds_barrier();
   {
     uint32_t *ug = (uint32_t *)used_g;
-    for (int i = lane; i < nw; i += 64) ug[i] = usedw[i];
-    if (lane < (n & 3)) used_g[4 * nw + lane] = used[4 * nw + lane];
-    for (int i = lane; i < acc - acc0; i += 64) {
+    for (int i = tid; i < nw; i += SW_WAVES * 64) ug[i] = lds[i];
+    if (tid < (n & 3)) used_g[4 * nw + tid] = used[4 * nw + tid];
+    for (int i = tid; i < acc - acc0; i += SW_WAVES * 64) {
       piv_p[acc0 + i] = bp[i];
       piv_q[acc0 + i] = bq[i];
       piv_pos[acc0 + i] = bpos[i];
     }
   }
-  if (lane == 0) {
+  if (tid == 0) {
     st->sl = acc;
     st->consumed = c;
+    st->list_len = len;
+    st->sel_round = round;
     if (acc >= n2 || c >= len) st->sel_done = 1;
   }
 }
 
-// Filter: survivors of [consumed, len) whose indices are both free, counted
-// per tile of FILT_TILE entries ...
-__global__ void __launch_bounds__(256) k_sel_filter_count(int n, const uint64_t *k0, const uint64_t *k1,
-                                                          const uint32_t *v0, const uint32_t *v1,
-                                                          const int *sort_skip, ListPtrs lp,
-                                                          const uint8_t *used_g, int *fcount, const StepState *st,
-                                                          int round) {
-  extern __shared__ uint8_t used[];
-  __shared__ int wc[4];
+// The n flag bytes of used_g (a 256-byte aligned allocation) into LDS, by words.
+__device__ __forceinline__ void copy_used(uint8_t *used, const uint8_t *used_g, int n) {
+  const int nw = n >> 2;
+  for (int i = threadIdx.x; i < nw; i += blockDim.x) ((uint32_t *)used)[i] = ((const uint32_t *)used_g)[i];
+  if ((int)threadIdx.x < (n & 3)) used[4 * nw + threadIdx.x] = used_g[4 * nw + threadIdx.x];
+}
+
+// Filter of round r: survivors of [consumed, list_len) of the round's list whose
+// indices are both free.  fcount[t] = survivors of tile t, fcount[ntiles +
+// FT_WAVES t + w] = survivors of its wave sub-tile w ...
+__global__ void __launch_bounds__(FT_WAVES * 64) k_sel_count(int n, const uint64_t *k0, const uint64_t *k1,
+                                                             const uint32_t *v0, const uint32_t *v1,
+                                                             const int *sort_skip, ListPtrs lp,
+                                                             const uint8_t *used_g, int *fcount, int ntiles,
+                                                             const StepState *st, int round) {
+  extern __shared__ __attribute__((aligned(16))) uint8_t used[];
+  __shared__ int wc[FT_WAVES];
   if (st->sel_done || st->sel_round != round) return;
   const int c0 = st->consumed, len = st->list_len;
-  const int base = c0 + blockIdx.x * FILT_TILE;
-  if (base >= len) {
-    if (threadIdx.x == 0) fcount[blockIdx.x] = 0;
-    return;
-  }
-  for (int i = threadIdx.x; i < n; i += blockDim.x) used[i] = used_g[i];
+  const int base = c0 + blockIdx.x * FT_TILE;
+  if (base >= len) return;  // counts of tiles past the list are never read
+  copy_used(used, used_g, n);
   __syncthreads();
   SortResult sr = sort_result_dev(k0, k1, v0, v1, sort_skip);
   const uint32_t *vals = (round == 0) ? sr.vals : lp.val[(round - 1) & 1];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int sub = base + wave * FT_SUB;
+  const int end = (sub + FT_SUB < len) ? sub + FT_SUB : len;
+  typedef uint32_t u4 __attribute__((ext_vector_type(4)));
   int cnt = 0;
-  for (int e = base + threadIdx.x; e < base + FILT_TILE && e < len; e += blockDim.x) {
-    const uint32_t v = vals[e];
-    cnt += (!used[pq_p(v)] && !used[pq_q(v)]) ? 1 : 0;
+  // c0 is a multiple of SW_CHUNK (the walk stops at whole super-chunks or at the end of its
+  // list), so every group of 4 entries is 16-byte aligned; groups reach at most 3 entries past
+  // the list, inside the padded allocation
+  for (int e0 = sub + 4 * lane; e0 < end; e0 += 256) {
+    const u4 v = *(const u4 *)(vals + e0);
+    const uint32_t vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) cnt += (e0 + i < end && (used[vv[i] & 0xFFFFu] | used[vv[i] >> 16]) == 0) ? 1 : 0;
   }
   for (int o = 32; o > 0; o >>= 1) cnt += __shfl_down(cnt, o);
-  if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = cnt;
-  __syncthreads();
-  if (threadIdx.x == 0) fcount[blockIdx.x] = wc[0] + wc[1] + wc[2] + wc[3];
-}
-
-// ... exclusive scan of the tile counts (one workgroup) ...
-__global__ void __launch_bounds__(1024) k_sel_filter_scan(int *fcount, int ntiles, StepState *st, int round) {
-  __shared__ int part[1024];
-  if (st->sel_done || st->sel_round != round) return;
-  const int per = (ntiles + 1023) / 1024;
-  const int lo = threadIdx.x * per;
-  int s = 0;
-  for (int i = 0; i < per; ++i)
-    if (lo + i < ntiles) s += fcount[lo + i];
-  part[threadIdx.x] = s;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    int v = (threadIdx.x >= off) ? part[threadIdx.x - off] : 0;
-    __syncthreads();
-    part[threadIdx.x] += v;
-    __syncthreads();
+  if (lane == 0) {
+    wc[wave] = cnt;
+    fcount[ntiles + FT_WAVES * blockIdx.x + wave] = cnt;
   }
-  int run = part[threadIdx.x] - s;
-  for (int i = 0; i < per; ++i)
-    if (lo + i < ntiles) {
-      int v = fcount[lo + i];
-      fcount[lo + i] = run;
-      run += v;
-    }
-  if (threadIdx.x == 1023) fcount[ntiles] = part[1023];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int w = 0; w < FT_WAVES; ++w) t += wc[w];
+    fcount[blockIdx.x] = t;
+  }
 }
 
-// ... and written in order to the next list.
-__global__ void __launch_bounds__(256) k_sel_filter_write(int n, const uint64_t *k0, const uint64_t *k1,
-                                                          const uint32_t *v0, const uint32_t *v1,
-                                                          const int *sort_skip, ListPtrs lp,
-                                                          const uint8_t *used_g, const int *fcount,
-                                                          const StepState *st, int round) {
-  extern __shared__ uint8_t used[];
-  __shared__ int wc[4];
-  __shared__ int run;
+// ... and written in order to the next round's list (values and 1-based
+// positions in the sorted list); the tile holding the list's last entry
+// stores the new length in st->next_len.
+__global__ void __launch_bounds__(FT_WAVES * 64) k_sel_write(int n, const uint64_t *k0, const uint64_t *k1,
+                                                             const uint32_t *v0, const uint32_t *v1,
+                                                             const int *sort_skip, ListPtrs lp,
+                                                             const uint8_t *used_g, const int *fcount, int ntiles,
+                                                             StepState *st, int round) {
+  extern __shared__ __attribute__((aligned(16))) uint8_t used[];
+  __shared__ int red[FT_WAVES];
   if (st->sel_done || st->sel_round != round) return;
   const int c0 = st->consumed, len = st->list_len;
-  const int base = c0 + blockIdx.x * FILT_TILE;
-  if (base < len) {
-    for (int i = threadIdx.x; i < n; i += blockDim.x) used[i] = used_g[i];
-    if (threadIdx.x == 0) run = fcount[blockIdx.x];
-    __syncthreads();
-    SortResult sr = sort_result_dev(k0, k1, v0, v1, sort_skip);
-    const uint64_t *keys = (round == 0) ? sr.keys : lp.key[(round - 1) & 1];
-    const uint32_t *vals = (round == 0) ? sr.vals : lp.val[(round - 1) & 1];
-    const uint32_t *poss = (round == 0) ? nullptr : lp.pos[(round - 1) & 1];
-    uint64_t *okey = lp.key[round & 1];
-    uint32_t *oval = lp.val[round & 1], *opos = lp.pos[round & 1];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    for (int e0 = base; e0 < base + FILT_TILE && e0 < len; e0 += blockDim.x) {
-      const int e = e0 + threadIdx.x;
-      bool keep = false;
-      uint32_t kk = 0;
-      if (e < base + FILT_TILE && e < len) {
-        kk = vals[e];
-        keep = !used[pq_p(kk)] && !used[pq_q(kk)];
+  const int t = blockIdx.x;
+  const int base = c0 + t * FT_TILE;
+  if (base >= len) return;
+  copy_used(used, used_g, n);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // survivors of the tiles before this one (each tile sums them itself: no pass over the counts)
+  int s = 0;
+  for (int i = threadIdx.x; i < t; i += blockDim.x) s += fcount[i];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o);
+  if (lane == 0) red[wave] = s;
+  __syncthreads();
+  int out = 0;
+  for (int w = 0; w < FT_WAVES; ++w) out += red[w];
+  const int tile_pre = out;
+  for (int w = 0; w < wave; ++w) out += fcount[ntiles + FT_WAVES * t + w];
+  if (threadIdx.x == 0 && t == (len - 1 - c0) / FT_TILE) st->next_len = tile_pre + fcount[t];
+  SortResult sr = sort_result_dev(k0, k1, v0, v1, sort_skip);
+  const uint32_t *vals = (round == 0) ? sr.vals : lp.val[(round - 1) & 1];
+  const uint32_t *poss = (round == 0) ? nullptr : lp.pos[(round - 1) & 1];
+  uint32_t *oval = lp.val[round & 1], *opos = lp.pos[round & 1];
+  const int sub = base + wave * FT_SUB;
+  const int end = (sub + FT_SUB < len) ? sub + FT_SUB : len;
+  typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+  for (int g = sub; g < end; g += 256) {  // uniform over the wave
+    const int e0 = g + 4 * lane;
+    uint32_t vv[4] = {0u, 0u, 0u, 0u}, pv[4];
+    uint32_t alive = 0;
+    if (e0 < end) {
+      const u4 v = *(const u4 *)(vals + e0);
+      vv[0] = v.x;
+      vv[1] = v.y;
+      vv[2] = v.z;
+      vv[3] = v.w;
+      if (poss) {
+        const u4 p = *(const u4 *)(poss + e0);
+        pv[0] = p.x;
+        pv[1] = p.y;
+        pv[2] = p.z;
+        pv[3] = p.w;
+      } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) pv[i] = (uint32_t)(e0 + i + 1);
       }
-      uint64_t bal = __ballot(keep);
-      if (lane == 0) wc[wave] = __popcll(bal);
-      __syncthreads();
-      int off = run;
-      for (int w = 0; w < wave; ++w) off += wc[w];
-      if (keep) {
-        int o = off + __popcll(bal & lt);
-        okey[o] = keys[e];
-        oval[o] = kk;
-        opos[o] = poss ? poss[e] : (uint32_t)(e + 1);
-      }
-      __syncthreads();
-      if (threadIdx.x == 0) run += wc[0] + wc[1] + wc[2] + wc[3];
-      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        alive |= ((e0 + i < end) && (used[vv[i] & 0xFFFFu] | used[vv[i] >> 16]) == 0 ? 1u : 0u) << i;
     }
+    int tot;
+    int o = out + wave_prefix_small(__popc(alive), tot);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if ((alive >> i) & 1u) {
+        oval[o] = vv[i];
+        opos[o] = pv[i];
+        ++o;
+      }
+    out += tot;
   }
-}
-
-// list bookkeeping for the next round (fcount[ntiles] holds the total)
-__global__ void k_sel_filter_end(const int *fcount, int ntiles, StepState *st, int round) {
-  if (st->sel_done || st->sel_round != round) return;
-  st->list_len = fcount[ntiles];
-  st->consumed = 0;
-  st->sel_round = round + 1;
-  if (fcount[ntiles] == 0) st->sel_done = 1;
 }
 
 // ------------------------------------------------------------------ (d) pivots
@@ -956,46 +1014,50 @@ void weights_launch(DeviceStep &d, hipStream_t s) {
 }
 
 static size_t select_lds_bytes(const DeviceStep &d) {
-  return (((size_t)d.n >> 2) + 1) * 4 + (size_t)3 * d.n2max * sizeof(int);
+  return (((size_t)d.n >> 2) + 1) * 4 + (size_t)3 * d.n2max * sizeof(int) + (size_t)2 * SW_CHUNK * sizeof(uint32_t);
 }
 
 int select_prepare(DeviceStep &d) {
   const size_t lds = select_lds_bytes(d);
-  if (lds > 160 * 1024) return -1;
-  if (hipFuncSetAttribute((const void *)k_sel_resolve<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+  if (lds > 160 * 1024 - 256) return -1;
+  if (hipFuncSetAttribute((const void *)k_sel_walk<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                           (int)lds) != hipSuccess ||
-      hipFuncSetAttribute((const void *)k_sel_resolve<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+      hipFuncSetAttribute((const void *)k_sel_walk<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                           (int)lds) != hipSuccess)
     return -2;
   return 0;
 }
 
+int select_tiles(int nn) { return (nn + FT_TILE - 1) / FT_TILE; }
+
+size_t select_count_ints(int nn) { return (size_t)(FT_WAVES + 1) * (select_tiles(nn) + 1) + 2; }
+
 void select_launch(DeviceStep &d, hipStream_t s) {
-  const int SEL_ROUNDS = d.sel_rounds > 0 ? d.sel_rounds : 2;
+  const int rounds = d.sel_rounds > 0 ? d.sel_rounds : 2;
   if (d.ncp3) {
     ncp3_launch(d, s);
     return;
   }
-  const int ntiles = (d.nn + FILT_TILE - 1) / FILT_TILE;
+  const int ntiles = select_tiles(d.nn);
   const size_t lds = (size_t)d.n + 1;
   const size_t ldsw = select_lds_bytes(d);
-  for (int r = 0; r < SEL_ROUNDS; ++r) {
-    const int limit = (r == SEL_ROUNDS - 1) ? 0 : d.select_limit;
+  // a round walks whole super-chunks (the filter's groups of 4 entries stay aligned)
+  const int limit = ((d.select_limit + SW_CHUNK - 1) / SW_CHUNK) * SW_CHUNK;
+  for (int r = 0; r < rounds; ++r) {
+    const int lim = (r == rounds - 1) ? 0 : limit;
     if (r == 0)
-      hipLaunchKernelGGL(k_sel_resolve<false>, dim3(1), dim3(64), ldsw, s, d.n, d.sort.k0, d.sort.k1, d.sort.v0,
-                         d.sort.v1, d.sort_skip, d.lists, d.used, d.piv_p, d.piv_q, d.piv_pos, d.st, r, limit,
-                         d.nn, d.n2max);
-    else
-      hipLaunchKernelGGL(k_sel_resolve<true>, dim3(1), dim3(64), ldsw, s, d.n, d.sort.k0, d.sort.k1,
+      hipLaunchKernelGGL(k_sel_walk<false>, dim3(1), dim3(SW_WAVES * 64), ldsw, s, d.n, d.sort.k0, d.sort.k1,
                          d.sort.v0, d.sort.v1, d.sort_skip, d.lists, d.used, d.piv_p, d.piv_q, d.piv_pos, d.st, r,
-                         limit, d.nn, d.n2max);
-    if (r == SEL_ROUNDS - 1) break;
-    hipLaunchKernelGGL(k_sel_filter_count, dim3(ntiles), dim3(256), lds, s, d.n, d.sort.k0, d.sort.k1, d.sort.v0,
-                       d.sort.v1, d.sort_skip, d.lists, d.used, d.fcount, d.st, r);
-    hipLaunchKernelGGL(k_sel_filter_scan, dim3(1), dim3(1024), 0, s, d.fcount, ntiles, d.st, r);
-    hipLaunchKernelGGL(k_sel_filter_write, dim3(ntiles), dim3(256), lds, s, d.n, d.sort.k0, d.sort.k1, d.sort.v0,
-                       d.sort.v1, d.sort_skip, d.lists, d.used, d.fcount, d.st, r);
-    hipLaunchKernelGGL(k_sel_filter_end, dim3(1), dim3(1), 0, s, d.fcount, ntiles, d.st, r);
+                         lim, d.nn, d.n2max);
+    else
+      hipLaunchKernelGGL(k_sel_walk<true>, dim3(1), dim3(SW_WAVES * 64), ldsw, s, d.n, d.sort.k0, d.sort.k1,
+                         d.sort.v0, d.sort.v1, d.sort_skip, d.lists, d.used, d.piv_p, d.piv_q, d.piv_pos, d.st, r,
+                         lim, d.nn, d.n2max);
+    if (r == rounds - 1) break;
+    hipLaunchKernelGGL(k_sel_count, dim3(ntiles), dim3(FT_WAVES * 64), lds, s, d.n, d.sort.k0, d.sort.k1, d.sort.v0,
+                       d.sort.v1, d.sort_skip, d.lists, d.used, d.fcount, ntiles, d.st, r);
+    hipLaunchKernelGGL(k_sel_write, dim3(ntiles), dim3(FT_WAVES * 64), lds, s, d.n, d.sort.k0, d.sort.k1, d.sort.v0,
+                       d.sort.v1, d.sort_skip, d.lists, d.used, d.fcount, ntiles, d.st, r);
   }
 }
 

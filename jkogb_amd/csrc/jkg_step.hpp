@@ -11,12 +11,13 @@
 namespace jkg {
 
 // Entries allocated past the capacity of the sort's value buffers and of the
-// candidate lists: the greedy walk loads whole chunks of 256 entries.
+// candidate lists: the filter reads groups of 4 entries that may reach past
+// the end of a list.
 constexpr int LIST_PAD = 256;
 
-// Candidate lists of the selection rounds (ping-pong), passed by value.
+// Candidate lists of the selection rounds after the first (ping-pong): the
+// packed pairs and their 1-based positions in the sorted list; passed by value.
 struct ListPtrs {
-  uint64_t *key[2];
   uint32_t *val[2];
   uint32_t *pos[2];
 };
@@ -55,6 +56,7 @@ struct StepState {
   int sel_round;  // round of the current list
   int list_len;   // length of the current candidate list
   int consumed;   // entries of the current list already walked
+  int next_len;   // length of the list the last filter wrote (the next round's)
   int trace_cap;  // capacity of the trace buffer
   int nrec;       // trace records written (last step index + 1)
   int ntrig, nhyp;
@@ -89,7 +91,7 @@ struct DeviceStep {
   int *sort_skip;
   // selection lists (round >= 1): keys, pair indices, 1-based sorted positions
   ListPtrs lists;
-  int *fcount;            // per-tile survivor counts of the filter
+  int *fcount;            // survivor counts of the filter: per tile, then per wave sub-tile
   uint8_t *used;          // N bytes: index already in a chosen pivot
   // pivots
   int *piv_p, *piv_q, *piv_pos, *piv_info;
@@ -166,7 +168,9 @@ int zstep_prepare(DeviceStep &d);
 void zinit_identity_launch(DeviceStep &d, hipStream_t s);
 void zfinalize_launch(DeviceStep &d, hipStream_t s, double *sigma);
 void select_launch(DeviceStep &d, hipStream_t s);
-int select_prepare(DeviceStep &d);  // the walk's LDS (flags and pivots of a round) for the problem size
+int select_prepare(DeviceStep &d);  // the walk's LDS (flags, pivots of a round, queue) for the problem size
+int select_tiles(int nn);            // workgroups of a filter pass over nn entries
+size_t select_count_ints(int nn);    // ints of DeviceStep::fcount
 void pivots_launch(DeviceStep &d, hipStream_t s);
 void rotate_launch(DeviceStep &d, hipStream_t s);
 void step_end_launch(DeviceStep &d, hipStream_t s);

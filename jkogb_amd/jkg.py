@@ -438,7 +438,7 @@ class Context:
         buf, cnt = getattr(self, "_frames", None) or (np.zeros((0, 0, 0), np.uint8), np.zeros(1, np.int32))
         return buf[:int(cnt[0])].copy()
 
-    def set_select(self, rounds=0, limit=49152):
+    def set_select(self, rounds=0, limit=65536):
         """The greedy walk's schedule (jkg_set_select); the pivots do not depend on it."""
         _chk(_set_select(self._h, rounds, limit), "jkg_set_select")
 

@@ -128,7 +128,7 @@ def test_build_bit_exact_vs_oracle(ctx, orc, cfg, n, seed, n2):
 
 
 def test_build_deep_walk_uses_filter_rounds(ctx, orc):
-    """N = 700: NN = 244650 candidates, more than one walk round (49152
+    """N = 700: NN = 244650 candidates, more than one walk round (65536
     entries), so the filter/compaction rounds decide the tail of the walk."""
     y, j = inputs("c2", 700, 9)
     assert check_build(ctx, orc, y, j, 350) == 350
@@ -137,7 +137,7 @@ def test_build_deep_walk_uses_filter_rounds(ctx, orc):
 @pytest.mark.parametrize("rounds,limit", [(2, 2048), (5, 2049), (16, 3001), (3, 100000)])
 def test_build_any_walk_schedule(ctx, orc, rounds, limit):
     """The pivots do not depend on the walk's schedule: rounds that stop after
-    `limit` entries (not a multiple of the walk's 256-entry chunks), each
+    `limit` entries (rounded up to the walk's 4096-entry super-chunks), each
     followed by a filter pass, and a last unlimited round give the oracle's
     selection."""
     y, j = inputs("c2", 700, 9)
