@@ -5,16 +5,29 @@
 ! Same command line, files and messages as the reference's djk.exe/zjk.exe
 ! (src/D/djk.F90, src/Z/zjk.F90, src/shared/readcl.F90): FN.Y (N x N, FP64 or
 ! complex128, column-major) and FN.J (N int64 signs) in; FN.YU, FN.ZZ (N x N)
-! and FN.SY (N FP64) out; two stdout lines; the per-step CSV on stderr.  The
-! HSVD itself runs on the GPU through libjkg.so (jkg_dsolve/jkg_zsolve).
+! and FN.SY (N FP64) out; two stdout lines; the per-step CSV on stderr, line by
+! line while the loop runs (the lines of each batch of steps the host polls
+! for together).  The HSVD itself runs on the GPU through libjkg.so.
 !
-! Deviations from the reference, all on its failure paths or in timing:
-!   * a failure ends with ERROR STOP (nonzero exit status; the reference's
-!     STOP exits with 0) -- the message text is the reference's;
-!   * the CSV is written after the loop, with the mean GPU step time in the
-!     BUILDs column and 0 in TRANSFs (the steps are not timed one by one);
-!   * ID_NCP=3 (best start offset) is limited to N <= 8192.
-! Environment: JKG_DEVICE (default 0), JKG_FUSED=1 selects FMAD = fma.
+! The solver (environment variable JKG_SOLVER):
+!   plain   (default) jkg_dsolve / jkg_zsolve: the reference's arithmetic and
+!           trajectory, bit for bit (YU, ZZ, SY and every CSV field but the
+!           two timing columns)
+!   mixed   jkg_dsolve_mixed / jkg_zsolve_mixed: Gram-form hyperbolic weights
+!           on the matrix cores and the relative convergence test
+!   learned the mixed solver with the trained step controller, whose
+!           JKG_CTL_NPARAM FP32 parameters are read from the file named by
+!           JKG_CTL_PARAMS (default: jkogb_amd/ctl_params.f32 next to this
+!           program's directory) -- real problems only
+!   warm    the mixed solver starting with FP32 steps (fp32_warm)
+! Other environment: JKG_DEVICE (default 0), JKG_FUSED=1 selects FMAD = fma.
+!
+! Deviations from the reference: a failure before the loop (arguments, files,
+! the device) ends with ERROR STOP (nonzero exit status; the reference's STOP
+! exits with 0) and the reference's message; BUILDs and TRANSFs are device
+! times; ID_NCP=3 (best start offset) is limited to N <= 8192.  As in the
+! reference, a loop that ends with an error (INFO < 0) prints its ERROR line and
+! still writes FN.YU, FN.ZZ and FN.SY.
 PROGRAM jk
   USE, INTRINSIC :: ISO_C_BINDING
   USE, INTRINSIC :: ISO_FORTRAN_ENV, ONLY: OUTPUT_UNIT, ERROR_UNIT
@@ -28,17 +41,18 @@ PROGRAM jk
   REAL(c_double), ALLOCATABLE, TARGET :: Y(:,:), YU(:,:), ZZ(:,:)
 #endif
   REAL(c_double), ALLOCATABLE :: SY(:)
+  REAL(c_float), ALLOCATABLE, TARGET :: CTL(:)
   INTEGER(c_int64_t), ALLOCATABLE :: J(:)
-  TYPE(jkg_steprec), ALLOCATABLE, TARGET :: TRACE(:)
   CHARACTER(LEN=252) :: FN
+  CHARACTER(LEN=1024) :: PATH
   INTEGER :: N, N_2, ID_NCP, ID_TRU, INFO, I, NP, NM, U1, U2, CAP
   INTEGER(c_int) :: RC, DEV
   INTEGER(c_int64_t) :: TT
   TYPE(c_ptr) :: CTX
   TYPE(jkg_opts) :: OPTS
+  TYPE(jkg_mixed_opts) :: MOPTS
   TYPE(jkg_result) :: RES
-  CHARACTER(LEN=32) :: ENVV
-  REAL(c_double) :: TSTEP
+  CHARACTER(LEN=32) :: ENVV, SOLVER
 
   CALL READCL(FN, N, N_2, ID_NCP, ID_TRU, INFO)
   IF (INFO .NE. 0) ERROR STOP USAGE
@@ -46,6 +60,12 @@ PROGRAM jk
   N_2 = jkg_jstep_len(INT(N,c_int), INT(N_2,c_int))
   IF (N_2 .LE. 0) ERROR STOP 'JSTEP_LEN'
   IF ((ID_NCP .LT. 0) .OR. (ID_NCP .GT. 3) .OR. (ID_TRU .LT. 0) .OR. (ID_TRU .GT. 2)) ERROR STOP 'DPROC_INIT'
+  SOLVER = 'plain'
+  CALL GET_ENVIRONMENT_VARIABLE('JKG_SOLVER', ENVV, STATUS=INFO)
+  IF (INFO .EQ. 0) SOLVER = ADJUSTL(ENVV)
+  IF ((SOLVER .NE. 'plain') .AND. (SOLVER .NE. 'mixed') .AND. (SOLVER .NE. 'learned') .AND. &
+      (SOLVER .NE. 'warm')) ERROR STOP 'JKG_SOLVER: plain, mixed, learned or warm'
+  IF ((SOLVER .NE. 'plain') .AND. (ID_NCP .EQ. 3)) ERROR STOP 'JKG_SOLVER: ID_NCP=3 needs the plain solver'
 
   ALLOCATE(Y(N,N), J(N))
   OPEN(NEWUNIT=U1, FILE=TRIM(FN)//'.Y', STATUS='OLD', ACTION='READ', ACCESS='STREAM', FORM='UNFORMATTED', &
@@ -96,33 +116,65 @@ PROGRAM jk
   OPTS%steps_per_sync = 0
   OPTS%trace_cap = CAP
   OPTS%trace = C_LOC(TRACE)
+  OPTS%progress = C_FUNLOC(on_progress)
+  OPTS%user = C_NULL_PTR
+  TRACE_UNIT = ERROR_UNIT
+  TRACE_PRINTED = 0
+
+  CALL jkg_mixed_opts_default(MOPTS)
+  IF (SOLVER .EQ. 'warm') MOPTS%fp32_warm = 1
+  IF (SOLVER .EQ. 'learned') THEN
+     CALL GET_ENVIRONMENT_VARIABLE('JKG_CTL_PARAMS', PATH, STATUS=INFO)
+     IF (INFO .NE. 0) THEN
+        CALL GET_COMMAND_ARGUMENT(0, PATH)
+        I = INDEX(PATH, '/', BACK=.TRUE.)
+        PATH = PATH(1:I)//'../jkogb_amd/ctl_params.f32'
+     END IF
+     ALLOCATE(CTL(JKG_CTL_NPARAM))
+     OPEN(NEWUNIT=U1, FILE=TRIM(PATH), STATUS='OLD', ACTION='READ', ACCESS='STREAM', FORM='UNFORMATTED', &
+          IOSTAT=INFO)
+     IF (INFO .EQ. 0) READ (U1, IOSTAT=INFO) CTL
+     IF (INFO .NE. 0) THEN
+        WRITE (ERROR_UNIT,'(A,I11,2A)') 'INFO=', INFO, ' ', TRIM(PATH)
+        ERROR STOP 'JKG_CTL_PARAMS'
+     END IF
+     CLOSE(U1)
+     MOPTS%controller = 1
+     MOPTS%ctl = C_LOC(CTL)
+  END IF
 
   RC = jkg_create(CTX, DEV)
   IF (RC .NE. JKG_OK) THEN
      WRITE (ERROR_UNIT,'(A,I11,2A)') 'INFO=', RC, ' ', TRIM(jkg_message(RC))
      ERROR STOP 'jkg_create'
   END IF
+  CALL write_trace_header(ERROR_UNIT)
 #ifdef ZJK
-  RC = jkg_zsolve(CTX, INT(N,c_int), INT(N_2,c_int), Y, J, YU, ZZ, SY, OPTS, RES)
+  IF (SOLVER .EQ. 'plain') THEN
+     RC = jkg_zsolve(CTX, INT(N,c_int), INT(N_2,c_int), Y, J, YU, ZZ, SY, OPTS, RES)
+  ELSE
+     RC = jkg_zsolve_mixed(CTX, INT(N,c_int), INT(N_2,c_int), Y, J, YU, ZZ, SY, OPTS, MOPTS, RES)
+  END IF
 #else
-  RC = jkg_dsolve(CTX, INT(N,c_int), INT(N_2,c_int), Y, J, YU, ZZ, SY, OPTS, RES)
+  IF (SOLVER .EQ. 'plain') THEN
+     RC = jkg_dsolve(CTX, INT(N,c_int), INT(N_2,c_int), Y, J, YU, ZZ, SY, OPTS, RES)
+  ELSE
+     RC = jkg_dsolve_mixed(CTX, INT(N,c_int), INT(N_2,c_int), Y, J, YU, ZZ, SY, OPTS, MOPTS, RES)
+  END IF
 #endif
   CALL jkg_destroy(CTX)
   IF (RC .NE. JKG_OK) THEN
      WRITE (ERROR_UNIT,'(A,I11,2A)') 'INFO=', RC, ' ', TRIM(jkg_message(RC))
      ERROR STOP 'DSTEP_LOOP'
   END IF
-  TSTEP = 0.0_c_double
-  IF (RES%steps .GT. 0) TSTEP = RES%seconds / RES%steps
-  CALL write_trace(ERROR_UNIT, TRACE, MIN(INT(RES%steps), CAP), TSTEP)
+  CALL write_trace_upto(MIN(INT(RES%steps), CAP))
   IF (RES%info .GE. 0) THEN
      WRITE (OUTPUT_UNIT,'(A,I11,A,F13.6,A)') 'Executed ', RES%info, ' steps with transformations in ', &
           RES%seconds, ' s'
      FLUSH(OUTPUT_UNIT)
-  ELSE
+  ELSE ! the loop's error: reported, and the factors written all the same (src/D/djk.F90:86-114)
      WRITE (ERROR_UNIT,'(A,I11,A,F13.6,A)') 'ERROR ', RES%info, ' after ', RES%seconds, ' s'
      FLUSH(ERROR_UNIT)
-     ERROR STOP
   END IF
 
   OPEN(NEWUNIT=U1, FILE=TRIM(FN)//'.YU', STATUS='REPLACE', ACTION='WRITE', ACCESS='STREAM', &

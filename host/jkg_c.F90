@@ -6,10 +6,12 @@ MODULE jkg_c
   IMPLICIT NONE
 
   INTEGER(c_int), PARAMETER :: JKG_OK = 0_c_int
+  INTEGER(c_int), PARAMETER :: JKG_CTL_NPARAM = 5378_c_int
 
   TYPE, BIND(C) :: jkg_steprec
      INTEGER(c_int32_t) :: step, oldlen, newlen, trig, hyp, it
      REAL(c_double) :: tw
+     REAL(c_float) :: build_s, transf_s
   END TYPE jkg_steprec
 
   TYPE, BIND(C) :: jkg_opts
@@ -24,6 +26,21 @@ MODULE jkg_c
      REAL(c_double) :: seconds
      INTEGER(c_int64_t) :: sum_sl, sum_it, sum_b2, sum_b4, sum_abod
   END TYPE jkg_result
+
+  ! jkg_mixed_opts: the options of the mixed solver (see include/jkg.h)
+  TYPE, BIND(C) :: jkg_mixed_opts
+     INTEGER(c_int32_t) :: weights, fp32_warm
+     REAL(c_double) :: switch_tol
+     INTEGER(c_int32_t) :: key_bits, controller
+     TYPE(c_ptr) :: ctl
+     REAL(c_double) :: threshold, tol
+     TYPE(c_ptr) :: ctl_trace
+     INTEGER(c_int32_t) :: ctl_trace_cap, ordering
+     TYPE(c_ptr) :: warm
+     REAL(c_double) :: ctl_tau_max
+     REAL(c_float) :: ctl_handoff
+     INTEGER(c_int32_t) :: reserved
+  END TYPE jkg_mixed_opts
 
   INTERFACE
      INTEGER(c_int) FUNCTION jkg_create(ctx, device) BIND(C, NAME='jkg_create')
@@ -47,6 +64,11 @@ MODULE jkg_c
        INTEGER(c_int), VALUE :: n, n2
      END FUNCTION jkg_jstep_len
 
+     SUBROUTINE jkg_mixed_opts_default(m) BIND(C, NAME='jkg_mixed_opts_default')
+       IMPORT :: jkg_mixed_opts
+       TYPE(jkg_mixed_opts), INTENT(OUT) :: m
+     END SUBROUTINE jkg_mixed_opts_default
+
      INTEGER(c_int) FUNCTION jkg_dsolve(ctx, n, n2, y, j, yu, zz, sy, opts, res) BIND(C, NAME='jkg_dsolve')
        IMPORT :: c_int, c_ptr, c_double, c_int64_t, jkg_opts, jkg_result
        TYPE(c_ptr), VALUE :: ctx
@@ -57,6 +79,19 @@ MODULE jkg_c
        TYPE(jkg_opts), INTENT(IN) :: opts
        TYPE(jkg_result), INTENT(OUT) :: res
      END FUNCTION jkg_dsolve
+
+     INTEGER(c_int) FUNCTION jkg_dsolve_mixed(ctx, n, n2, y, j, yu, zz, sy, opts, m, res) &
+          BIND(C, NAME='jkg_dsolve_mixed')
+       IMPORT :: c_int, c_ptr, c_double, c_int64_t, jkg_opts, jkg_mixed_opts, jkg_result
+       TYPE(c_ptr), VALUE :: ctx
+       INTEGER(c_int), VALUE :: n, n2
+       REAL(c_double), INTENT(IN) :: y(*)
+       INTEGER(c_int64_t), INTENT(IN) :: j(*)
+       REAL(c_double), INTENT(OUT) :: yu(*), zz(*), sy(*)
+       TYPE(jkg_opts), INTENT(IN) :: opts
+       TYPE(jkg_mixed_opts), INTENT(IN) :: m
+       TYPE(jkg_result), INTENT(OUT) :: res
+     END FUNCTION jkg_dsolve_mixed
 
      INTEGER(c_int) FUNCTION jkg_zsolve(ctx, n, n2, y, j, yu, zz, sy, opts, res) BIND(C, NAME='jkg_zsolve')
        IMPORT :: c_int, c_ptr, c_double, c_double_complex, c_int64_t, jkg_opts, jkg_result
@@ -69,7 +104,27 @@ MODULE jkg_c
        TYPE(jkg_opts), INTENT(IN) :: opts
        TYPE(jkg_result), INTENT(OUT) :: res
      END FUNCTION jkg_zsolve
+
+     INTEGER(c_int) FUNCTION jkg_zsolve_mixed(ctx, n, n2, y, j, yu, zz, sy, opts, m, res) &
+          BIND(C, NAME='jkg_zsolve_mixed')
+       IMPORT :: c_int, c_ptr, c_double, c_double_complex, c_int64_t, jkg_opts, jkg_mixed_opts, jkg_result
+       TYPE(c_ptr), VALUE :: ctx
+       INTEGER(c_int), VALUE :: n, n2
+       COMPLEX(c_double_complex), INTENT(IN) :: y(*)
+       INTEGER(c_int64_t), INTENT(IN) :: j(*)
+       COMPLEX(c_double_complex), INTENT(OUT) :: yu(*), zz(*)
+       REAL(c_double), INTENT(OUT) :: sy(*)
+       TYPE(jkg_opts), INTENT(IN) :: opts
+       TYPE(jkg_mixed_opts), INTENT(IN) :: m
+       TYPE(jkg_result), INTENT(OUT) :: res
+     END FUNCTION jkg_zsolve_mixed
   END INTERFACE
+
+  ! The per-step CSV of the reference (src/D/dstep.F90:388-440, 559-562), printed
+  ! as the steps come: the trace buffer the solve fills and the lines printed.
+  TYPE(jkg_steprec), ALLOCATABLE, TARGET :: TRACE(:)
+  INTEGER :: TRACE_PRINTED = 0
+  INTEGER :: TRACE_UNIT = 0
 
 CONTAINS
 
@@ -87,20 +142,32 @@ CONTAINS
     END DO
   END FUNCTION jkg_message
 
-  ! the per-step CSV of the reference (src/D/dstep.F90:388-440); the GPU
-  ! steps are timed as a whole, so BUILDs carries the mean step time and
-  ! TRANSFs 0
-  SUBROUTINE write_trace(u, rec, nrec, tstep)
-    INTEGER, INTENT(IN) :: u, nrec
-    TYPE(jkg_steprec), INTENT(IN) :: rec(*)
-    REAL(c_double), INTENT(IN) :: tstep
-    INTEGER :: k
+  ! the CSV header line of the reference (src/D/dstep.F90:559-562)
+  SUBROUTINE write_trace_header(u)
+    INTEGER, INTENT(IN) :: u
     WRITE (u,'(A)') '"STEP","OLDLEN","BUILDs","TRANSFs","NEWLEN","NORMF2D","TRIG","HYP"'
-    DO k = 1, nrec
-       WRITE (u,'(I11,A,I11,A,F12.6,A,F12.6,A,I11,A,ES25.17E3,2(A,I11))') rec(k)%step, ',', &
-            rec(k)%oldlen, ',', tstep, ',', 0.0_c_double, ',', rec(k)%newlen, ',', rec(k)%tw, ',', &
-            rec(k)%trig, ',', rec(k)%hyp
-    END DO
     FLUSH(u)
-  END SUBROUTINE write_trace
+  END SUBROUTINE write_trace_header
+
+  ! CSV lines of the records TRACE(TRACE_PRINTED+1 : upto), in the reference's
+  ! formats (src/D/dstep.F90:388-440); BUILDs and TRANSFs are the device times
+  SUBROUTINE write_trace_upto(upto)
+    INTEGER, INTENT(IN) :: upto
+    INTEGER :: k
+    DO k = TRACE_PRINTED + 1, MIN(upto, SIZE(TRACE))
+       WRITE (TRACE_UNIT,'(I11,A,I11,A,F12.6,A,F12.6,A,I11,A,ES25.17E3,2(A,I11))') TRACE(k)%step, ',', &
+            TRACE(k)%oldlen, ',', REAL(TRACE(k)%build_s, c_double), ',', REAL(TRACE(k)%transf_s, c_double), ',', &
+            TRACE(k)%newlen, ',', TRACE(k)%tw, ',', TRACE(k)%trig, ',', TRACE(k)%hyp
+       TRACE_PRINTED = k
+    END DO
+    FLUSH(TRACE_UNIT)
+  END SUBROUTINE write_trace_upto
+
+  ! progress hook of the solve: the library has copied the records of the
+  ! steps done so far into TRACE before calling it
+  SUBROUTINE on_progress(user, steps, done) BIND(C)
+    TYPE(c_ptr), VALUE :: user
+    INTEGER(c_int32_t), VALUE :: steps, done
+    CALL write_trace_upto(INT(steps))
+  END SUBROUTINE on_progress
 END MODULE jkg_c

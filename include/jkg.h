@@ -44,12 +44,15 @@ typedef struct {
   int32_t p, q, b, i;
 } jkg_aw;
 
-/* One line of the reference's per-step stderr CSV (src/D/dstep.F90:410-440)
- * minus the two timing columns: STEP, OLDLEN, NEWLEN, TRIG, HYP, NORMF2D;
- * it = number of finite weights (IT) of the step. */
+/* One line of the reference's per-step stderr CSV (src/D/dstep.F90:388-440):
+ * STEP, OLDLEN, NEWLEN, TRIG, HYP, NORMF2D (tw); it = number of finite weights
+ * (IT) of the step; build_s and transf_s = BUILDs and TRANSFs, the step's
+ * device wall time up to its 2x2 pivots and from there to its end (seconds,
+ * 100 MHz device clock). */
 typedef struct {
   int32_t step, oldlen, newlen, trig, hyp, it;
   double tw;
+  float build_s, transf_s;
 } jkg_steprec;
 
 typedef struct {
@@ -61,8 +64,10 @@ typedef struct {
   int32_t trace_cap;      /* records available in trace */
   jkg_steprec *trace;     /* host buffer for the per-step trace, or NULL */
   /* progress hook, or NULL: called on the calling thread at every host poll of the
-   * device (every steps_per_sync steps) with the index of the current step and
-   * whether the loop has ended; for long solves that report as they go */
+   * device (every steps_per_sync steps) with the number of steps executed so far
+   * and whether the loop has ended; trace[0 .. min(steps, trace_cap)) already
+   * holds their records, so a caller can print the CSV as the reference does
+   * (line by line during the loop); for long solves that report as they go */
   void (*progress)(void *user, int32_t step, int32_t done);
   void *user;
 } jkg_opts;
@@ -96,7 +101,9 @@ void jkg_destroy(jkg_ctx *ctx);
  * (src/D/djk.F90:27-121): JSTEP_LEN, R%TRU, DSTEP_LOOP (U=I, Z=I, steps until
  * no pivot or no effective transform), SY = diag(A), ZZ = J Z^T J.
  * y is N x N (ld = n), j holds N int64 signs; yu, zz (N x N) and sy (N) are
- * written only on success (JKG_OK and res->info >= 0).
+ * written whenever the loop ran (JKG_OK): after a failed 2x2 (res->info < 0)
+ * they hold U, J Z^T J and diag(A) at the failure, which the reference also
+ * writes after its ERROR line (src/D/djk.F90:86-114).
  * --------------------------------------------------------------------------- */
 int jkg_dsolve(jkg_ctx *ctx, int n, int n2, const double *y, const int64_t *j, double *yu, double *zz,
                double *sy, const jkg_opts *opts, jkg_result *res);
@@ -185,8 +192,8 @@ int jkg_dprofile(jkg_ctx *ctx, int n, int n2, const double *y, const int64_t *j,
 #define JKG_W_GRAM 1 /* Gram form (default) */
 typedef struct {
   int32_t weights;    /* JKG_W_PAIR or JKG_W_GRAM */
-  int32_t fp32_warm;  /* 1 (complex solver): FP32 steps until the off-norm falls below switch_tol or
-                         stalls, then FP64 (see jkg_warm_info) */
+  int32_t fp32_warm;  /* 1: FP32 steps until the off-norm falls below switch_tol or stalls, then
+                         FP64 (see jkg_warm_info); Gram-form weights only */
   double switch_tol;  /* relative off-norm that ends the FP32 phase (default 1e-5) */
   int32_t key_bits;   /* 64: full sort keys (default); 32: the warm start's keys cut to their top
                          32 bits, ties by pair index */
@@ -204,10 +211,12 @@ typedef struct {
                          transform -- the classic baseline (no controller or threshold) */
   struct jkg_warm_info *warm; /* out (fp32_warm): what the warm start did, or NULL */
   double ctl_tau_max; /* largest threshold a controller may set (0: 1e-6, the network's range) */
-  int32_t reserved[2];
+  float ctl_handoff;  /* log10(TW / TW_0) below which a controller hands every step back to the plain
+                         full ordering, in [-16, 0) (0: -10) */
+  int32_t reserved;
 } jkg_mixed_opts;
 
-/* The FP32 warm start of jkg_zsolve_mixed: steps on FP32 copies of A, U and Z
+/* The FP32 warm start of jkg_dsolve_mixed and jkg_zsolve_mixed: steps on FP32 copies of A, U and Z
  * (weights and 2x2 parameters in FP64) until A's relative off-norm
  * sqrt(off(A)^2 / |A|_F^2) falls below switch_tol (or has not improved by 1 %
  * in 64 steps); then U and Z are replaced by their polar factors in the
@@ -238,7 +247,7 @@ typedef struct jkg_warm_info {
  * Phase table t[8]: phase 0 while log10(TW/TW_0) > t[0], phase 1 while > t[1],
  * then phase 2; actions (N_2 fraction, log10 tau) in t[2..7].
  * Fallback (both): the plain full ordering on the first step and once TW has
- * fallen below 1e-10 TW_0.  A trace record holds the 16 features after the
+ * fallen below 10^ctl_handoff TW_0 (default 1e-10).  A trace record holds the 16 features after the
  * step, the step's N_2 fraction and log10 tau, its device time (us) and TW. */
 #define JKG_CTL_NPARAM 5378
 #define JKG_CTL_REC 20

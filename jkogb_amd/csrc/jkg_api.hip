@@ -35,6 +35,7 @@ struct jkg_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // buffers of the current problem size
   int n = 0, cplx = -1, trace_cap = 0;
+  int streamed = 0;  // trace records of the running solve already copied to the caller's buffer
   size_t sort_n = 0;
   std::vector<void *> allocs;
   DeviceStep d{};
@@ -420,7 +421,8 @@ double jkg_key_weight(uint64_t key) {
 
 // ---------------------------------------------------------------- problem setup
 static int upload_problem(jkg_ctx *c, int n, int n2, const void *y, const int64_t *j, int *npos, int *nneg,
-                          int max_steps, int trace_cap, double tau = 0.0, double tau_cap = 1e-6) {
+                          int max_steps, int trace_cap, double tau = 0.0, double tau_cap = 1e-6,
+                          float handoff = -10.f) {
   // (the element size follows the buffers: c->d.cplx)
   DeviceStep &d = c->d;
   std::vector<int> J32(n), pl, mi;
@@ -448,6 +450,7 @@ static int upload_problem(jkg_ctx *c, int n, int n2, const void *y, const int64_
   s.ltau = (tau > 0.0) ? (float)std::log10(tau) : -16.f;
   s.off_best = HUGE_VAL;
   s.ltau_cap = (float)std::log10(tau_cap);
+  s.lhandoff = handoff;
   {
     double y2 = 0.0;  // ||Y||_F^2 (the controller's scale)
     const double *yd = (const double *)y;
@@ -496,11 +499,18 @@ static int launch_until_done(jkg_ctx *c, const jkg_opts *o, long &remaining, lon
     launched += nb;
     if (remaining >= 0) remaining -= nb;
     if (remaining == 0) return JKG_OK;
-    int head[3] = {0, 0, 0};  // StepState: step, active, done
-    HIPCHK(hipMemcpyAsync(head, d.st, sizeof(head), hipMemcpyDeviceToHost, c->stream));
+    StepState hs;
+    HIPCHK(hipMemcpyAsync(&hs, d.st, sizeof(hs), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    const int done = head[2];
-    if (o->progress) o->progress(o->user, head[0], done);
+    const int done = hs.done;
+    // the trace records of the steps done so far, for a caller that prints them as they come
+    const int avail = std::min(hs.nrec, o->trace ? std::min(o->trace_cap, c->trace_cap) : 0);
+    if (avail > c->streamed) {
+      HIPCHK(hipMemcpy(o->trace + c->streamed, d.trace + c->streamed, sizeof(jkg_steprec) * (avail - c->streamed),
+                       hipMemcpyDeviceToHost));
+      c->streamed = avail;
+    }
+    if (o->progress) o->progress(o->user, hs.nrec, done);
     if (done) return JKG_OK;
     if (launched > cap) return JKG_ENOCONV;
     if (o->steps_per_sync <= 0 && batch < 64) batch *= 2;
@@ -555,7 +565,7 @@ static int run_loop(jkg_ctx *c, const jkg_opts *o, jkg_result *res) {
 // if the start is too far from the group (||.||_F >= 1/2) or the iteration
 // grows.
 static int polar_ns(jkg_ctx *c, void *X, const int *J, int *iters, double *res) {
-  const int n = c->d.n;
+  const int n = c->d.n, cx = c->d.cplx;
   hipStream_t s = c->stream;
   const int nb = ns_blocks(n);
   std::vector<double> part(nb);
@@ -564,12 +574,12 @@ static int polar_ns(jkg_ctx *c, void *X, const int *J, int *iters, double *res) 
   *res = HUGE_VAL;
   for (int it = 0; it < 12; ++it) {
     if (J) {
-      jrows_launch(n, X, J, c->wm_T, s);
-      zgemm_h_launch(n, X, c->wm_T, c->wm_M, s);  // X^H J X
+      jrows_launch(n, cx, X, J, c->wm_T, s);
+      gemm_h_launch(n, cx, X, c->wm_T, c->wm_M, s);  // X^H J X
     } else {
-      zgemm_h_launch(n, X, X, c->wm_M, s);  // X^H X
+      gemm_h_launch(n, cx, X, X, c->wm_M, s);  // X^H X
     }
-    ns_step_launch(n, c->wm_M, J, c->wm_T, c->wm_part, s);  // T = (3 I - S M) / 2
+    ns_step_launch(n, cx, c->wm_M, J, c->wm_T, c->wm_part, s);  // T = (3 I - S M) / 2
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(part.data(), c->wm_part, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -579,8 +589,8 @@ static int polar_ns(jkg_ctx *c, void *X, const int *J, int *iters, double *res) 
     *res = r;
     if (!(r < 0.5) || (it > 0 && r > prev)) return 0;
     if (r < 1e-14) break;  // already in the group to rounding
-    zadj_launch(n, X, c->wm_P, s);
-    zgemm_h_launch(n, c->wm_P, c->wm_T, X, s);  // (X^H)^H T = X T
+    adj_launch(n, cx, X, c->wm_P, s);
+    gemm_h_launch(n, cx, c->wm_P, c->wm_T, X, s);  // (X^H)^H T = X T
     HIPCHK(hipGetLastError());
     ++*iters;
     if (r < 1e-12 * n || r > 0.5 * prev) break;
@@ -601,6 +611,7 @@ struct MixedCfg {
   float *ctl_trace = nullptr;
   int ctl_trace_cap = 0;
   double tau_cap = 1e-6;
+  float handoff = -10.f;  // log10(TW / TW_0) of the controller's hand-back to the full ordering
   int cyclic = 0;  // round-robin sweeps in place of the dynamic ordering
   // FP32 warm start
   int warm = 0, cut32 = 0;
@@ -651,9 +662,9 @@ static int run_warm(jkg_ctx *c, const jkg_opts *o, jkg_result *res, const MixedC
   warm_promote_launch(d, c->stream);
   const int ok = polar_ns(c, d.U, nullptr, &wi.ns_u, &wi.res_u) && polar_ns(c, d.Z, d.J, &wi.ns_z, &wi.res_z);
   if (ok) {
-    zgemm_h_launch(d.n, d.U, d.A, c->wm_M, c->stream);  // U^H Y
-    zadj_launch(d.n, c->wm_M, c->wm_P, c->stream);
-    zgemm_h_launch(d.n, c->wm_P, d.Z, d.A, c->stream);  // (U^H Y) Z
+    gemm_h_launch(d.n, d.cplx, d.U, d.A, c->wm_M, c->stream);  // U^H Y
+    adj_launch(d.n, d.cplx, c->wm_M, c->wm_P, c->stream);
+    gemm_h_launch(d.n, d.cplx, c->wm_P, d.Z, d.A, c->stream);  // (U^H Y) Z
     s.step = s.nrec;
     s.done = 0;
     s.info = 0;
@@ -759,11 +770,12 @@ static int solve_common(jkg_ctx *c, int cplx, int n, int n2, const void *y, cons
   }
   // TT: the number of trigonometric pairs (the reference's R%TRU result)
   int npos = 0, nneg = 0;
-  rc = upload_problem(c, n, n2, y, j, &npos, &nneg, o.max_steps, c->trace_cap, tau, mc.tau_cap);
+  rc = upload_problem(c, n, n2, y, j, &npos, &nneg, o.max_steps, c->trace_cap, tau, mc.tau_cap, mc.handoff);
   if (rc) return rc;
   res->npos = npos;
   res->nneg = nneg;
   res->tt = (int)((int64_t)npos * (npos - 1) / 2 + (int64_t)nneg * (nneg - 1) / 2);
+  c->streamed = 0;
   rc = mc.warm ? run_warm(c, &o, res, mc) : run_loop(c, &o, res);
   if (rc) return rc;
   DeviceStep &d = c->d;
@@ -783,12 +795,12 @@ static int solve_common(jkg_ctx *c, int cplx, int n, int n2, const void *y, cons
     if (cnt > 0)
       HIPCHK(hipMemcpy(mc.ctl_trace, d.ctl_trace, sizeof(float) * CTL_REC * cnt, hipMemcpyDeviceToHost));
   }
-  if (res->info >= 0) {
-    const size_t esz = cplx ? 16 : 8;
-    HIPCHK(hipMemcpy(yu, d.U, esz * n * n, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(zz, d.Z, esz * n * n, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(sy, c->sigma, sizeof(double) * n, hipMemcpyDeviceToHost));
-  }
+  // written also after a failed step (res->info < 0): the loop's finalisation ran, and the
+  // reference writes U, Z and S after its ERROR line (src/D/djk.F90:86-114, dstep.F90:594-603)
+  const size_t esz = cplx ? 16 : 8;
+  HIPCHK(hipMemcpy(yu, d.U, esz * n * n, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(zz, d.Z, esz * n * n, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(sy, c->sigma, sizeof(double) * n, hipMemcpyDeviceToHost));
   return JKG_OK;
 }
 
@@ -815,11 +827,18 @@ extern "C" int jkg_dsolve_mixed(jkg_ctx *c, int n, int n2, const double *y, cons
   if (!(m.tol >= 0.0) || !(m.threshold >= 0.0 && m.threshold < 1.0)) return -10;
   if (m.controller < 0 || m.controller > 2 || (m.controller && !m.ctl) || m.ctl_trace_cap < 0) return -10;
   if (!(m.ctl_tau_max >= 0.0 && m.ctl_tau_max <= 1e-6)) return -10;
+  if (!(m.ctl_handoff >= -16.f && m.ctl_handoff <= 0.f)) return -10;
   if (m.ordering < 0 || m.ordering > 1) return -10;
-  if (m.fp32_warm != 0 || m.key_bits != 64) return JKG_ENOTSUP;
-  if (m.weights == JKG_W_PAIR && (m.tol != 0.0 || m.controller)) return JKG_ENOTSUP;
-  if (m.ordering == 1 && (m.controller || m.threshold != 0.0)) return JKG_ENOTSUP;
+  if (m.fp32_warm < 0 || m.fp32_warm > 1 || (m.key_bits != 32 && m.key_bits != 64)) return -10;
+  if (m.fp32_warm && !(m.switch_tol > 0.0 && m.switch_tol < 1.0)) return -10;
+  if (m.key_bits == 32 && !m.fp32_warm) return JKG_ENOTSUP;  // 32-bit keys: the warm start only
+  if (m.weights == JKG_W_PAIR && (m.tol != 0.0 || m.controller || m.fp32_warm)) return JKG_ENOTSUP;
+  if (m.ordering == 1 && (m.controller || m.threshold != 0.0 || m.fp32_warm)) return JKG_ENOTSUP;
   MixedCfg mc;
+  mc.warm = m.fp32_warm;
+  mc.cut32 = m.key_bits == 32;
+  mc.switch_tol = m.switch_tol;
+  mc.wi = m.warm;
   mc.cyclic = m.ordering;
   mc.wmode = m.weights == JKG_W_GRAM ? 1 : 0;
   mc.tol = m.tol;
@@ -829,6 +848,7 @@ extern "C" int jkg_dsolve_mixed(jkg_ctx *c, int n, int n2, const double *y, cons
   mc.ctl_trace = m.ctl_trace;
   mc.ctl_trace_cap = m.ctl_trace ? m.ctl_trace_cap : 0;
   mc.tau_cap = m.ctl_tau_max > 0.0 ? m.ctl_tau_max : 1e-6;
+  mc.handoff = m.ctl_handoff != 0.f ? m.ctl_handoff : -10.f;
   return solve_common(c, 0, n, n2, y, j, yu, zz, sy, opts, res, mc);
 }
 

@@ -158,6 +158,7 @@ struct fz {
   float re, im;
 };
 __device__ __forceinline__ dz ldz(const double &a) { return dz{a, 0.0}; }
+__device__ __forceinline__ dz ldz(const float &a) { return dz{(double)a, 0.0}; }
 __device__ __forceinline__ dz ldz(const dz &a) { return a; }
 __device__ __forceinline__ dz ldz(const fz &a) { return dz{(double)a.re, (double)a.im}; }
 template <class E>

@@ -137,6 +137,7 @@ void frame_launch(DeviceStep &d, hipStream_t s) {
   };
   if (d.cplx && d.prec32) go(k_frame<fz>, (const fz *)d.A32);
   else if (d.cplx) go(k_frame<dz>, (const dz *)d.A);
+  else if (d.prec32) go(k_frame<float>, (const float *)d.A32);
   else go(k_frame<double>, (const double *)d.A);
 }
 

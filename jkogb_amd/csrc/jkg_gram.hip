@@ -190,10 +190,11 @@ void gemm_tn_launch(int k, const double *X, int ldx, const int *xcols, int m, co
   else go(zero_diag ? k_gemm_tn<true, 0, double> : k_gemm_tn<false, 0, double>);
 }
 
-// The complex Gram product of the FP32 warm start: FP32 entries, FP64 products and sums.
-// The warm start's complex Gram product on the FP32 matrix cores (twice the FP64
-// rate): the tiling of k_gemm_tn with FP32 operands in LDS and FP32 sums over
-// each K chunk; the chunks' planes are added in FP64 by the weight kernel.
+// The warm start's Gram product on the FP32 matrix cores (twice the FP64 rate):
+// the tiling of k_gemm_tn with FP32 operands in LDS and FP32 sums over each K
+// chunk; the chunks' planes are added in FP64 by the weight kernel.  CX = 0: real
+// columns (the diagonal of O is row c of column c); CX = 1, 2: complex columns
+// read as interleaved rows, as in k_gemm_tn.
 // MFMA f32 16x16x4 lane layout: A[i = l % 16][k = l / 16], B[k = l / 16][j =
 // l % 16], D item t: [i = 4 (l / 16) + t][j = l % 16].
 typedef float f4v __attribute__((ext_vector_type(4)));
@@ -216,6 +217,7 @@ __global__ void __launch_bounds__(256) k_zgram_f32(int k, int kchunk, long plane
   const int xi = i0 + lc, yj = j0 + lc;
   const int xcol = (xi < m) ? xcols[xi] : -1;
   const int ycol = (yj < nc) ? ycols[yj] : -1;
+  constexpr int DS = CX ? 1 : 0;  // flat row r belongs to matrix row r >> DS
   const float *xp = X + (size_t)(xcol < 0 ? 0 : xcol) * ldx;
   const float *yp = X + (size_t)(ycol < 0 ? 0 : ycol) * ldx;
   float xr[8], yr[8];
@@ -245,8 +247,8 @@ __global__ void __launch_bounds__(256) k_zgram_f32(int k, int kchunk, long plane
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int r = rb + u;
-        xr[u] = (xcol >= 0 && r < ke && (r >> 1) != xcol) ? xp[r] : 0.0f;
-        yr[u] = (ycol >= 0 && r < ke && (r >> 1) != ycol) ? yp[r] : 0.0f;
+        xr[u] = (xcol >= 0 && r < ke && (r >> DS) != xcol) ? xp[r] : 0.0f;
+        yr[u] = (ycol >= 0 && r < ke && (r >> DS) != ycol) ? yp[r] : 0.0f;
       }
     }
   };
@@ -256,8 +258,8 @@ __global__ void __launch_bounds__(256) k_zgram_f32(int k, int kchunk, long plane
     for (int u = 0; u < 8; ++u) {
       float x = xr[u], y = yr[u];
       if (raw) {
-        x = (xcol < 0 || ((rb + u) >> 1) == xcol) ? 0.0f : x;
-        y = (ycol < 0 || ((rb + u) >> 1) == ycol) ? 0.0f : y;
+        x = (xcol < 0 || ((rb + u) >> DS) == xcol) ? 0.0f : x;
+        y = (ycol < 0 || ((rb + u) >> DS) == ycol) ? 0.0f : y;
       }
       yr[u] = y;
       Xs[buf][lh + u][lc] = x;
@@ -316,23 +318,25 @@ static void zgemm_tn_f32_launch(int k, const float *X, int ldx, const int *xcols
                                 double *C, long csi, long csj, const StepState *st, hipStream_t s, int cx, int ks) {
   if (m <= 0 || nc <= 0) return;
   dim3 grid((nc + GB_N - 1) / GB_N, (m + GB_M - 1) / GB_M, ks);
-  auto kern = (cx == 1) ? k_zgram_f32<1> : k_zgram_f32<2>;
+  auto kern = (cx == 1) ? k_zgram_f32<1> : ((cx == 2) ? k_zgram_f32<2> : k_zgram_f32<0>);
   hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, k, kchunk_of(k, ks), (long)m * nc, X, ldx, xcols, m, ycols, nc, C,
                      csi, csj, st);
 }
 
 // ------------------------------------------------------------------ column norms
-// c[col] = sum_{r != col} A[r, col]^2 and dg[col] = A[col, col], one wave per column.
-__global__ void __launch_bounds__(256) k_colnorm_off(int n, const double *__restrict__ A, int lda,
+// c[col] = sum_{r != col} A[r, col]^2 and dg[col] = A[col, col], one wave per column
+// (E = float: the FP32 warm start's A, promoted).
+template <class E>
+__global__ void __launch_bounds__(256) k_colnorm_off(int n, const E *__restrict__ A, int lda,
                                                      double *__restrict__ c, double *__restrict__ dg,
                                                      const StepState *st) {
   if (!st->active) return;
   const int col = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (col >= n) return;
-  const double *a = A + (size_t)col * lda;
+  const E *a = A + (size_t)col * lda;
   double s = 0.0;
   for (int r = lane; r < n; r += 64) {
-    const double v = a[r];
+    const double v = (double)a[r];
     if (r == col) dg[col] = v;
     else s = __builtin_fma(v, v, s);
   }
@@ -349,15 +353,15 @@ __global__ void __launch_bounds__(256) k_colnorm_off(int n, const double *__rest
 // ABODNDF2 sum is replaced by the quadratic form above.
 constexpr int WT = 64;
 
-template <bool F>
-__global__ void __launch_bounds__(256) k_hyp_weights_gram(int n, const double *__restrict__ A, int lda,
+template <bool F, class E>
+__global__ void __launch_bounds__(256) k_hyp_weights_gram(int n, const E *__restrict__ A, int lda,
                                                           const int *__restrict__ J, const int *__restrict__ plus,
                                                           int npl, const int *__restrict__ minus, int nmi,
                                                           const double *__restrict__ G, int ldg, int gks,
                                                           const double *__restrict__ coln,
                                                           const double *__restrict__ dg,
                                                           uint64_t *__restrict__ keys, uint32_t *__restrict__ vals,
-                                                          StepState *st, double tol) {
+                                                          StepState *st, double tol, int cut) {
   if (!st->active) return;
   __shared__ double E1[WT][WT + 1];  // E1[a][b] = A(minus b, plus a)
   __shared__ double E2[WT][WT + 1];  // E2[b][a] = A(plus a, minus b)
@@ -369,9 +373,9 @@ __global__ void __launch_bounds__(256) k_hyp_weights_gram(int n, const double *_
   for (int e = tid; e < WT * WT; e += 256) {
     const int hi = e / WT, lo = e % WT;
     const int c1 = pc[hi], r1 = mc[lo];  // column plus hi, row minus lo
-    E1[hi][lo] = (c1 >= 0 && r1 >= 0) ? A[(size_t)c1 * lda + r1] : 0.0;
+    E1[hi][lo] = (c1 >= 0 && r1 >= 0) ? (double)A[(size_t)c1 * lda + r1] : 0.0;
     const int c2 = mc[hi], r2 = pc[lo];  // column minus hi, row plus lo
-    E2[hi][lo] = (c2 >= 0 && r2 >= 0) ? A[(size_t)c2 * lda + r2] : 0.0;
+    E2[hi][lo] = (c2 >= 0 && r2 >= 0) ? (double)A[(size_t)c2 * lda + r2] : 0.0;
   }
   __syncthreads();
   unsigned nfull = 0;
@@ -409,7 +413,7 @@ __global__ void __launch_bounds__(256) k_hyp_weights_gram(int n, const double *_
       }
     }
     const uint32_t k = pair_index(n, p0 + 1, q0 + 1);
-    keys[k] = key;
+    keys[k] = key_cut(key, cut);
     vals[k] = pack_pq(p0, q0);
   }
   // work counter: pairs whose weight needed the column sums
@@ -417,21 +421,39 @@ __global__ void __launch_bounds__(256) k_hyp_weights_gram(int n, const double *_
   if ((tid & 63) == 0 && nfull) atomicAdd((unsigned long long *)&st->acc_abod, (unsigned long long)nfull);
 }
 
+static void zgemm_tn_f32_launch(int k, const float *X, int ldx, const int *xcols, int m, const int *ycols, int nc,
+                                double *C, long csi, long csj, const StepState *st, hipStream_t s, int cx, int ks);
+template <class DG>
+__global__ void k_warm_offnorm(int n, const double *__restrict__ c, const DG *__restrict__ dg, StepState *st,
+                               double switch_tol);
+
 void gram_hyp_weights_launch(DeviceStep &d, hipStream_t s) {
+  const int ks = gram_ksplit(d.npl, d.nmi, d.n);
+  dim3 g((d.nmi + WT - 1) / WT, (d.npl + WT - 1) / WT);
+  auto weights = [&](auto kern, const auto *A, int cut) {
+    hipLaunchKernelGGL(kern, g, dim3(256), 0, s, d.n, A, d.lda, d.J, d.plus, d.npl, d.minus, d.nmi, d.gram, d.nmi,
+                       ks, d.coln, d.coln + d.n, d.sort.k0, d.sort.v0, d.st, d.neg_tol, cut);
+  };
+  if (d.prec32) {  // the FP32 warm start: column norms, its end test, the FP32 Gram product
+    const float *A = (const float *)d.A32;
+    hipLaunchKernelGGL(k_colnorm_off<float>, dim3((d.n + 3) / 4), dim3(256), 0, s, d.n, A, d.lda, d.coln,
+                       d.coln + d.n, d.st);
+    hipLaunchKernelGGL(k_warm_offnorm<double>, dim3(1), dim3(256), 0, s, d.n, d.coln, d.coln + d.n, d.st,
+                       d.switch_tol);
+    if (d.npl <= 0 || d.nmi <= 0) return;
+    zgemm_tn_f32_launch(d.n, A, d.lda, d.plus, d.npl, d.minus, d.nmi, d.gram, (long)d.nmi, 1L, d.st, s, 0, ks);
+    if (d.fused) weights(k_hyp_weights_gram<true, float>, A, d.cut32);
+    else weights(k_hyp_weights_gram<false, float>, A, d.cut32);
+    return;
+  }
   if (d.npl <= 0 || d.nmi <= 0) return;
   const double *A = (const double *)d.A;
-  hipLaunchKernelGGL(k_colnorm_off, dim3((d.n + 3) / 4), dim3(256), 0, s, d.n, A, d.lda, d.coln, d.coln + d.n,
-                     d.st);
-  const int ks = gram_ksplit(d.npl, d.nmi, d.n);
+  hipLaunchKernelGGL(k_colnorm_off<double>, dim3((d.n + 3) / 4), dim3(256), 0, s, d.n, A, d.lda, d.coln,
+                     d.coln + d.n, d.st);
   gemm_tn_launch(d.n, A, d.lda, d.plus, d.npl, A, d.lda, d.minus, d.nmi, d.gram, (long)d.nmi, 1L, true, d.st, s, 0,
                  ks);
-  dim3 g((d.nmi + WT - 1) / WT, (d.npl + WT - 1) / WT);
-  if (d.fused)
-    hipLaunchKernelGGL(k_hyp_weights_gram<true>, g, dim3(256), 0, s, d.n, A, d.lda, d.J, d.plus, d.npl, d.minus,
-                       d.nmi, d.gram, d.nmi, ks, d.coln, d.coln + d.n, d.sort.k0, d.sort.v0, d.st, d.neg_tol);
-  else
-    hipLaunchKernelGGL(k_hyp_weights_gram<false>, g, dim3(256), 0, s, d.n, A, d.lda, d.J, d.plus, d.npl, d.minus,
-                       d.nmi, d.gram, d.nmi, ks, d.coln, d.coln + d.n, d.sort.k0, d.sort.v0, d.st, d.neg_tol);
+  if (d.fused) weights(k_hyp_weights_gram<true, double>, A, 0);
+  else weights(k_hyp_weights_gram<false, double>, A, 0);
 }
 
 // ------------------------------------------------------------------ complex
@@ -468,14 +490,17 @@ __global__ void __launch_bounds__(256) k_zcolnorm_off(int n, const E *__restrict
 // sqrt(sum c / (sum c + sum |a_ii|^2)) of this step's A; below switch_tol, or
 // no 1 % improvement in WARM_STALL steps, ends the FP32 phase before this step
 // (the rest of the step is skipped; the loop reports done).
-__global__ void __launch_bounds__(256) k_warm_offnorm(int n, const double *__restrict__ c,
-                                                      const dz *__restrict__ dg, StepState *st, double switch_tol) {
+__device__ __forceinline__ double sq_acc(double v, double b) { return __builtin_fma(v, v, b); }
+__device__ __forceinline__ double sq_acc(dz v, double b) { return __builtin_fma(v.re, v.re, __builtin_fma(v.im, v.im, b)); }
+template <class DG>
+__global__ void __launch_bounds__(256) k_warm_offnorm(int n, const double *__restrict__ c, const DG *__restrict__ dg,
+                                                      StepState *st, double switch_tol) {
   if (!st->active) return;
   __shared__ double so[256], sd[256];
   double a = 0.0, b = 0.0;
   for (int i = threadIdx.x; i < n; i += 256) {
     a += c[i];
-    b = __builtin_fma(dg[i].re, dg[i].re, __builtin_fma(dg[i].im, dg[i].im, b));
+    b = sq_acc(dg[i], b);
   }
   so[threadIdx.x] = a;
   sd[threadIdx.x] = b;
@@ -604,7 +629,7 @@ void zgram_hyp_weights_launch(DeviceStep &d, hipStream_t s) {
   if (d.prec32) {
     const fz *A = (const fz *)d.A32;
     hipLaunchKernelGGL(k_zcolnorm_off<fz>, dim3((d.n + 3) / 4), dim3(256), 0, s, d.n, A, d.lda, d.coln, dg, d.st);
-    hipLaunchKernelGGL(k_warm_offnorm, dim3(1), dim3(256), 0, s, d.n, d.coln, dg, d.st, d.switch_tol);
+    hipLaunchKernelGGL(k_warm_offnorm<dz>, dim3(1), dim3(256), 0, s, d.n, d.coln, dg, d.st, d.switch_tol);
     if (d.npl <= 0 || d.nmi <= 0) return;
     const float *X = (const float *)d.A32;
     zgemm_tn_f32_launch(2 * d.n, X, 2 * d.lda, d.plus, d.npl, d.minus, d.nmi, Gre, (long)d.nmi, 1L, d.st, s, 1, ks);

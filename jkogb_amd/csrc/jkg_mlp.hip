@@ -200,8 +200,8 @@ __global__ void k_mlp_adam(float *__restrict__ master, uint16_t *__restrict__ q,
 //   mode 2: a phase table t[8]: phases by log10(TW / TW_0) against t[0] > t[1],
 //           actions (N_2 fraction, log10 tau) in t[2..7].
 // The plain full ordering (all of N_2, no threshold) is the fallback: on the
-// first step, once TW has fallen below 1e-10 TW_0, and for outputs that are
-// not finite.  log10 tau <= -15.5 means no threshold.
+// first step, once TW has fallen below 10^lhandoff TW_0 (st->lhandoff, 1e-10 by
+// default), and for outputs that are not finite.  log10 tau <= -15.5 means no threshold.
 __global__ void __launch_bounds__(64) k_controller(StepState *st, int mode, const float *__restrict__ master,
                                                    const uint16_t *__restrict__ q, const float *__restrict__ table) {
   __shared__ float h1[MLP_H], h2[MLP_H];
@@ -226,7 +226,7 @@ __global__ void __launch_bounds__(64) k_controller(StepState *st, int mode, cons
       for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off);  // same tree in every lane
       y[o] = a + master[MLP_B3 + o];
     }
-    if (y[0] == y[0] && y[1] == y[1] && lrel > -10.f) {
+    if (y[0] == y[0] && y[1] == y[1] && lrel > st->lhandoff) {
       const float y0 = y[0] < 0.f ? 0.f : (y[0] > 1.f ? 1.f : y[0]);
       const float y1 = y[1] < 0.f ? 0.f : (y[1] > 1.f ? 1.f : y[1]);
       n2f = 0.75f + 0.25f * y0;
@@ -236,7 +236,7 @@ __global__ void __launch_bounds__(64) k_controller(StepState *st, int mode, cons
     const int ph = first ? 0 : (lrel > table[0] ? 0 : (lrel > table[1] ? 1 : 2));
     n2f = table[2 + 2 * ph];
     ltau = table[3 + 2 * ph];
-    if (!first && lrel <= -10.f) {
+    if (!first && lrel <= st->lhandoff) {
       n2f = 1.f;
       ltau = -16.f;
     }

@@ -39,4 +39,23 @@ __device__ __forceinline__ Rot rot_ab(const double B[4]) {
   return R;
 }
 
+// The same transform on FP32 data (the FP32 warm start): the descriptor formed in
+// FP64 and rounded once, the products in FP32.
+struct RotF {
+  float r1, r2, m1, m2;
+  int sa, sb;
+};
+template <bool F>
+__device__ __forceinline__ void rot_apply(const RotF &R, float &x, float &y) {
+  float xx = fmadf<F>(R.sa ? y : x, R.r1, R.sa ? x : y) * R.m1;
+  float yy = fmadf<F>(R.sb ? x : y, R.r2, R.sb ? y : x) * R.m2;
+  x = xx;
+  y = yy;
+}
+// the descriptor in the precision of the data it is applied to
+__device__ __forceinline__ const Rot &as_rot(const Rot &R, const double *) { return R; }
+__device__ __forceinline__ RotF as_rot(const Rot &R, const float *) {
+  return RotF{(float)R.r1, (float)R.r2, (float)R.m1, (float)R.m2, R.sa, R.sb};
+}
+
 }  // namespace jkg

@@ -69,12 +69,13 @@ __global__ void k_step_begin(StepState *st, int nn, int n, int *colpiv) {
 // column loads, and the pairs are handed to the lanes along the tile's
 // diagonals (q - p fixed within 32 lanes), where consecutive lanes have
 // consecutive pair indices, so the keys and values leave as contiguous runs.
+// E = float: the FP32 warm start (entries promoted; keys cut to their top 32 bits when cut).
 constexpr int TWT = 32;
-template <bool F>
-__global__ void __launch_bounds__(256) k_trig_weights(int n, const double *__restrict__ A, int lda,
+template <bool F, class E>
+__global__ void __launch_bounds__(256) k_trig_weights(int n, const E *__restrict__ A, int lda,
                                                        const int *__restrict__ J, uint64_t *__restrict__ keys,
                                                        uint32_t *__restrict__ vals, const StepState *st,
-                                                       double tol) {
+                                                       double tol, int cut) {
   const int bp = blockIdx.y, bq = blockIdx.x;
   if (bp > bq || !st->active) return;
   __shared__ double X[TWT][TWT + 1];  // X[qi][pi] = A(Q0 + qi, P0 + pi)
@@ -84,16 +85,16 @@ __global__ void __launch_bounds__(256) k_trig_weights(int n, const double *__res
   const int P0 = bp * TWT, Q0 = bq * TWT, t = threadIdx.x;
   for (int e = t; e < TWT * TWT; e += 256) {
     const int r = e % TWT, c = e / TWT;
-    X[r][c] = (P0 + c < n && Q0 + r < n) ? A[(size_t)(P0 + c) * lda + Q0 + r] : 0.0;
-    Y[r][c] = (P0 + r < n && Q0 + c < n) ? A[(size_t)(Q0 + c) * lda + P0 + r] : 0.0;
+    X[r][c] = (P0 + c < n && Q0 + r < n) ? (double)A[(size_t)(P0 + c) * lda + Q0 + r] : 0.0;
+    Y[r][c] = (P0 + r < n && Q0 + c < n) ? (double)A[(size_t)(Q0 + c) * lda + P0 + r] : 0.0;
   }
   if (t < TWT) {
     const int P = P0 + t;
-    dgp[t] = (P < n) ? A[(size_t)P * lda + P] : 0.0;
+    dgp[t] = (P < n) ? (double)A[(size_t)P * lda + P] : 0.0;
     jp[t] = (P < n) ? J[P] : 0;
   } else if (t < 2 * TWT) {
     const int Q = Q0 + t - TWT;
-    dgq[t - TWT] = (Q < n) ? A[(size_t)Q * lda + Q] : 0.0;
+    dgq[t - TWT] = (Q < n) ? (double)A[(size_t)Q * lda + Q] : 0.0;
     jq[t - TWT] = (Q < n) ? J[Q] : 0;
   }
   __syncthreads();
@@ -109,7 +110,7 @@ __global__ void __launch_bounds__(256) k_trig_weights(int n, const double *__res
     bool go = (aqp != 0.0) || (apq != 0.0) || order;
     if (tol > 0.0 && go && !order && negligible(aqp, apq, app, aqq, tol)) go = false;
     const uint32_t k = pair_index(n, P + 1, Q + 1);
-    keys[k] = go ? weight_key(dasum2<F>(aqp, apq)) : KEY_NAN;
+    keys[k] = key_cut(go ? weight_key(dasum2<F>(aqp, apq)) : KEY_NAN, cut);
     vals[k] = pack_pq(P, Q);
   }
 }
@@ -597,8 +598,8 @@ __global__ void __launch_bounds__(FT_WAVES * 64) k_sel_write(int n, const uint64
 }
 
 // ------------------------------------------------------------------ (d) pivots
-template <bool F>
-__global__ void __launch_bounds__(256) k_pivots(int n, const double *__restrict__ A, int lda,
+template <bool F, class E>
+__global__ void __launch_bounds__(256) k_pivots(int n, const E *__restrict__ A, int lda,
                                                  const int *__restrict__ J, const int *__restrict__ piv_p,
                                                  const int *__restrict__ piv_q, const int *__restrict__ piv_pos,
                                                  const uint64_t *k0, const uint64_t *k1, const uint32_t *v0,
@@ -609,10 +610,11 @@ __global__ void __launch_bounds__(256) k_pivots(int n, const double *__restrict_
   if (!st->active) return;
   const int sl = st->sl;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) st->t_transf = wall_clock64();
   if (i >= sl) return;
   const int p = piv_p[i], q = piv_q[i];
   const size_t P = p, Q = q;
-  double b[4] = {A[P * lda + P], A[P * lda + Q], A[Q * lda + P], A[Q * lda + Q]};
+  double b[4] = {(double)A[P * lda + P], (double)A[P * lda + Q], (double)A[Q * lda + P], (double)A[Q * lda + Q]};
   int k[2] = {J[p], J[q]};
   const int pos = piv_pos[i];
   if (pos > 0) {
@@ -657,14 +659,16 @@ __global__ void __launch_bounds__(256) k_pivots(int n, const double *__restrict_
 }
 
 // ------------------------------------------------------------------ (e) rotate
-template <bool F>
-__global__ void __launch_bounds__(1024) k_rotate(int n, double *__restrict__ A, int lda, double *__restrict__ U,
-                                                double *__restrict__ Z, const int *__restrict__ piv_p,
+// E = float: the FP32 warm start (descriptors rounded once per pivot, products in FP32).
+template <bool F, class E>
+__global__ void __launch_bounds__(1024) k_rotate(int n, E *__restrict__ A, int lda, E *__restrict__ U,
+                                                E *__restrict__ Z, const int *__restrict__ piv_p,
                                                 const int *__restrict__ piv_q, const int *__restrict__ piv_info,
                                                 const double *__restrict__ piv_d, const Rot *__restrict__ rot_row,
                                                 const Rot *__restrict__ rot_col, const int *__restrict__ colpiv,
                                                 const StepState *st) {
-  extern __shared__ double L[];  // column c in L[0..n), its partner in L[n..2n)
+  extern __shared__ __attribute__((aligned(16))) unsigned char l_raw[];
+  E *L = (E *)l_raw;  // column c in L[0..n), its partner in L[n..2n)
   if (!st->active) return;
   const int sl = st->sl;
   if (sl <= 0) return;
@@ -678,8 +682,8 @@ __global__ void __launch_bounds__(1024) k_rotate(int n, double *__restrict__ A, 
   const bool diag_set = (j >= 0) && err == 0 && info >= 0 && !(info & 8);
   const int tid = threadIdx.x, bs = blockDim.x;
   if (nrow > 0 || col_rot || diag_set) {
-    double *col0 = A + (size_t)c * lda;
-    double *col1 = (c1 >= 0) ? A + (size_t)c1 * lda : nullptr;
+    E *col0 = A + (size_t)c * lda;
+    E *col1 = (c1 >= 0) ? A + (size_t)c1 * lda : nullptr;
     for (int r = tid; r < n; r += bs) {
       L[r] = col0[r];
       if (col1) L[n + r] = col1[r];
@@ -690,9 +694,9 @@ __global__ void __launch_bounds__(1024) k_rotate(int n, double *__restrict__ A, 
       for (int i = tid; i < sl; i += bs) {
         const int inf = piv_info[i];
         if (inf < 0 || !(inf & 2)) continue;
-        const Rot R = rot_row[i];
+        const auto R = as_rot(rot_row[i], L);
         const int p = piv_p[i], q = piv_q[i];
-        double x = L[p], y = L[q];
+        E x = L[p], y = L[q];
         rot_apply<F>(R, x, y);
         L[p] = x;
         L[q] = y;
@@ -709,15 +713,14 @@ __global__ void __launch_bounds__(1024) k_rotate(int n, double *__restrict__ A, 
     if (col1) {
       // phase 2 on A: the column transform and the new 2x2 diagonal
       const int p = c, q = c1;
-      Rot R;
-      if (col_rot) R = rot_col[j];
-      const double d0 = diag_set ? piv_d[2 * j] : 0.0, d1 = diag_set ? piv_d[2 * j + 1] : 0.0;
+      const auto R = as_rot(col_rot ? rot_col[j] : Rot{}, L);
+      const E d0 = (E)(diag_set ? piv_d[2 * j] : 0.0), d1 = (E)(diag_set ? piv_d[2 * j + 1] : 0.0);
       for (int r = tid; r < n; r += bs) {
-        double x = L[r], y = L[n + r];
+        E x = L[r], y = L[n + r];
         if (col_rot) rot_apply<F>(R, x, y);
         if (diag_set) {
-          if (r == p) { x = d0; y = 0.0; }
-          if (r == q) { x = 0.0; y = d1; }
+          if (r == p) { x = d0; y = (E)0; }
+          if (r == q) { x = (E)0; y = d1; }
         }
         col0[r] = x;
         col1[r] = y;
@@ -729,20 +732,20 @@ __global__ void __launch_bounds__(1024) k_rotate(int n, double *__restrict__ A, 
   // phase 1 on U (columns p,q times V^T) and Z (columns p,q times W)
   if (j >= 0 && info >= 0) {
     if (info & 2) {
-      const Rot R = rot_row[j];
-      double *u0 = U + (size_t)c * lda, *u1 = U + (size_t)c1 * lda;
+      const auto R = as_rot(rot_row[j], U);
+      E *u0 = U + (size_t)c * lda, *u1 = U + (size_t)c1 * lda;
       for (int r = tid; r < n; r += bs) {
-        double x = u0[r], y = u1[r];
+        E x = u0[r], y = u1[r];
         rot_apply<F>(R, x, y);
         u0[r] = x;
         u1[r] = y;
       }
     }
     if (info & 4) {
-      const Rot R = rot_col[j];
-      double *z0 = Z + (size_t)c * lda, *z1 = Z + (size_t)c1 * lda;
+      const auto R = as_rot(rot_col[j], Z);
+      E *z0 = Z + (size_t)c * lda, *z1 = Z + (size_t)c1 * lda;
       for (int r = tid; r < n; r += bs) {
-        double x = z0[r], y = z1[r];
+        E x = z0[r], y = z1[r];
         rot_apply<F>(R, x, y);
         z0[r] = x;
         z1[r] = y;
@@ -821,6 +824,10 @@ __global__ void __launch_bounds__(256) k_step_end(const int *__restrict__ J, con
   rec.step = s;
   rec.oldlen = sl;
   rec.it = st->it;
+  // BUILD: from the step's start to its pivots kernel; TRANSF: from there to here (100 MHz ticks)
+  const unsigned long long now = wall_clock64();
+  rec.build_s = (sl > 0) ? (float)(st->t_transf - st->t_begin) * 1e-8f : (float)(now - st->t_begin) * 1e-8f;
+  rec.transf_s = (sl > 0) ? (float)(now - st->t_transf) * 1e-8f : 0.f;
   if (sl <= 0) {
     if (threadIdx.x == 0) {
       rec.newlen = -1;
@@ -989,15 +996,21 @@ void weights_launch(DeviceStep &d, hipStream_t s) {
     zweights_launch(d, s);
     return;
   }
-  const double *A = (const double *)d.A;
   const int nt = (d.n + TWT - 1) / TWT;
-  if (d.fused) {
-    hipLaunchKernelGGL(k_trig_weights<true>, dim3(nt, nt), dim3(256), 0, s, d.n, A, d.lda, d.J, d.sort.k0,
-                       d.sort.v0, d.st, d.wmode ? d.neg_tol : 0.0);
-  } else {
-    hipLaunchKernelGGL(k_trig_weights<false>, dim3(nt, nt), dim3(256), 0, s, d.n, A, d.lda, d.J, d.sort.k0,
-                       d.sort.v0, d.st, d.wmode ? d.neg_tol : 0.0);
+  const double tol = d.wmode ? d.neg_tol : 0.0;
+  auto trig = [&](auto kern, const auto *A, int cut) {
+    hipLaunchKernelGGL(kern, dim3(nt, nt), dim3(256), 0, s, d.n, A, d.lda, d.J, d.sort.k0, d.sort.v0, d.st, tol, cut);
+  };
+  if (d.prec32) {  // the FP32 warm start (Gram-form hyperbolic weights only)
+    const float *A = (const float *)d.A32;
+    if (d.fused) trig(k_trig_weights<true, float>, A, d.cut32);
+    else trig(k_trig_weights<false, float>, A, d.cut32);
+    gram_hyp_weights_launch(d, s);
+    return;
   }
+  const double *A = (const double *)d.A;
+  if (d.fused) trig(k_trig_weights<true, double>, A, 0);
+  else trig(k_trig_weights<false, double>, A, 0);
   if (d.wmode == 1) {
     gram_hyp_weights_launch(d, s);
     return;
@@ -1067,16 +1080,18 @@ void pivots_launch(DeviceStep &d, hipStream_t s) {
     return;
   }
   const int g = (d.n2max + 255) / 256;
-  if (d.fused)
-    hipLaunchKernelGGL(k_pivots<true>, dim3(g), dim3(256), 0, s, d.n, (const double *)d.A, d.lda, d.J, d.piv_p,
-                       d.piv_q, d.piv_pos, d.sort.k0, d.sort.k1, d.sort.v0, d.sort.v1, d.sort_skip, d.piv_w,
-                       d.piv_info, (double *)d.piv_V, (double *)d.piv_W, d.piv_d, d.rot_row, d.rot_col,
-                       d.colpiv, d.st, d.neg_tol);
-  else
-    hipLaunchKernelGGL(k_pivots<false>, dim3(g), dim3(256), 0, s, d.n, (const double *)d.A, d.lda, d.J, d.piv_p,
-                       d.piv_q, d.piv_pos, d.sort.k0, d.sort.k1, d.sort.v0, d.sort.v1, d.sort_skip, d.piv_w,
-                       d.piv_info, (double *)d.piv_V, (double *)d.piv_W, d.piv_d, d.rot_row, d.rot_col,
-                       d.colpiv, d.st, d.neg_tol);
+  auto go = [&](auto kern, const auto *A) {
+    hipLaunchKernelGGL(kern, dim3(g), dim3(256), 0, s, d.n, A, d.lda, d.J, d.piv_p, d.piv_q, d.piv_pos, d.sort.k0,
+                       d.sort.k1, d.sort.v0, d.sort.v1, d.sort_skip, d.piv_w, d.piv_info, (double *)d.piv_V,
+                       (double *)d.piv_W, d.piv_d, d.rot_row, d.rot_col, d.colpiv, d.st, d.neg_tol);
+  };
+  if (d.prec32) {
+    if (d.fused) go(k_pivots<true, float>, (const float *)d.A32);
+    else go(k_pivots<false, float>, (const float *)d.A32);
+  } else {
+    if (d.fused) go(k_pivots<true, double>, (const double *)d.A);
+    else go(k_pivots<false, double>, (const double *)d.A);
+  }
 }
 
 void rotate_launch(DeviceStep &d, hipStream_t s) {
@@ -1084,13 +1099,19 @@ void rotate_launch(DeviceStep &d, hipStream_t s) {
     zrotate_launch(d, s);
     return;
   }
-  const size_t lds = (size_t)2 * d.n * sizeof(double);
-  if (d.fused)
-    hipLaunchKernelGGL(k_rotate<true>, dim3(d.n), dim3(1024), lds, s, d.n, (double *)d.A, d.lda, (double *)d.U,
-                       (double *)d.Z, d.piv_p, d.piv_q, d.piv_info, d.piv_d, d.rot_row, d.rot_col, d.colpiv, d.st);
-  else
-    hipLaunchKernelGGL(k_rotate<false>, dim3(d.n), dim3(1024), lds, s, d.n, (double *)d.A, d.lda, (double *)d.U,
-                       (double *)d.Z, d.piv_p, d.piv_q, d.piv_info, d.piv_d, d.rot_row, d.rot_col, d.colpiv, d.st);
+  auto go = [&](auto kern, auto *A, auto *U, auto *Z) {
+    hipLaunchKernelGGL(kern, dim3(d.n), dim3(1024), (size_t)2 * d.n * sizeof(*A), s, d.n, A, d.lda, U, Z, d.piv_p,
+                       d.piv_q, d.piv_info, d.piv_d, d.rot_row, d.rot_col, d.colpiv, d.st);
+  };
+  if (d.prec32) {
+    float *A = (float *)d.A32, *U = (float *)d.U32, *Z = (float *)d.Z32;
+    if (d.fused) go(k_rotate<true, float>, A, U, Z);
+    else go(k_rotate<false, float>, A, U, Z);
+  } else {
+    double *A = (double *)d.A, *U = (double *)d.U, *Z = (double *)d.Z;
+    if (d.fused) go(k_rotate<true, double>, A, U, Z);
+    else go(k_rotate<false, double>, A, U, Z);
+  }
 }
 
 void step_end_launch(DeviceStep &d, hipStream_t s) {
@@ -1104,11 +1125,12 @@ int step_prepare(DeviceStep &d) {
   if (d.cplx) return zstep_prepare(d);
   const size_t lds = (size_t)2 * d.n * sizeof(double);
   if (lds > 160 * 1024) return -1;  // two columns must fit the LDS of one CU
-  if (hipFuncSetAttribute((const void *)k_rotate<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
-          hipSuccess ||
-      hipFuncSetAttribute((const void *)k_rotate<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
-          hipSuccess)
-    return -2;
+  const void *k[4] = {(const void *)k_rotate<true, double>, (const void *)k_rotate<false, double>,
+                      (const void *)k_rotate<true, float>, (const void *)k_rotate<false, float>};
+  for (int i = 0; i < 4; ++i)
+    if (hipFuncSetAttribute(k[i], hipFuncAttributeMaxDynamicSharedMemorySize, (int)(i < 2 ? lds : lds / 2)) !=
+        hipSuccess)
+      return -2;
   return 0;
 }
 

@@ -26,6 +26,7 @@ struct ListPtrs {
 struct StepRec {
   int32_t step, oldlen, newlen, trig, hyp, it;
   double tw;
+  float build_s, transf_s;  // device wall time of the step's BUILD and TRANSF parts
 };
 
 // Column-rotation / row-rotation descriptor of a 2x2 factor B in the ratio
@@ -68,8 +69,10 @@ struct StepState {
   double wmax;    // largest weight of this step
   double tw0, twprev, ynorm2;  // TW of the first and of the previous step; ||Y||_F^2
   unsigned long long t_begin;  // wall clock (100 MHz) at the start of this step
+  unsigned long long t_transf; // wall clock at the start of its TRANSF part (the 2x2 pivots)
   float ema, n2f, ltau;        // smoothed TW rate; the action of this step (N_2 / n2max, log10 tau)
   float ltau_cap;              // largest log10 tau a controller may set
+  float lhandoff;              // log10(TW / TW_0) below which a controller hands back to the full ordering
   float feat[16];              // controller features after the last step
   // FP32 warm start: relative off-norm of A (this step's), the best seen and its step, why the phase ended
   double off_rel, off_best;
@@ -185,11 +188,11 @@ void controller_launch(DeviceStep &d, hipStream_t s);
 // FP32 warm start and its hand-over (jkg_warm.hip)
 void warm_begin_launch(DeviceStep &d, hipStream_t s);
 void warm_promote_launch(DeviceStep &d, hipStream_t s);
-void zgemm_h_launch(int n, const void *X, const void *Y, void *C, hipStream_t s);
-void zadj_launch(int n, const void *X, void *P, hipStream_t s);
-void jrows_launch(int n, const void *X, const int *J, void *Y, hipStream_t s);
+void gemm_h_launch(int n, int cplx, const void *X, const void *Y, void *C, hipStream_t s);
+void adj_launch(int n, int cplx, const void *X, void *P, hipStream_t s);
+void jrows_launch(int n, int cplx, const void *X, const int *J, void *Y, hipStream_t s);
 int ns_blocks(int n);
-void ns_step_launch(int n, const void *M, const int *J, void *B, double *part, hipStream_t s);
+void ns_step_launch(int n, int cplx, const void *M, const int *J, void *B, double *part, hipStream_t s);
 void gemm_tn_launch(int k, const double *X, int ldx, const int *xcols, int m, const double *Y, int ldy,
                     const int *ycols, int nc, double *C, long csi, long csj, bool zero_diag, const StepState *st,
                     hipStream_t s, int cx = 0, int ks = 1);

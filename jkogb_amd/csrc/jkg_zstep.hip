@@ -242,6 +242,7 @@ __global__ void __launch_bounds__(256) k_zpivots(int n, const E *__restrict__ A,
   if (!st->active) return;
   const int sl = st->sl;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) st->t_transf = wall_clock64();
   if (i >= sl) return;
   const int p = piv_p[i], q = piv_q[i];
   const size_t P = p, Q = q;

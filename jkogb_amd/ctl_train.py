@@ -215,6 +215,8 @@ def main(argv=None):
     p = cands[pick - 1]
     log("picked candidate %d: validation %.3f s vs %.3f s without a controller" % (pick, tot[pick], tot[0]))
     np.save(args.out, p)
+    # the same parameters as raw little-endian FP32, for the drop-in programs (JKG_SOLVER=learned)
+    np.ascontiguousarray(p, dtype="<f4").tofile(os.path.splitext(args.out)[0] + ".f32")
     log("wrote %s" % args.out)
     rep.append({"validation_seconds": [float(t) for t in tot], "validation_ok": [bool(o) for o in ok],
                 "picked": int(pick)})

@@ -32,7 +32,7 @@ JKG_EJ = -1003
 
 AW_DTYPE = np.dtype([("w", "<f8"), ("p", "<i4"), ("q", "<i4"), ("b", "<i4"), ("i", "<i4")])
 STEPREC_DTYPE = np.dtype([("step", "<i4"), ("oldlen", "<i4"), ("newlen", "<i4"), ("trig", "<i4"),
-                          ("hyp", "<i4"), ("it", "<i4"), ("tw", "<f8")])
+                          ("hyp", "<i4"), ("it", "<i4"), ("tw", "<f8"), ("build_s", "<f4"), ("transf_s", "<f4")])
 
 
 PROGRESS = C.CFUNCTYPE(None, C.c_void_p, C.c_int32, C.c_int32)
@@ -66,7 +66,7 @@ class MixedOpts(C.Structure):
                 ("key_bits", C.c_int32), ("controller", C.c_int32), ("ctl", C.c_void_p),
                 ("threshold", C.c_double), ("tol", C.c_double), ("ctl_trace", C.c_void_p),
                 ("ctl_trace_cap", C.c_int32), ("ordering", C.c_int32), ("warm", C.c_void_p),
-                ("ctl_tau_max", C.c_double), ("reserved", C.c_int32 * 2)]
+                ("ctl_tau_max", C.c_double), ("ctl_handoff", C.c_float), ("reserved", C.c_int32)]
 
 
 class WarmInfo(C.Structure):

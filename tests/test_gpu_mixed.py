@@ -178,3 +178,54 @@ def test_mixed_threshold_rejects_out_of_range(ctx):
     for t in (-1.0, 1.0, float("nan")):
         with pytest.raises(jkg.JKGError):
             ctx.dsolve_mixed(Y, J, 8, threshold=t)
+
+
+# ---------------------------------------------------------------- FP32 warm start (real)
+@pytest.mark.parametrize("cfg,n,key_bits", [("c1", 64, 64), ("c5", 80, 32), ("c2", 96, 64)])
+def test_real_warm_start_vs_oracle(ctx, orc, cfg, n, key_bits):
+    """FP32 steps until the relative off-norm is below 1e-5, polar factors of U and Z in FP64,
+    A = U^T Y Z, FP64 steps: every singular value within 64 N eps of the oracle's (the
+    reference's arithmetic, run on the CPU), orthogonality and reconstruction to the same order."""
+    Y, J = inputs(cfg, n, 17)
+    out = ctx.dsolve_mixed(Y, J, n // 2, fp32_warm=1, key_bits=key_bits)
+    w = out["warm"]
+    assert out["info"] >= 0
+    assert w["fallback"] == 0 and w["warm_end"] in (1, 2, 3), w
+    assert w["steps32"] > 0 and w["ns_u"] >= 1 and w["ns_z"] >= 1, w
+    o = orc.solve(Y, J, n // 2)
+    rel = np.abs(out["SY"] - o["SY"]) / np.abs(o["SY"])
+    assert rel.max() <= 64 * n * EPS, (rel.max() / (n * EPS))
+    rec, orth, jorth = residuals(Y, J, out)
+    orec, oorth, ojorth = residuals(Y, J, o)
+    assert rec <= 64 * n * EPS and orth <= 64 * n * EPS, (rec, orth)
+    assert jorth <= 4 * ojorth + 2 * n * EPS, (jorth, ojorth)
+
+
+@pytest.mark.parametrize("cfg,n", [("c5", 160), ("c2", 203)])
+def test_real_warm_start_vs_plain(ctx, cfg, n):
+    """Larger sizes against the plain solver (bit-identical to the oracle, see the parity tests):
+    64 N eps index by index, residuals within 64 N eps."""
+    Y, J = inputs(cfg, n, 18)
+    out = ctx.dsolve_mixed(Y, J, n // 2, fp32_warm=1, key_bits=32)
+    plain = ctx.dsolve(Y, J, n // 2)
+    assert out["warm"]["fallback"] == 0
+    rel = np.abs(out["SY"] - plain["SY"]) / np.abs(plain["SY"])
+    assert rel.max() <= 64 * n * EPS, (rel.max() / (n * EPS))
+    rec, orth, jorth = residuals(Y, J, out)
+    prec, porth, pjorth = residuals(Y, J, plain)
+    assert rec <= 64 * n * EPS and orth <= 64 * n * EPS, (rec, orth)
+    assert jorth <= 4 * pjorth + 2 * n * EPS, (jorth, pjorth)
+
+
+def test_real_warm_start_step_limit_and_frames(ctx):
+    n = 64
+    Y, J = inputs("c2", n, 19)
+    out = ctx.dsolve_mixed(Y, J, n // 2, fp32_warm=1, max_steps=5)
+    assert out["steps"] == 5 and out["warm"]["warm_end"] == 4 and out["warm"]["steps32"] == 5
+    ctx.set_frames(every=1, size=8, max_frames=4096)
+    try:
+        full = ctx.dsolve_mixed(Y, J, n // 2, fp32_warm=1)
+        assert ctx.frames().shape[0] == full["steps"]
+    finally:
+        ctx.set_frames(0)
+    assert full["steps"] > 5
