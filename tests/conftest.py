@@ -3,6 +3,13 @@ import sys
 
 import pytest
 
+# torch, where a test uses it beside the library, has to be loaded first: it carries its own HIP
+# runtime under the same soname as the one libjkg.so links, and a process can hold only one
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)

@@ -363,13 +363,18 @@ def test_context_reuse_with_another_j_split(ctx, orc):
 
 
 def test_progress_hook(ctx):
-    # the hook sees the current step at every poll, increasing, and the end exactly once
+    # the hook sees the steps executed so far at every poll, increasing, with their trace records
+    # already in the caller's buffer, and the end exactly once
     calls = []
     Y, J = inputs("c2", 64, 5)
-    r = ctx.dsolve(Y, J, 32, steps_per_sync=8, progress=lambda s, d: calls.append((s, d)))
+
+    def hook(s, d):
+        calls.append((s, d))
+    r = ctx.dsolve(Y, J, 32, steps_per_sync=8, progress=hook)
     assert calls and calls[-1][1] and sum(d for _, d in calls) == 1
     assert all(a[0] <= b[0] for a, b in zip(calls, calls[1:]))
-    assert calls[-1][0] == r["info"]
+    assert calls[-1][0] == r["steps"] == r["info"] + 1
+    assert all(s % 8 == 0 for s, d in calls if not d)
 
 
 # --------------------------------------------------------------- ID_NCP=3
