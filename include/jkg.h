@@ -110,6 +110,19 @@ int jkg_dsolve(jkg_ctx *ctx, int n, int n2, const double *y, const int64_t *j, d
 int jkg_zsolve(jkg_ctx *ctx, int n, int n2, const double *y, const int64_t *j, double *yu, double *zz,
                double *sy, const jkg_opts *opts, jkg_result *res);
 
+/* jkg_dsolve_noalloc: jkg_dsolve that allocates nothing.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  y, yu, zz (N x N,
+ * column-major) and sy (N) are DEVICE buffers on the context's device; j is a
+ * host array.  Every working buffer lives in one fixed scratch array of the
+ * library (6 MiB per GPU, held until the process ends): N <= 256 (JKG_ENOTSUP
+ * beyond, outputs untouched), trace records <= 64 N + 64, ID_NCP 0-2.  The steps
+ * are launched one by one on the context's stream (no graph); results, INFO and
+ * trace are jkg_dsolve's bit for bit; res->seconds is host wall time.  All calls
+ * on one GPU share the scratch array: calls that overlap on one GPU race. */
+int jkg_dsolve_noalloc(jkg_ctx *ctx, int n, int n2, const double *y, const int64_t *j, double *yu, double *zz,
+                       double *sy, const jkg_opts *opts, jkg_result *res);
+
 /* ---------------------------------------------------------------------------
  * Strategy level: the reference's DPROC plug-in procedures
  * (src/D/dtypes.F90:5-30), bound by DPROC_INIT (src/D/dstep.F90:76-134).

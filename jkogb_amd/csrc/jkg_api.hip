@@ -8,6 +8,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/jkg.h"
@@ -333,6 +335,8 @@ const char *jkg_strerror(int code) {
 
 int jkg_last_hip_error(void) { return g_last_hip; }
 
+static int noalloc_setup(int device);
+
 int jkg_create(jkg_ctx **out, int device) {
   if (!out) return -1;
   *out = nullptr;
@@ -340,6 +344,10 @@ int jkg_create(jkg_ctx **out, int device) {
   HIPCHK(hipGetDeviceCount(&count));
   if (device < 0 || device >= count) return -2;
   HIPCHK(hipSetDevice(device));
+  {
+    const int rc = noalloc_setup(device);  // the device address of this GPU's scratch array, once
+    if (rc) return rc;
+  }
   jkg_ctx *c = new jkg_ctx();
   c->device = device;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
@@ -880,6 +888,190 @@ extern "C" int jkg_zsolve_mixed(jkg_ctx *c, int n, int n2, const double *y, cons
   mc.switch_tol = m.switch_tol;
   mc.wi = m.warm;
   return solve_common(c, 1, n, n2, y, j, yu, zz, sy, opts, res, mc);
+}
+
+// ---------------------------------------------------------------- allocation-free solve
+// jkg_dsolve_noalloc: jkg_dsolve for callers that cannot allocate device memory
+// (code under HIP graph capture, serving loops, harnesses that forbid it).  Every
+// buffer of the step -- A, U, Z, keys, lists, pivots, loop state and trace -- lives
+// in `scratch`, a file-scope device array of NOALLOC_BYTES held by each GPU until
+// the process ends; the caller passes Y, YU, ZZ and SY as device buffers.  All calls
+// on one GPU share the array (it keeps the last call's contents; every part a step
+// reads is written first in the same call), so calls that overlap on one GPU race.
+constexpr int NOALLOC_NMAX = 256;
+constexpr int NOALLOC_TRACE = 64 * NOALLOC_NMAX + 64;
+constexpr size_t NOALLOC_BYTES = (size_t)6 << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+static void *g_scratch_ptr[64];  // per device: the array's device address (hipGetSymbolAddress)
+static bool g_scratch_set[64];
+
+// Once per device, with that device current (called by jkg_create).
+static int noalloc_setup(int device) {
+  if (device < 0 || device >= 64) return JKG_ENOTSUP;
+  if (g_scratch_set[device]) return JKG_OK;
+  void *p = nullptr;
+  HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_scratch_ptr[device] = p;
+  g_scratch_set[device] = true;
+  return JKG_OK;
+}
+
+// The step's buffers for an n x n real problem laid out from base (256-byte aligned
+// pieces); returns the bytes used.  base == nullptr: the size only.
+static size_t noalloc_layout(DeviceStep &d, double **sigma, char *base, int n, int trace_cap) {
+  size_t off = 0;
+  auto take = [&](auto **p, size_t count) {
+    using T = typename std::remove_reference<decltype(**p)>::type;
+    off = (off + 255) & ~(size_t)255;
+    if (base) *p = (T *)(base + off);
+    off += sizeof(T) * (count ? count : 1);
+  };
+  const size_t nn = (size_t)n * (n - 1) / 2;
+  const int n2max = n / 2 > 0 ? n / 2 : 1;
+  memset(&d, 0, sizeof(d));
+  d.n = n;
+  d.lda = n;
+  d.cplx = 0;
+  d.nn = (int)nn;
+  d.n2max = n2max;
+  take((double **)&d.A, (size_t)n * n);
+  take((double **)&d.U, (size_t)n * n);
+  take((double **)&d.Z, (size_t)n * n);
+  take(&d.J, n);
+  take(&d.plus, n);
+  take(&d.minus, n);
+  take(&d.sort.k0, nn);
+  take(&d.sort.k1, nn);
+  take(&d.sort.v0, nn + LIST_PAD);
+  take(&d.sort.v1, nn + LIST_PAD);
+  take((char **)&d.sort.scratch, sort_scratch_bytes((int)nn));
+  if (base) d.sort_skip = sort_skip_flags(d.sort, (int)nn);
+  for (int k = 0; k < 2; ++k) {
+    take(&d.lists.val[k], nn + LIST_PAD);
+    take(&d.lists.pos[k], nn + LIST_PAD);
+  }
+  take(&d.fcount, select_count_ints((int)nn));
+  take(&d.used, n);
+  take(&d.piv_p, n2max);
+  take(&d.piv_q, n2max);
+  take(&d.piv_pos, n2max);
+  take(&d.piv_info, n2max);
+  take(&d.piv_w, n2max);
+  take((double **)&d.piv_V, 4 * (size_t)n2max);
+  take((double **)&d.piv_W, 4 * (size_t)n2max);
+  take(&d.piv_d, 2 * (size_t)n2max);
+  take(&d.rot_row, n2max);
+  take(&d.rot_col, n2max);
+  take(&d.colpiv, n);
+  take(&d.st, 1);
+  take(&d.trace, trace_cap);
+  take(sigma, n);
+  return off;
+}
+
+extern "C" int jkg_dsolve_noalloc(jkg_ctx *c, int n, int n2, const double *y, const int64_t *j, double *yu,
+                                  double *zz, double *sy, const jkg_opts *opts, jkg_result *res) {
+  if (!c) return -1;
+  if (n < 3) return -2;
+  if (!y) return -4;
+  if (!j) return -5;
+  if (!yu) return -6;
+  if (!zz) return -7;
+  if (!sy) return -8;
+  if (!res) return -10;
+  if (n > NOALLOC_NMAX) return JKG_ENOTSUP;
+  jkg_opts o;
+  if (opts) o = *opts;
+  else jkg_opts_default(&o);
+  if (o.id_ncp < 0 || o.id_ncp > 2 || o.id_tru < 0 || o.id_tru > 2) return -9;
+  memset(res, 0, sizeof(*res));
+  n2 = jkg_jstep_len(n, n2);
+  res->n2 = n2;
+  if (n2 <= 0) return -3;
+  if (!g_scratch_set[c->device]) return JKG_ENOTSUP;
+  HIPCHK(hipSetDevice(c->device));
+  const int trace_cap = std::min(std::max(o.trace_cap, 1), NOALLOC_TRACE);
+  DeviceStep d;
+  double *sigma = nullptr;
+  if (noalloc_layout(d, &sigma, nullptr, n, trace_cap) > NOALLOC_BYTES) return JKG_ENOTSUP;
+  noalloc_layout(d, &sigma, (char *)g_scratch_ptr[c->device], n, trace_cap);
+  d.fused = o.fused ? 1 : 0;
+  d.select_limit = c->sel_limit;
+  d.sel_rounds = c->sel_rounds > 0 ? c->sel_rounds : select_rounds_for((size_t)d.nn);
+  // the problem: J and its classes from fixed host arrays (no allocation), Y copied on the device
+  int J32[NOALLOC_NMAX], pl[NOALLOC_NMAX], mi[NOALLOC_NMAX];
+  int npl = 0, nmi = 0;
+  for (int i = 0; i < n; ++i) {
+    if (j[i] != 1 && j[i] != -1) return JKG_EJ;
+    J32[i] = (int)j[i];
+    if (j[i] == 1) pl[npl++] = i;
+    else mi[nmi++] = i;
+  }
+  d.npl = npl;
+  d.nmi = nmi;
+  hipStream_t s = c->stream;
+  HIPCHK(hipMemcpyAsync(d.A, y, sizeof(double) * n * n, hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipMemcpyAsync(d.J, J32, sizeof(int) * n, hipMemcpyHostToDevice, s));
+  if (npl) HIPCHK(hipMemcpyAsync(d.plus, pl, sizeof(int) * npl, hipMemcpyHostToDevice, s));
+  if (nmi) HIPCHK(hipMemcpyAsync(d.minus, mi, sizeof(int) * nmi, hipMemcpyHostToDevice, s));
+  StepState st{};
+  st.max_steps = o.max_steps;
+  st.n2 = n2;
+  st.n2max = n2;
+  st.trace_cap = trace_cap;
+  st.n2f = 1.f;
+  st.ltau = -16.f;
+  st.off_best = HUGE_VAL;
+  st.ltau_cap = -6.f;
+  st.lhandoff = -10.f;
+  st.ynorm2 = 1.0;
+  HIPCHK(hipMemcpyAsync(d.st, &st, sizeof(st), hipMemcpyHostToDevice, s));
+  init_identity_launch(d, s);
+  HIPCHK(hipGetLastError());
+  res->npos = npl;
+  res->nneg = nmi;
+  res->tt = (int)((int64_t)npl * (npl - 1) / 2 + (int64_t)nmi * (nmi - 1) / 2);
+  // the steps, launched one by one (no graph: instantiating one may allocate), the done flag
+  // polled every `batch` steps
+  const int batch = o.steps_per_sync > 0 ? o.steps_per_sync : 32;
+  const long cap = 1000L * n + 1000;
+  long launched = 0, remaining = (o.max_steps >= 0) ? (long)o.max_steps : -1;
+  const auto t0 = std::chrono::steady_clock::now();
+  for (;;) {
+    const long nb = (remaining >= 0 && remaining < batch) ? remaining : batch;
+    for (long k = 0; k < nb; ++k) step_launch(d, s);
+    HIPCHK(hipGetLastError());
+    launched += nb;
+    if (remaining >= 0) remaining -= nb;
+    if (remaining == 0) {
+      step_begin_launch(d, s);  // records the end of a step-limited run, as run_loop does
+      break;
+    }
+    HIPCHK(hipMemcpyAsync(&st, d.st, sizeof(st), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (o.progress) o.progress(o.user, st.nrec, st.done);
+    if (st.done) break;
+    if (launched > cap) return JKG_ENOCONV;
+  }
+  finalize_launch(d, s, sigma);
+  HIPCHK(hipMemcpyAsync(yu, d.U, sizeof(double) * n * n, hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipMemcpyAsync(zz, d.Z, sizeof(double) * n * n, hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipMemcpyAsync(sy, sigma, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipMemcpyAsync(&st, d.st, sizeof(st), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (o.trace && o.trace_cap > 0) {
+    const int cnt = std::min(st.nrec, std::min(o.trace_cap, trace_cap));
+    if (cnt > 0) HIPCHK(hipMemcpy(o.trace, d.trace, sizeof(jkg_steprec) * cnt, hipMemcpyDeviceToHost));
+  }
+  res->info = st.info;
+  res->steps = st.nrec;
+  res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  res->sum_sl = (int64_t)st.acc_sl;
+  res->sum_it = (int64_t)st.acc_it;
+  res->sum_b2 = (int64_t)st.acc_b2;
+  res->sum_b4 = (int64_t)st.acc_b4;
+  res->sum_abod = (int64_t)st.acc_abod;
+  return JKG_OK;
 }
 
 // ---------------------------------------------------------------- step level

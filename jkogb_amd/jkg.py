@@ -113,6 +113,8 @@ _strerror = _sig("jkg_strerror", C.c_char_p, [_i])
 _last_hip = _sig("jkg_last_hip_error", _i, [])
 _dsolve = _sig("jkg_dsolve", _i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, C.POINTER(Opts), C.POINTER(Result)])
 _zsolve = _sig("jkg_zsolve", _i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, C.POINTER(Opts), C.POINTER(Result)])
+_dsolve_noalloc = _sig("jkg_dsolve_noalloc", _i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, C.POINTER(Opts),
+                                                  C.POINTER(Result)])
 _srt = _sig("jkg_srt", _i, [_vp, _i, _i, _vp, C.POINTER(C.c_int64)])
 _ncp = _sig("jkg_ncp", _i, [_vp, _i, _i, _i, _vp, _i, C.POINTER(C.c_int32), _vp, C.POINTER(C.c_int64)])
 _ncp3 = _sig("jkg_ncp3", _i, [_vp, _i, _i, _i, _vp, _i, C.POINTER(C.c_int32), _vp, C.POINTER(C.c_int64)])
@@ -153,7 +155,8 @@ EXPORTS = ["jkg_create", "jkg_destroy", "jkg_strerror", "jkg_last_hip_error", "j
            "jkg_dweights", "jkg_zweights", "jkg_dbuild", "jkg_dstep", "jkg_dprofile", "jkg_pair_index",
            "jkg_key_weight", "jkg_mixed_opts_default", "jkg_dsolve_mixed", "jkg_dweights_mixed",
            "jkg_dprofile_mixed", "jkg_ctl_grad", "jkg_ctl_fit", "jkg_batch_opts_default", "jkg_dsolve_batched",
-           "jkg_zsolve_mixed", "jkg_zweights_mixed", "jkg_zprofile", "jkg_zprofile_mixed", "jkg_ncp3", "jkg_dgen", "jkg_set_select", "jkg_set_frames"]
+           "jkg_zsolve_mixed", "jkg_zweights_mixed", "jkg_zprofile", "jkg_zprofile_mixed", "jkg_ncp3", "jkg_dgen",
+           "jkg_set_select", "jkg_set_frames", "jkg_dsolve_noalloc"]
 
 
 def mixed_opts(**kw):
@@ -324,6 +327,21 @@ class Context:
         if wi is not None:
             out["warm"] = {f: getattr(wi, f) for f, _ in WarmInfo._fields_ if f != "reserved"}
         return out
+
+    def dsolve_noalloc(self, n, d_y, J, d_yu, d_zz, d_sy, n2=0, fused=False, max_steps=-1, trace_cap=None):
+        """jkg_dsolve_noalloc: dsolve on device buffers (addresses of n x n column-major FP64 arrays
+        d_y, d_yu, d_zz and n FP64 d_sy on this context's device), with no device allocation; N <= 256.
+        Returns dict(info, trace, tt, pn, n2, seconds, steps)."""
+        J64 = np.ascontiguousarray(J, dtype=np.int64)
+        if trace_cap is None:
+            trace_cap = 64 * n + 64 if max_steps < 0 else max_steps + 1
+        trace = np.zeros(max(trace_cap, 1), STEPREC_DTYPE)
+        o = Opts(0, 0, 1 if fused else 0, max_steps, 0, trace_cap, _p(trace))
+        r = Result()
+        _chk(_dsolve_noalloc(self._h, n, n2, C.c_void_p(d_y), _p(J64), C.c_void_p(d_yu), C.c_void_p(d_zz),
+                             C.c_void_p(d_sy), C.byref(o), C.byref(r)), "jkg_dsolve_noalloc")
+        return dict(info=r.info, trace=trace[:min(r.steps, trace_cap)], tt=r.tt, pn=(r.npos, r.nneg), n2=r.n2,
+                    seconds=r.seconds, steps=r.steps, **_counts(r))
 
     # ---------------------------------------------------------------- batched solver
     def dsolve_batched(self, Ys, Js, n2s=None, fused=False, max_steps=-1, grid=0, variant=-1, tol=0.0):
