@@ -22,6 +22,22 @@
 !   warm    the mixed solver starting with FP32 steps (fp32_warm)
 ! Other environment: JKG_DEVICE (default 0), JKG_FUSED=1 selects FMAD = fma.
 !
+! Checkpoint and resume (plain solver; jkg_dsolve_from / jkg_zsolve_from):
+!   JKG_CHECKPOINT=K  the loop stops every K steps and writes its state: FN.CA,
+!                     FN.CU, FN.CZ (A, U and Z as the loop holds them, N x N
+!                     column-major, the element type of FN.Y) and FN.CS (the
+!                     next step's index, one int64; -1 while the others are
+!                     being written), then goes on; the outputs, stdout and the
+!                     CSV are those of one uninterrupted run
+!   JKG_RESUME=1      starts from FN.CA, FN.CU, FN.CZ at step FN.CS instead of
+!                     FN.Y (FN.J is still read); the CSV continues from that
+!                     step and the last stdout line counts all steps (its time:
+!                     this run's)
+! Frames (the role of the reference's ANIMATE build, src/D/dstep.F90:485-592):
+!   JKG_FRAMES=E[,S[,M]]  a S x S byte picture of |A| at the start of every step
+!                     s = 0 mod E (jkg_set_frames; S = 64, at most M = 1024 per
+!                     solve call), appended row-major to FN.FRM
+!
 ! Deviations from the reference: a failure before the loop (arguments, files,
 ! the device) ends with ERROR STOP (nonzero exit status; the reference's STOP
 ! exits with 0) and the reference's message; BUILDs and TRANSFs are device
@@ -35,19 +51,25 @@ PROGRAM jk
   IMPLICIT NONE
 #ifdef ZJK
   CHARACTER(LEN=*), PARAMETER :: USAGE = 'zjk.exe FN N N_2 [ID_NCP [ID_TRU]]'
-  COMPLEX(c_double_complex), ALLOCATABLE, TARGET :: Y(:,:), YU(:,:), ZZ(:,:)
+  COMPLEX(c_double_complex), ALLOCATABLE, TARGET :: Y(:,:), YU(:,:), ZZ(:,:), CU(:,:), CZ(:,:)
+  COMPLEX(c_double_complex), PARAMETER :: ONE = (1.0_c_double, 0.0_c_double)
 #else
   CHARACTER(LEN=*), PARAMETER :: USAGE = 'djk.exe FN N N_2 [ID_NCP [ID_TRU]]'
-  REAL(c_double), ALLOCATABLE, TARGET :: Y(:,:), YU(:,:), ZZ(:,:)
+  REAL(c_double), ALLOCATABLE, TARGET :: Y(:,:), YU(:,:), ZZ(:,:), CU(:,:), CZ(:,:)
+  REAL(c_double), PARAMETER :: ONE = 1.0_c_double
 #endif
   REAL(c_double), ALLOCATABLE :: SY(:)
   REAL(c_float), ALLOCATABLE, TARGET :: CTL(:)
   INTEGER(c_int64_t), ALLOCATABLE :: J(:)
+  INTEGER(c_int8_t), ALLOCATABLE :: FRM(:)
+  INTEGER(c_int32_t) :: NFRM
   CHARACTER(LEN=252) :: FN
   CHARACTER(LEN=1024) :: PATH
   INTEGER :: N, N_2, ID_NCP, ID_TRU, INFO, I, NP, NM, U1, U2, CAP
+  INTEGER :: CKPT, RESUME, FEVERY, FSIZE, FMAX, UF, STEP0
   INTEGER(c_int) :: RC, DEV
-  INTEGER(c_int64_t) :: TT
+  INTEGER(c_int64_t) :: TT, S64
+  REAL(c_double) :: SECS
   TYPE(c_ptr) :: CTX
   TYPE(jkg_opts) :: OPTS
   TYPE(jkg_mixed_opts) :: MOPTS
@@ -66,24 +88,86 @@ PROGRAM jk
   IF ((SOLVER .NE. 'plain') .AND. (SOLVER .NE. 'mixed') .AND. (SOLVER .NE. 'learned') .AND. &
       (SOLVER .NE. 'warm')) ERROR STOP 'JKG_SOLVER: plain, mixed, learned or warm'
   IF ((SOLVER .NE. 'plain') .AND. (ID_NCP .EQ. 3)) ERROR STOP 'JKG_SOLVER: ID_NCP=3 needs the plain solver'
+  CKPT = 0
+  CALL GET_ENVIRONMENT_VARIABLE('JKG_CHECKPOINT', ENVV, STATUS=INFO)
+  IF (INFO .EQ. 0) THEN
+     READ (ENVV,*,IOSTAT=INFO) CKPT
+     IF ((INFO .NE. 0) .OR. (CKPT .LE. 0)) ERROR STOP 'JKG_CHECKPOINT: a positive number of steps'
+  END IF
+  RESUME = 0
+  CALL GET_ENVIRONMENT_VARIABLE('JKG_RESUME', ENVV, STATUS=INFO)
+  IF (INFO .EQ. 0) THEN
+     READ (ENVV,*,IOSTAT=INFO) RESUME
+     IF ((INFO .NE. 0) .OR. (RESUME .LT. 0) .OR. (RESUME .GT. 1)) ERROR STOP 'JKG_RESUME: 0 or 1'
+  END IF
+  IF (((CKPT .GT. 0) .OR. (RESUME .EQ. 1)) .AND. (SOLVER .NE. 'plain')) &
+       ERROR STOP 'JKG_CHECKPOINT, JKG_RESUME: the plain solver only'
+  FEVERY = 0
+  FSIZE = 64
+  FMAX = 1024
+  CALL GET_ENVIRONMENT_VARIABLE('JKG_FRAMES', ENVV, STATUS=INFO)
+  IF (INFO .EQ. 0) THEN
+     PATH = TRIM(ENVV)//' /'  ! '/' ends the list: absent values keep their defaults
+     READ (PATH,*,IOSTAT=INFO) FEVERY, FSIZE, FMAX
+     IF ((INFO .NE. 0) .OR. (FEVERY .LE. 0) .OR. (FSIZE .LE. 0) .OR. (FMAX .LE. 0)) &
+          ERROR STOP 'JKG_FRAMES: E[,S[,M]], positive'
+  END IF
 
   ALLOCATE(Y(N,N), J(N))
-  OPEN(NEWUNIT=U1, FILE=TRIM(FN)//'.Y', STATUS='OLD', ACTION='READ', ACCESS='STREAM', FORM='UNFORMATTED', &
-       IOSTAT=INFO)
-  IF (INFO .EQ. 0) OPEN(NEWUNIT=U2, FILE=TRIM(FN)//'.J', STATUS='OLD', ACTION='READ', ACCESS='STREAM', &
-       FORM='UNFORMATTED', IOSTAT=INFO)
-  IF (INFO .NE. 0) THEN
-     WRITE (ERROR_UNIT,'(A,I11)') 'INFO=', INFO
-     ERROR STOP 'OPEN_YJ_RO'
+  STEP0 = 0
+  IF (RESUME .EQ. 1) THEN  ! the checkpoint in place of FN.Y
+     ALLOCATE(CU(N,N), CZ(N,N))
+     CALL read_state(TRIM(FN)//'.CA', Y)
+     CALL read_state(TRIM(FN)//'.CU', CU)
+     CALL read_state(TRIM(FN)//'.CZ', CZ)
+     OPEN(NEWUNIT=U1, FILE=TRIM(FN)//'.CS', STATUS='OLD', ACTION='READ', ACCESS='STREAM', FORM='UNFORMATTED', &
+          IOSTAT=INFO)
+     IF (INFO .EQ. 0) READ (U1, IOSTAT=INFO) S64
+     IF ((INFO .NE. 0) .OR. (S64 .LT. 0_c_int64_t) .OR. (S64 .GT. INT(HUGE(STEP0),c_int64_t))) THEN
+        WRITE (ERROR_UNIT,'(A,I11)') 'INFO=', INFO
+        ERROR STOP 'READ_CHECKPOINT'
+     END IF
+     CLOSE(U1)
+     STEP0 = INT(S64)
+     OPEN(NEWUNIT=U2, FILE=TRIM(FN)//'.J', STATUS='OLD', ACTION='READ', ACCESS='STREAM', FORM='UNFORMATTED', &
+          IOSTAT=INFO)
+     IF (INFO .NE. 0) THEN
+        WRITE (ERROR_UNIT,'(A,I11)') 'INFO=', INFO
+        ERROR STOP 'OPEN_YJ_RO'
+     END IF
+     READ (U2, IOSTAT=INFO) J
+     IF (INFO .NE. 0) THEN
+        WRITE (ERROR_UNIT,'(A,I11)') 'INFO=', INFO
+        ERROR STOP 'DREAD_YJ'
+     END IF
+     CLOSE(U2)
+  ELSE
+     OPEN(NEWUNIT=U1, FILE=TRIM(FN)//'.Y', STATUS='OLD', ACTION='READ', ACCESS='STREAM', FORM='UNFORMATTED', &
+          IOSTAT=INFO)
+     IF (INFO .EQ. 0) OPEN(NEWUNIT=U2, FILE=TRIM(FN)//'.J', STATUS='OLD', ACTION='READ', ACCESS='STREAM', &
+          FORM='UNFORMATTED', IOSTAT=INFO)
+     IF (INFO .NE. 0) THEN
+        WRITE (ERROR_UNIT,'(A,I11)') 'INFO=', INFO
+        ERROR STOP 'OPEN_YJ_RO'
+     END IF
+     READ (U1, IOSTAT=INFO) Y
+     IF (INFO .EQ. 0) READ (U2, IOSTAT=INFO) J
+     IF (INFO .NE. 0) THEN
+        WRITE (ERROR_UNIT,'(A,I11)') 'INFO=', INFO
+        ERROR STOP 'DREAD_YJ'
+     END IF
+     CLOSE(U2)
+     CLOSE(U1)
+     IF (CKPT .GT. 0) THEN  ! the loop's own start: U = I, Z = I
+        ALLOCATE(CU(N,N), CZ(N,N))
+        CU = 0
+        CZ = 0
+        DO I = 1, N
+           CU(I,I) = ONE
+           CZ(I,I) = ONE
+        END DO
+     END IF
   END IF
-  READ (U1, IOSTAT=INFO) Y
-  IF (INFO .EQ. 0) READ (U2, IOSTAT=INFO) J
-  IF (INFO .NE. 0) THEN
-     WRITE (ERROR_UNIT,'(A,I11)') 'INFO=', INFO
-     ERROR STOP 'DREAD_YJ'
-  END IF
-  CLOSE(U2)
-  CLOSE(U1)
 
   ! the pair census of R%TRU (src/shared/jstep.F90:32-61)
   NP = 0
@@ -148,26 +232,65 @@ PROGRAM jk
      WRITE (ERROR_UNIT,'(A,I11,2A)') 'INFO=', RC, ' ', TRIM(jkg_message(RC))
      ERROR STOP 'jkg_create'
   END IF
+  IF (FEVERY .GT. 0) THEN
+     ALLOCATE(FRM(INT(FSIZE,c_int64_t) * FSIZE * FMAX))
+     RC = jkg_set_frames(CTX, INT(FEVERY,c_int), INT(FSIZE,c_int), INT(FMAX,c_int), FRM, NFRM)
+     IF (RC .NE. JKG_OK) THEN
+        WRITE (ERROR_UNIT,'(A,I11,2A)') 'INFO=', RC, ' ', TRIM(jkg_message(RC))
+        ERROR STOP 'JKG_FRAMES'
+     END IF
+     OPEN(NEWUNIT=UF, FILE=TRIM(FN)//'.FRM', STATUS='REPLACE', ACTION='WRITE', ACCESS='STREAM', &
+          FORM='UNFORMATTED', IOSTAT=INFO)
+     IF (INFO .NE. 0) ERROR STOP 'OPEN_FRM'
+  END IF
   CALL write_trace_header(ERROR_UNIT)
+  IF (ALLOCATED(CU)) THEN  ! the loop in parts, from and to its state
+     SECS = 0.0_c_double
+     DO
+        IF (CKPT .GT. 0) OPTS%max_steps = CKPT
+        TRACE_PRINTED = 0
 #ifdef ZJK
-  IF (SOLVER .EQ. 'plain') THEN
-     RC = jkg_zsolve(CTX, INT(N,c_int), INT(N_2,c_int), Y, J, YU, ZZ, SY, OPTS, RES)
-  ELSE
-     RC = jkg_zsolve_mixed(CTX, INT(N,c_int), INT(N_2,c_int), Y, J, YU, ZZ, SY, OPTS, MOPTS, RES)
-  END IF
+        RC = jkg_zsolve_from(CTX, INT(N,c_int), INT(N_2,c_int), Y, CU, CZ, J, INT(STEP0,c_int), YU, ZZ, SY, &
+             OPTS, RES)
 #else
-  IF (SOLVER .EQ. 'plain') THEN
-     RC = jkg_dsolve(CTX, INT(N,c_int), INT(N_2,c_int), Y, J, YU, ZZ, SY, OPTS, RES)
-  ELSE
-     RC = jkg_dsolve_mixed(CTX, INT(N,c_int), INT(N_2,c_int), Y, J, YU, ZZ, SY, OPTS, MOPTS, RES)
-  END IF
+        RC = jkg_dsolve_from(CTX, INT(N,c_int), INT(N_2,c_int), Y, CU, CZ, J, INT(STEP0,c_int), YU, ZZ, SY, &
+             OPTS, RES)
 #endif
+        IF (RC .NE. JKG_OK) EXIT
+        CALL write_trace_upto(MIN(INT(RES%steps), CAP))
+        CALL write_frames()
+        SECS = SECS + RES%seconds
+        ! ended: no pivot, no effective transform, or a failed 2x2 (INFO < 0)
+        IF ((CKPT .LE. 0) .OR. (RES%info .LT. STEP0 + CKPT)) EXIT
+        STEP0 = RES%info
+        CALL write_checkpoint()
+     END DO
+     RES%seconds = SECS
+  ELSE
+#ifdef ZJK
+     IF (SOLVER .EQ. 'plain') THEN
+        RC = jkg_zsolve(CTX, INT(N,c_int), INT(N_2,c_int), Y, J, YU, ZZ, SY, OPTS, RES)
+     ELSE
+        RC = jkg_zsolve_mixed(CTX, INT(N,c_int), INT(N_2,c_int), Y, J, YU, ZZ, SY, OPTS, MOPTS, RES)
+     END IF
+#else
+     IF (SOLVER .EQ. 'plain') THEN
+        RC = jkg_dsolve(CTX, INT(N,c_int), INT(N_2,c_int), Y, J, YU, ZZ, SY, OPTS, RES)
+     ELSE
+        RC = jkg_dsolve_mixed(CTX, INT(N,c_int), INT(N_2,c_int), Y, J, YU, ZZ, SY, OPTS, MOPTS, RES)
+     END IF
+#endif
+     IF (RC .EQ. JKG_OK) THEN
+        CALL write_trace_upto(MIN(INT(RES%steps), CAP))
+        CALL write_frames()
+     END IF
+  END IF
   CALL jkg_destroy(CTX)
   IF (RC .NE. JKG_OK) THEN
      WRITE (ERROR_UNIT,'(A,I11,2A)') 'INFO=', RC, ' ', TRIM(jkg_message(RC))
      ERROR STOP 'DSTEP_LOOP'
   END IF
-  CALL write_trace_upto(MIN(INT(RES%steps), CAP))
+  IF (FEVERY .GT. 0) CLOSE(UF)
   IF (RES%info .GE. 0) THEN
      WRITE (OUTPUT_UNIT,'(A,I11,A,F13.6,A)') 'Executed ', RES%info, ' steps with transformations in ', &
           RES%seconds, ' s'
@@ -195,6 +318,69 @@ PROGRAM jk
   END IF
 
 CONTAINS
+
+  ! an N x N state matrix of a checkpoint
+  SUBROUTINE read_state(NAME, X)
+    CHARACTER(LEN=*), INTENT(IN) :: NAME
+#ifdef ZJK
+    COMPLEX(c_double_complex), INTENT(OUT) :: X(:,:)
+#else
+    REAL(c_double), INTENT(OUT) :: X(:,:)
+#endif
+    INTEGER :: U, IOS
+    OPEN(NEWUNIT=U, FILE=NAME, STATUS='OLD', ACTION='READ', ACCESS='STREAM', FORM='UNFORMATTED', IOSTAT=IOS)
+    IF (IOS .EQ. 0) READ (U, IOSTAT=IOS) X
+    IF (IOS .NE. 0) THEN
+       WRITE (ERROR_UNIT,'(A,I11,2A)') 'INFO=', IOS, ' ', NAME
+       ERROR STOP 'READ_CHECKPOINT'
+    END IF
+    CLOSE(U)
+  END SUBROUTINE read_state
+
+  ! FN.CS := -1, FN.CA, FN.CU, FN.CZ := the loop's state, FN.CS := STEP0
+  SUBROUTINE write_checkpoint()
+    INTEGER :: U, IOS
+    CALL write_step(-1_c_int64_t)
+    OPEN(NEWUNIT=U, FILE=TRIM(FN)//'.CA', STATUS='REPLACE', ACTION='WRITE', ACCESS='STREAM', &
+         FORM='UNFORMATTED', IOSTAT=IOS)
+    IF (IOS .EQ. 0) WRITE (U, IOSTAT=IOS) Y
+    IF (IOS .EQ. 0) CLOSE(U)
+    IF (IOS .EQ. 0) OPEN(NEWUNIT=U, FILE=TRIM(FN)//'.CU', STATUS='REPLACE', ACTION='WRITE', ACCESS='STREAM', &
+         FORM='UNFORMATTED', IOSTAT=IOS)
+    IF (IOS .EQ. 0) WRITE (U, IOSTAT=IOS) CU
+    IF (IOS .EQ. 0) CLOSE(U)
+    IF (IOS .EQ. 0) OPEN(NEWUNIT=U, FILE=TRIM(FN)//'.CZ', STATUS='REPLACE', ACTION='WRITE', ACCESS='STREAM', &
+         FORM='UNFORMATTED', IOSTAT=IOS)
+    IF (IOS .EQ. 0) WRITE (U, IOSTAT=IOS) CZ
+    IF (IOS .EQ. 0) CLOSE(U)
+    IF (IOS .NE. 0) THEN
+       WRITE (ERROR_UNIT,'(A,I11)') 'INFO=', IOS
+       ERROR STOP 'WRITE_CHECKPOINT'
+    END IF
+    CALL write_step(INT(STEP0,c_int64_t))
+  END SUBROUTINE write_checkpoint
+
+  SUBROUTINE write_step(S)
+    INTEGER(c_int64_t), INTENT(IN) :: S
+    INTEGER :: U, IOS
+    OPEN(NEWUNIT=U, FILE=TRIM(FN)//'.CS', STATUS='REPLACE', ACTION='WRITE', ACCESS='STREAM', &
+         FORM='UNFORMATTED', IOSTAT=IOS)
+    IF (IOS .EQ. 0) WRITE (U, IOSTAT=IOS) S
+    IF (IOS .EQ. 0) CLOSE(U)
+    IF (IOS .NE. 0) THEN
+       WRITE (ERROR_UNIT,'(A,I11)') 'INFO=', IOS
+       ERROR STOP 'WRITE_CHECKPOINT'
+    END IF
+  END SUBROUTINE write_step
+
+  ! the frames of the last solve call, appended to FN.FRM
+  SUBROUTINE write_frames()
+    INTEGER :: IOS
+    IF (FEVERY .LE. 0) RETURN
+    IF (NFRM .LE. 0) RETURN
+    WRITE (UF, IOSTAT=IOS) FRM(1:INT(NFRM,c_int64_t) * FSIZE * FSIZE)
+    IF (IOS .NE. 0) ERROR STOP 'WRITE_FRM'
+  END SUBROUTINE write_frames
 
   ! positional command line, READCL semantics (src/shared/readcl.F90:1-85)
   SUBROUTINE READCL(FN, N, N_2, ID_NCP, ID_TRU, INFO)

@@ -80,6 +80,42 @@ MODULE jkg_c
        TYPE(jkg_result), INTENT(OUT) :: res
      END FUNCTION jkg_dsolve
 
+     ! the loop resumed from its state (a, u, z) at step step0; (a, u, z) returned after the last step
+     INTEGER(c_int) FUNCTION jkg_dsolve_from(ctx, n, n2, a, u, z, j, step0, yu, zz, sy, opts, res) &
+          BIND(C, NAME='jkg_dsolve_from')
+       IMPORT :: c_int, c_ptr, c_double, c_int64_t, jkg_opts, jkg_result
+       TYPE(c_ptr), VALUE :: ctx
+       INTEGER(c_int), VALUE :: n, n2, step0
+       REAL(c_double), INTENT(INOUT) :: a(*), u(*), z(*)
+       INTEGER(c_int64_t), INTENT(IN) :: j(*)
+       REAL(c_double), INTENT(OUT) :: yu(*), zz(*), sy(*)
+       TYPE(jkg_opts), INTENT(IN) :: opts
+       TYPE(jkg_result), INTENT(OUT) :: res
+     END FUNCTION jkg_dsolve_from
+
+     INTEGER(c_int) FUNCTION jkg_zsolve_from(ctx, n, n2, a, u, z, j, step0, yu, zz, sy, opts, res) &
+          BIND(C, NAME='jkg_zsolve_from')
+       IMPORT :: c_int, c_ptr, c_double, c_double_complex, c_int64_t, jkg_opts, jkg_result
+       TYPE(c_ptr), VALUE :: ctx
+       INTEGER(c_int), VALUE :: n, n2, step0
+       COMPLEX(c_double_complex), INTENT(INOUT) :: a(*), u(*), z(*)
+       INTEGER(c_int64_t), INTENT(IN) :: j(*)
+       COMPLEX(c_double_complex), INTENT(OUT) :: yu(*), zz(*)
+       REAL(c_double), INTENT(OUT) :: sy(*)
+       TYPE(jkg_opts), INTENT(IN) :: opts
+       TYPE(jkg_result), INTENT(OUT) :: res
+     END FUNCTION jkg_zsolve_from
+
+     ! pictures of |A| during later solves (size x size bytes every `every` steps)
+     INTEGER(c_int) FUNCTION jkg_set_frames(ctx, every, fsize, max_frames, frames, count) &
+          BIND(C, NAME='jkg_set_frames')
+       IMPORT :: c_int, c_ptr, c_int8_t, c_int32_t
+       TYPE(c_ptr), VALUE :: ctx
+       INTEGER(c_int), VALUE :: every, fsize, max_frames
+       INTEGER(c_int8_t), INTENT(OUT) :: frames(*)
+       INTEGER(c_int32_t), INTENT(OUT) :: count
+     END FUNCTION jkg_set_frames
+
      INTEGER(c_int) FUNCTION jkg_dsolve_mixed(ctx, n, n2, y, j, yu, zz, sy, opts, m, res) &
           BIND(C, NAME='jkg_dsolve_mixed')
        IMPORT :: c_int, c_ptr, c_double, c_int64_t, jkg_opts, jkg_mixed_opts, jkg_result

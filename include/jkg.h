@@ -25,7 +25,8 @@ extern "C" {
 #define JKG_ENOMEM (-1001)   /* device allocation failed */
 #define JKG_ENOTSUP (-1002)  /* option not implemented on the GPU (e.g. ID_NCP=3) */
 #define JKG_EJ (-1003)       /* a J entry outside {-1,+1} (the reference: R%TRU INFO=-2) */
-#define JKG_ENOCONV (-1004)  /* the step loop ran past its guard of 1000 N + 1000 steps */
+#define JKG_ENOCONV (-1004)  /* the step loop ran past its guard of 1000 N + 1000 steps (1000 NN + 1000 for
+                                the one-pair cyclic sweeps) */
 
 typedef struct jkg_ctx jkg_ctx;
 
@@ -109,6 +110,28 @@ int jkg_dsolve(jkg_ctx *ctx, int n, int n2, const double *y, const int64_t *j, d
                double *sy, const jkg_opts *opts, jkg_result *res);
 int jkg_zsolve(jkg_ctx *ctx, int n, int n2, const double *y, const int64_t *j, double *yu, double *zz,
                double *sy, const jkg_opts *opts, jkg_result *res);
+
+/* jkg_dsolve_from / jkg_zsolve_from: checkpoint and resume of the same loop.
+ * DSTEP_LOOP (src/D/dstep.F90:472-604; ZSTEP_LOOP, src/Z/zstep.F90) started from
+ * its own state instead of (Y, I, I): a (the iterate A), u and z (the factors U
+ * and Z as the steps accumulate them, before the finalisation's J Z^T J), all
+ * N x N column-major, complex interleaved for the z variant, and the index step0
+ * of the next step (0 with a = Y, u = z = I: jkg_dsolve's run).  The loop runs
+ * until it ends or, with opts->max_steps = K >= 0, until K steps have run; then
+ * (a, u, z) hold its state after the last step -- the checkpoint to resume from
+ * with step0 = res->info -- and yu, zz, sy the finalised outputs of that state,
+ * as jkg_dsolve writes them.  The loop ended (no pivot, no effective transform
+ * or a failed 2x2) iff max_steps < 0 or res->info < step0 + max_steps.  Steps
+ * keep their absolute index: the trace's STEP and res->info continue from step0,
+ * res->steps counts this call's records (trace[0] is step step0), and frames
+ * (jkg_set_frames) are taken at the steps s = 0 mod every.  Resuming is
+ * bit-identical to running on: the state is all the loop carries from one step
+ * to the next.  The plain solver's options (id_ncp, id_tru, fused) apply.
+ * Argument errors: -(position). */
+int jkg_dsolve_from(jkg_ctx *ctx, int n, int n2, double *a, double *u, double *z, const int64_t *j, int step0,
+                    double *yu, double *zz, double *sy, const jkg_opts *opts, jkg_result *res);
+int jkg_zsolve_from(jkg_ctx *ctx, int n, int n2, double *a, double *u, double *z, const int64_t *j, int step0,
+                    double *yu, double *zz, double *sy, const jkg_opts *opts, jkg_result *res);
 
 /* jkg_dsolve_noalloc: jkg_dsolve that allocates nothing.  No counterpart in the
  * reference: for callers that cannot allocate device memory (code under HIP graph
@@ -219,9 +242,13 @@ typedef struct {
                          0: the reference's test only */
   float *ctl_trace;   /* per-step controller records (JKG_CTL_REC floats each), or NULL */
   int32_t ctl_trace_cap;
-  int32_t ordering;   /* 0: dynamic (the reference's, default); 1 (real): cyclic round-robin sweeps, N/2
-                         disjoint pairs per step, no weights, sort or walk, ended by a sweep without a
-                         transform -- the classic baseline (no controller or threshold) */
+  int32_t ordering;   /* 0: dynamic (the reference's, default).  Cyclic sweep baselines (real; no
+                         weights, sort or walk, no controller or threshold; a whole sweep without a
+                         transform ends the run): 1 round robin, N/2 disjoint pairs a step; 2 and 3
+                         the reference's pointwise sweeps (src/old/jk1.f90:101-146), one pair a
+                         step, row-cyclic (1,2), (1,3), .., (2,3), .. or column-cyclic (1,2), (1,3),
+                         (2,3), (1,4), .., with the reference's 2x2 and transforms (DHSVD2,
+                         DSTEP_TRANSF) on the full matrix */
   struct jkg_warm_info *warm; /* out (fp32_warm): what the warm start did, or NULL */
   double ctl_tau_max; /* largest threshold a controller may set (0: 1e-6, the network's range) */
   float ctl_handoff;  /* log10(TW / TW_0) below which a controller hands every step back to the plain

@@ -430,7 +430,7 @@ double jkg_key_weight(uint64_t key) {
 // ---------------------------------------------------------------- problem setup
 static int upload_problem(jkg_ctx *c, int n, int n2, const void *y, const int64_t *j, int *npos, int *nneg,
                           int max_steps, int trace_cap, double tau = 0.0, double tau_cap = 1e-6,
-                          float handoff = -10.f) {
+                          float handoff = -10.f, int step0 = 0) {
   // (the element size follows the buffers: c->d.cplx)
   DeviceStep &d = c->d;
   std::vector<int> J32(n), pl, mi;
@@ -449,7 +449,9 @@ static int upload_problem(jkg_ctx *c, int n, int n2, const void *y, const int64_
   if (!pl.empty()) HIPCHK(hipMemcpyAsync(d.plus, pl.data(), sizeof(int) * pl.size(), hipMemcpyHostToDevice, c->stream));
   if (!mi.empty()) HIPCHK(hipMemcpyAsync(d.minus, mi.data(), sizeof(int) * mi.size(), hipMemcpyHostToDevice, c->stream));
   StepState s{};
-  s.max_steps = max_steps;
+  s.step = step0;  // a resumed loop continues the step count (the trace's STEP, INFO)
+  s.step0 = step0;
+  s.max_steps = (max_steps >= 0) ? step0 + max_steps : -1;
   s.n2 = n2;
   s.trace_cap = trace_cap;
   s.tau = tau;
@@ -494,13 +496,14 @@ static int capture_step(jkg_ctx *c, int fused) {
 
 // Launches the captured step of the current precision until the device reports
 // done, or until `remaining` (if >= 0) steps are used up.  The reference needs
-// about 10 N steps; a run that has not ended after 1000 N + 1000 steps is
-// stopped and reported (a guard, never reached by a converging run).
+// about 10 N steps (10 NN with the one-pivot cyclic sweeps); a run that has not
+// ended after 1000 times that + 1000 steps is stopped and reported (a guard,
+// never reached by a converging run).
 static int launch_until_done(jkg_ctx *c, const jkg_opts *o, long &remaining, long &launched) {
   DeviceStep &d = c->d;
   hipGraphExec_t exec = c->exec[d.prec32 ? 1 : 0];
   int batch = o->steps_per_sync > 0 ? o->steps_per_sync : 16;
-  const long cap = 1000L * d.n + 1000;
+  const long cap = 1000L * (d.cyclic >= 2 ? (long)d.nn : (long)d.n) + 1000;
   for (;;) {
     const long nb = (remaining >= 0 && remaining < batch) ? remaining : batch;
     for (long k = 0; k < nb; ++k) HIPCHK(hipGraphLaunch(exec, c->stream));
@@ -525,10 +528,23 @@ static int launch_until_done(jkg_ctx *c, const jkg_opts *o, long &remaining, lon
   }
 }
 
-// After the last step: the end event, finalisation and the result record.
-static int finish_run(jkg_ctx *c, jkg_result *res) {
+// The loop's own state (A, U, Z before the finalisation), in and out of a resumed loop.
+struct LoopState {
+  void *a, *u, *z;
+  int step0;
+};
+
+// After the last step: the end event, the loop's state to the caller of a resumed
+// loop, finalisation and the result record.
+static int finish_run(jkg_ctx *c, jkg_result *res, const LoopState *ls = nullptr) {
   DeviceStep &d = c->d;
   HIPCHK(hipEventRecord(c->ev1, c->stream));
+  if (ls) {
+    const size_t bytes = (d.cplx ? 16 : 8) * (size_t)d.n * d.n;
+    HIPCHK(hipMemcpyAsync(ls->a, d.A, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(ls->u, d.U, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(ls->z, d.Z, bytes, hipMemcpyDeviceToHost, c->stream));
+  }
   finalize_launch(d, c->stream, c->sigma);
   HIPCHK(hipGetLastError());
   StepState s;
@@ -549,12 +565,18 @@ static int finish_run(jkg_ctx *c, jkg_result *res) {
 
 // With a step limit K, exactly K step graphs are launched and then the
 // step-begin kernel alone, which records the end (no idle graphs).
-static int run_loop(jkg_ctx *c, const jkg_opts *o, jkg_result *res) {
+static int run_loop(jkg_ctx *c, const jkg_opts *o, jkg_result *res, const LoopState *ls = nullptr) {
   DeviceStep &d = c->d;
   d.prec32 = 0;
   int rc = capture_step(c, o->fused ? 1 : 0);
   if (rc) return rc;
-  init_identity_launch(d, c->stream);
+  if (ls) {  // U and Z of the resumed loop (A came with the problem)
+    const size_t bytes = (d.cplx ? 16 : 8) * (size_t)d.n * d.n;
+    HIPCHK(hipMemcpyAsync(d.U, ls->u, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(d.Z, ls->z, bytes, hipMemcpyHostToDevice, c->stream));
+  } else {
+    init_identity_launch(d, c->stream);
+  }
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(c->ev0, c->stream));
   long launched = 0, remaining = (o->max_steps >= 0) ? (long)o->max_steps : -1;
@@ -564,7 +586,7 @@ static int run_loop(jkg_ctx *c, const jkg_opts *o, jkg_result *res) {
     step_begin_launch(d, c->stream);
     HIPCHK(hipGetLastError());
   }
-  return finish_run(c, res);
+  return finish_run(c, res, ls);
 }
 
 // Newton-Schulz iteration for the polar factor of X in the unitary group (J
@@ -709,7 +731,8 @@ static int run_warm(jkg_ctx *c, const jkg_opts *o, jkg_result *res, const MixedC
 }
 
 static int solve_common(jkg_ctx *c, int cplx, int n, int n2, const void *y, const int64_t *j, void *yu, void *zz,
-                        double *sy, const jkg_opts *opts, jkg_result *res, const MixedCfg &mc = MixedCfg{}) {
+                        double *sy, const jkg_opts *opts, jkg_result *res, const MixedCfg &mc = MixedCfg{},
+                        const LoopState *ls = nullptr) {
   const int wmode = mc.wmode;
   const double tol = mc.tol, tau = mc.tau;
   if (!c) return -1;
@@ -778,13 +801,14 @@ static int solve_common(jkg_ctx *c, int cplx, int n, int n2, const void *y, cons
   }
   // TT: the number of trigonometric pairs (the reference's R%TRU result)
   int npos = 0, nneg = 0;
-  rc = upload_problem(c, n, n2, y, j, &npos, &nneg, o.max_steps, c->trace_cap, tau, mc.tau_cap, mc.handoff);
+  rc = upload_problem(c, n, n2, y, j, &npos, &nneg, o.max_steps, c->trace_cap, tau, mc.tau_cap, mc.handoff,
+                      ls ? ls->step0 : 0);
   if (rc) return rc;
   res->npos = npos;
   res->nneg = nneg;
   res->tt = (int)((int64_t)npos * (npos - 1) / 2 + (int64_t)nneg * (nneg - 1) / 2);
   c->streamed = 0;
-  rc = mc.warm ? run_warm(c, &o, res, mc) : run_loop(c, &o, res);
+  rc = mc.warm ? run_warm(c, &o, res, mc) : run_loop(c, &o, res, ls);
   if (rc) return rc;
   DeviceStep &d = c->d;
   if (o.trace && o.trace_cap > 0) {
@@ -792,7 +816,10 @@ static int solve_common(jkg_ctx *c, int cplx, int n, int n2, const void *y, cons
     if (cnt > 0) HIPCHK(hipMemcpy(o.trace, d.trace, sizeof(jkg_steprec) * cnt, hipMemcpyDeviceToHost));
   }
   if (c->d.frames && c->frames_host) {
-    const int cnt = res->steps > 0 ? std::min((res->steps - 1) / c->frame_every + 1, c->frame_max) : 0;
+    // the steps s = 0 mod every among this run's step0 .. step0 + steps - 1
+    const int s0 = ls ? ls->step0 : 0, ev = c->frame_every;
+    const int first = (s0 + ev - 1) / ev, last = (s0 + res->steps - 1) / ev;
+    const int cnt = res->steps > 0 ? std::min(std::max(last - first + 1, 0), c->frame_max) : 0;
     if (cnt > 0)
       HIPCHK(hipMemcpy(c->frames_host, c->frames_dev, (size_t)cnt * c->frame_size * c->frame_size,
                        hipMemcpyDeviceToHost));
@@ -817,6 +844,38 @@ extern "C" int jkg_dsolve(jkg_ctx *c, int n, int n2, const double *y, const int6
   return solve_common(c, 0, n, n2, y, j, yu, zz, sy, opts, res);
 }
 
+// DSTEP_LOOP resumed from its own state (src/D/dstep.F90:472-604 and src/Z/zstep.F90
+// after the identity set-up): (a, u, z) in place of (Y, I, I), steps counted from step0.
+static int solve_from(jkg_ctx *c, int cplx, int n, int n2, void *a, void *u, void *z, const int64_t *j, int step0,
+                      void *yu, void *zz, double *sy, const jkg_opts *opts, jkg_result *res) {
+  // negative argument positions, as for jkg_dsolve
+  if (!c) return -1;
+  if (n < 3) return -2;
+  if (!a) return -4;
+  if (!u) return -5;
+  if (!z) return -6;
+  if (!j) return -7;
+  if (step0 < 0) return -8;
+  if (!yu) return -9;
+  if (!zz) return -10;
+  if (!sy) return -11;
+  if (!res) return -13;
+  LoopState ls{a, u, z, step0};
+  return solve_common(c, cplx, n, n2, a, j, yu, zz, sy, opts, res, MixedCfg{}, &ls);
+}
+
+extern "C" int jkg_dsolve_from(jkg_ctx *c, int n, int n2, double *a, double *u, double *z, const int64_t *j,
+                               int step0, double *yu, double *zz, double *sy, const jkg_opts *opts,
+                               jkg_result *res) {
+  return solve_from(c, 0, n, n2, a, u, z, j, step0, yu, zz, sy, opts, res);
+}
+
+extern "C" int jkg_zsolve_from(jkg_ctx *c, int n, int n2, double *a, double *u, double *z, const int64_t *j,
+                               int step0, double *yu, double *zz, double *sy, const jkg_opts *opts,
+                               jkg_result *res) {
+  return solve_from(c, 1, n, n2, a, u, z, j, step0, yu, zz, sy, opts, res);
+}
+
 extern "C" void jkg_mixed_opts_default(jkg_mixed_opts *m) {
   memset(m, 0, sizeof(*m));
   m->weights = JKG_W_GRAM;
@@ -836,12 +895,12 @@ extern "C" int jkg_dsolve_mixed(jkg_ctx *c, int n, int n2, const double *y, cons
   if (m.controller < 0 || m.controller > 2 || (m.controller && !m.ctl) || m.ctl_trace_cap < 0) return -10;
   if (!(m.ctl_tau_max >= 0.0 && m.ctl_tau_max <= 1e-6)) return -10;
   if (!(m.ctl_handoff >= -16.f && m.ctl_handoff <= 0.f)) return -10;
-  if (m.ordering < 0 || m.ordering > 1) return -10;
+  if (m.ordering < 0 || m.ordering > 3) return -10;
   if (m.fp32_warm < 0 || m.fp32_warm > 1 || (m.key_bits != 32 && m.key_bits != 64)) return -10;
   if (m.fp32_warm && !(m.switch_tol > 0.0 && m.switch_tol < 1.0)) return -10;
   if (m.key_bits == 32 && !m.fp32_warm) return JKG_ENOTSUP;  // 32-bit keys: the warm start only
   if (m.weights == JKG_W_PAIR && (m.tol != 0.0 || m.controller || m.fp32_warm)) return JKG_ENOTSUP;
-  if (m.ordering == 1 && (m.controller || m.threshold != 0.0 || m.fp32_warm)) return JKG_ENOTSUP;
+  if (m.ordering != 0 && (m.controller || m.threshold != 0.0 || m.fp32_warm)) return JKG_ENOTSUP;
   MixedCfg mc;
   mc.warm = m.fp32_warm;
   mc.cut32 = m.key_bits == 32;

@@ -101,14 +101,15 @@ namespace jkg {
 // records it) an S x S map, pixel (u, v) = the largest |a_ij| over the rows
 // and columns that map to it, on a fixed logarithmic scale: 8 levels per
 // decade from 1e-16 (level 1) to 1e+15.875 (255); 0 for exact zeros.  One
-// workgroup per pixel row; frame f goes to frames[f * S * S], row-major.
+// workgroup per pixel row; the f-th frame of the run goes to frames[f * S * S], row-major.
 template <class E>
 __global__ void __launch_bounds__(256) k_frame(int n, const E *__restrict__ A, int lda, int size, int every,
                                                int max_frames, uint8_t *__restrict__ frames, const StepState *st) {
   if (!st->active) return;
-  const int s = st->step;
-  if (s % every != 0 || s / every >= max_frames) return;
-  uint8_t *f = frames + (size_t)(s / every) * size * size + (size_t)blockIdx.x * size;
+  // frames at the steps s = 0 mod every; a resumed loop (step0 > 0) numbers its own from 0
+  const int s = st->step, first = (st->step0 + every - 1) / every;
+  if (s % every != 0 || s / every - first >= max_frames) return;
+  uint8_t *f = frames + (size_t)(s / every - first) * size * size + (size_t)blockIdx.x * size;
   const int u = blockIdx.x;
   const int r0 = (int)((long)u * n / size), r1 = (int)((long)(u + 1) * n / size);
   for (int v = threadIdx.x; v < size; v += blockDim.x) {

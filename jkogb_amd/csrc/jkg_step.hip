@@ -833,8 +833,8 @@ __global__ void __launch_bounds__(256) k_step_end(const int *__restrict__ J, con
       rec.newlen = -1;
       rec.tw = -0.0;
       rec.trig = rec.hyp = 0;
-      if (s < st->trace_cap) trace[s] = rec;
-      st->nrec = s + 1;
+      if (s - st->step0 < st->trace_cap) trace[s - st->step0] = rec;
+      st->nrec = s - st->step0 + 1;
       st->done = 1;
       st->info = s;
       if (ctl_on) controller_features(st, s, n, nn, 0, 0, 0, 0, -0.0, piv_pos, ctl_trace, ctl_cap);
@@ -897,8 +897,8 @@ __global__ void __launch_bounds__(256) k_step_end(const int *__restrict__ J, con
     rec.tw = tw;
     rec.trig = (err == 0) ? cnt[1] : 0;
     rec.hyp = (err == 0) ? cnt[2] : 0;
-    if (s < st->trace_cap) trace[s] = rec;
-    st->nrec = s + 1;
+    if (s - st->step0 < st->trace_cap) trace[s - st->step0] = rec;
+    st->nrec = s - st->step0 + 1;
     st->acc_b2 += (unsigned long long)cnt[3];
     st->acc_b4 += (unsigned long long)cnt[4];
     st->nl = nl;
@@ -956,13 +956,47 @@ __global__ void k_finalize(double *A, double *Z, int n, int lda, const int *J, d
 }
 
 // ------------------------------------------------------------------ cyclic ordering
-// The classic sweep baseline: step s takes round s mod (M - 1) of the round-robin
-// schedule over M = N (+1 if N is odd: a dummy index) positions -- pair k pairs
-// (r + k) and (r - k) mod (M - 1), pair 0 pairs M - 1 with r -- so every round is
-// N/2 disjoint pairs and M - 1 rounds are one sweep over all pairs.  No weights,
-// sort or walk; positions 0 mark the pivots as cyclic for k_pivots.
-__global__ void k_cyc_select(int n, int *piv_p, int *piv_q, int *piv_pos, StepState *st) {
+// The sweep baselines (no weights, sort or walk; positions 0 mark the pivots as
+// cyclic for k_pivots).  order 1, round robin: step s takes round s mod (M - 1)
+// of the schedule over M = N (+1 if N is odd: a dummy index) positions -- pair k
+// pairs (r + k) and (r - k) mod (M - 1), pair 0 pairs M - 1 with r -- so every
+// round is N/2 disjoint pairs and M - 1 rounds are one sweep over all pairs.
+// order 2 and 3, the reference's pointwise sweeps (src/old/jk1.f90:101-146): one
+// pivot a step, pair s mod NN of the row-cyclic order (1,2), (1,3), .., (1,N),
+// (2,3), .. or of the column-cyclic order (1,2), (1,3), (2,3), (1,4), ..
+__device__ inline long rows_before(long i, long n) { return (i - 1) * (2 * n - i) / 2; }  // pairs in rows < i
+__device__ inline long cols_before(long j) { return (j - 1) * (j - 2) / 2; }              // pairs in columns < j
+
+__global__ void k_cyc_select(int n, int order, int *piv_p, int *piv_q, int *piv_pos, StepState *st) {
   if (!st->active) return;
+  if (order >= 2) {
+    if (threadIdx.x == 0) {
+      const long nn = (long)n * (n - 1) / 2, t = st->step % nn;
+      long p, q;
+      if (order == 2) {  // the row i (1-based) with rows_before(i) <= t < rows_before(i + 1)
+        long i = (long)((2.0 * n - 1.0 - __builtin_sqrt((2.0 * n - 1.0) * (2.0 * n - 1.0) - 8.0 * t)) / 2.0) + 1;
+        i = i < 1 ? 1 : (i > n - 1 ? n - 1 : i);
+        while (i > 1 && rows_before(i, n) > t) --i;
+        while (i < n - 1 && rows_before(i + 1, n) <= t) ++i;
+        p = i;
+        q = i + 1 + (t - rows_before(i, n));
+      } else {  // the column j with cols_before(j) <= t < cols_before(j + 1)
+        long j = (long)((3.0 + __builtin_sqrt(1.0 + 8.0 * t)) / 2.0);
+        j = j < 2 ? 2 : (j > n ? n : j);
+        while (j > 2 && cols_before(j) > t) --j;
+        while (j < n && cols_before(j + 1) <= t) ++j;
+        p = 1 + (t - cols_before(j));
+        q = j;
+      }
+      piv_p[0] = (int)p - 1;
+      piv_q[0] = (int)q - 1;
+      piv_pos[0] = 0;
+      st->sl = 1;
+      st->it = 0;
+      st->sel_done = 1;
+    }
+    return;
+  }
   const int m = n + (n & 1), r = st->step % (m - 1);
   for (int k = threadIdx.x; k < m / 2; k += blockDim.x) {
     const int x = (k == 0) ? m - 1 : (r + k) % (m - 1);
@@ -1115,7 +1149,8 @@ void rotate_launch(DeviceStep &d, hipStream_t s) {
 }
 
 void step_end_launch(DeviceStep &d, hipStream_t s) {
-  const int sweep = d.cyclic ? d.n + (d.n & 1) - 1 : 0;
+  // cyclic orderings: the steps of one sweep (a sweep without a transform ends the run)
+  const int sweep = d.cyclic == 1 ? d.n + (d.n & 1) - 1 : (d.cyclic ? (int)d.nn : 0);
   hipLaunchKernelGGL(k_step_end, dim3(1), dim3(256), 0, s, d.J, d.piv_p, d.piv_q, d.piv_info, d.piv_w, d.st,
                      d.trace, d.n, d.nn, d.piv_pos, d.ctl_mode != 0 ? 1 : 0, d.ctl_trace, d.ctl_trace_cap, sweep);
 }
@@ -1138,7 +1173,7 @@ void step_launch(DeviceStep &d, hipStream_t s) {
   step_begin_launch(d, s);
   frame_launch(d, s);
   if (d.cyclic) {
-    hipLaunchKernelGGL(k_cyc_select, dim3(1), dim3(256), 0, s, d.n, d.piv_p, d.piv_q, d.piv_pos, d.st);
+    hipLaunchKernelGGL(k_cyc_select, dim3(1), dim3(256), 0, s, d.n, d.cyclic, d.piv_p, d.piv_q, d.piv_pos, d.st);
     pivots_launch(d, s);
     rotate_launch(d, s);
     step_end_launch(d, s);

@@ -59,7 +59,8 @@ struct StepState {
   int consumed;   // entries of the current list already walked
   int next_len;   // length of the list the last filter wrote (the next round's)
   int trace_cap;  // capacity of the trace buffer
-  int nrec;       // trace records written (last step index + 1)
+  int nrec;       // trace records written by this run (last step index - step0 + 1)
+  int step0;      // index of the first step of this run (a resumed loop starts past 0)
   int ntrig, nhyp;
   double tw;
   double tau;     // > 0: the walk stops at the first weight below tau * (largest weight) (mixed solver)

@@ -113,6 +113,10 @@ _strerror = _sig("jkg_strerror", C.c_char_p, [_i])
 _last_hip = _sig("jkg_last_hip_error", _i, [])
 _dsolve = _sig("jkg_dsolve", _i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, C.POINTER(Opts), C.POINTER(Result)])
 _zsolve = _sig("jkg_zsolve", _i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, C.POINTER(Opts), C.POINTER(Result)])
+_dsolve_from = _sig("jkg_dsolve_from", _i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, C.POINTER(Opts),
+                                            C.POINTER(Result)])
+_zsolve_from = _sig("jkg_zsolve_from", _i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, C.POINTER(Opts),
+                                            C.POINTER(Result)])
 _dsolve_noalloc = _sig("jkg_dsolve_noalloc", _i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, C.POINTER(Opts),
                                                   C.POINTER(Result)])
 _srt = _sig("jkg_srt", _i, [_vp, _i, _i, _vp, C.POINTER(C.c_int64)])
@@ -156,7 +160,7 @@ EXPORTS = ["jkg_create", "jkg_destroy", "jkg_strerror", "jkg_last_hip_error", "j
            "jkg_key_weight", "jkg_mixed_opts_default", "jkg_dsolve_mixed", "jkg_dweights_mixed",
            "jkg_dprofile_mixed", "jkg_ctl_grad", "jkg_ctl_fit", "jkg_batch_opts_default", "jkg_dsolve_batched",
            "jkg_zsolve_mixed", "jkg_zweights_mixed", "jkg_zprofile", "jkg_zprofile_mixed", "jkg_ncp3", "jkg_dgen",
-           "jkg_set_select", "jkg_set_frames", "jkg_dsolve_noalloc"]
+           "jkg_set_select", "jkg_set_frames", "jkg_dsolve_noalloc", "jkg_dsolve_from", "jkg_zsolve_from"]
 
 
 def mixed_opts(**kw):
@@ -273,6 +277,36 @@ class Context:
              "jkg_zsolve" if cplx else "jkg_dsolve")
         return dict(YU=YU, ZZ=ZZ, SY=SY, info=r.info, trace=trace[:min(r.steps, trace_cap)], tt=r.tt,
                     pn=(r.npos, r.nneg), n2=r.n2, seconds=r.seconds, steps=r.steps, **_counts(r))
+
+    def dsolve_from(self, A, U, Z, J, step0=0, n2=0, id_ncp=0, id_tru=0, fused=False, max_steps=-1,
+                    trace_cap=None, steps_per_sync=0, progress=None):
+        """The loop resumed from its state (jkg_dsolve_from / jkg_zsolve_from): A, U, Z (N x N, the
+        loop's own, as a previous call returned them; Y, I, I and step0 = 0 to start) and the next
+        step's index.  Returns dsolve's dict plus the state after the last step (A, U, Z) and
+        ended (False: stopped by max_steps; resume with step0 = info)."""
+        cplx = np.iscomplexobj(A)
+        dt = np.complex128 if cplx else np.float64
+        A, U, Z = (np.array(x, dtype=dt, order="F", copy=True) for x in (A, U, Z))
+        n = A.shape[0]
+        if A.shape != (n, n) or U.shape != (n, n) or Z.shape != (n, n):
+            raise ValueError("A, U and Z must be N x N")
+        J64 = np.ascontiguousarray(J, dtype=np.int64)
+        YU = np.zeros((n, n), dt, order="F")
+        ZZ = np.zeros((n, n), dt, order="F")
+        SY = np.zeros(n)
+        if trace_cap is None:
+            trace_cap = 64 * n + 64 if max_steps < 0 else max_steps + 1
+        trace = np.zeros(max(trace_cap, 1), STEPREC_DTYPE)
+        o = Opts(id_ncp, id_tru, 1 if fused else 0, max_steps, steps_per_sync, trace_cap, _p(trace))
+        keep = _hook(o, progress)
+        r = Result()
+        f = _zsolve_from if cplx else _dsolve_from
+        _chk(f(self._h, n, n2, _p(A), _p(U), _p(Z), _p(J64), step0, _p(YU), _p(ZZ), _p(SY), C.byref(o),
+               C.byref(r)), "jkg_zsolve_from" if cplx else "jkg_dsolve_from")
+        ended = max_steps < 0 or r.info < step0 + max_steps
+        return dict(A=A, U=U, Z=Z, YU=YU, ZZ=ZZ, SY=SY, info=r.info, ended=ended,
+                    trace=trace[:min(r.steps, trace_cap)], tt=r.tt, pn=(r.npos, r.nneg), n2=r.n2,
+                    seconds=r.seconds, steps=r.steps, **_counts(r))
 
     def dsolve_mixed(self, Y, J, n2=0, fused=False, max_steps=-1, trace_cap=None, steps_per_sync=0,
                      ctl_params=None, ctl_table=None, ctl_trace=False, progress=None, **mixed):

@@ -92,7 +92,7 @@ int32_t PFX##zstep_loop(int n, jk_z *U, int ldu, jk_z *A, int lda, jk_z *Z,     
                         jk_steprec *trace, int max_trace, int max_steps);                \
 int32_t PFX##dcyc_loop(int n, double *U, int ldu, double *A, int lda, double *Z, int ldz, \
                        const int32_t *J, double *sigma, double *W, jk_steprec *trace,    \
-                       int max_trace, int max_steps);
+                       int max_trace, int max_steps, int order);
 
 JK_ORACLE_DECL(orc_)
 JK_ORACLE_DECL(orcf_)

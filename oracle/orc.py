@@ -163,9 +163,10 @@ def solve(Y, J, n2=0, id_ncp=0, id_tru=1, fused=False, max_steps=-1, max_trace=N
                 trace=trace[:min(nsteps if nsteps else max_trace, max_trace)], tt=tt, pn=PN, A=A)
 
 
-def solve_cyclic(Y, J, fused=False, max_steps=-1, max_trace=None):
-    """The mixed solver's cyclic sweep baseline on the CPU (dcyc_loop, no convergence
-    tolerance).  Returns dict(YU, ZZ, SY, info, trace)."""
+def solve_cyclic(Y, J, fused=False, max_steps=-1, max_trace=None, order=1):
+    """The mixed solver's cyclic sweep baselines on the CPU (dcyc_loop, no convergence
+    tolerance): order 1 round robin, 2 row-cyclic and 3 column-cyclic pointwise (the
+    reference's sweeps, src/old/jk1.f90:101-146).  Returns dict(YU, ZZ, SY, info, trace, A)."""
     A = np.array(Y, dtype=np.float64, order="F", copy=True)
     n = A.shape[0]
     U = np.zeros((n, n), order="F")
@@ -179,10 +180,10 @@ def solve_cyclic(Y, J, fused=False, max_steps=-1, max_trace=None):
     f = getattr(lib(), pfx(fused) + "dcyc_loop")
     f.restype = C.c_int32
     f.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
-                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int]
-    info = f(n, _p(U), n, _p(A), n, _p(Z), n, _p(J32), _p(S), _p(W), _p(trace), max_trace, max_steps)
+                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int]
+    info = f(n, _p(U), n, _p(A), n, _p(Z), n, _p(J32), _p(S), _p(W), _p(trace), max_trace, max_steps, order)
     nsteps = info + 1 if info >= 0 else max_trace
-    return dict(YU=U, ZZ=Z, SY=S[:n].copy(), info=info, trace=trace[:min(nsteps, max_trace)])
+    return dict(YU=U, ZZ=Z, SY=S[:n].copy(), info=info, trace=trace[:min(nsteps, max_trace)], A=A)
 
 
 def pair_weights(A, J, fused=False):

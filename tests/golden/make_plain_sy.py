@@ -16,7 +16,7 @@ are made on the GPU and kept as vectors.
           build), also 52 000 steps: how far two valid builds of the reference land apart.
   c3      solve_c3_n2048_plain_sy.npy: N = 2048 complex input of the complex configuration
           (inputs.make("c3", seed=0): Re, Im ~ N(0, 1/2), J 1024/1024), unfused, run to the
-          reference's own termination (at most 64 N steps).
+          reference's own termination: 23 019 steps, about 9 minutes.
 
     python3 tests/golden/make_plain_sy.py c5 c5_fma c3 [--out DIR]   # on a machine with the GPU
 """

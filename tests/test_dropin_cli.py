@@ -37,3 +37,34 @@ def test_small_n_and_bad_ids():
 def test_missing_input_files(tmp_path):
     p = run(DJK, str(tmp_path / "none"), 8, 4)
     assert p.returncode != 0 and "OPEN_YJ_RO" in p.stderr + p.stdout
+
+
+def run_env(exe, env, *args):
+    e = dict(os.environ)
+    e.update(env)
+    return subprocess.run([exe] + [str(a) for a in args], capture_output=True, text=True, timeout=60, env=e)
+
+
+@pytest.mark.parametrize("env,msg", [
+    ({"JKG_CHECKPOINT": "0"}, "JKG_CHECKPOINT"), ({"JKG_CHECKPOINT": "x"}, "JKG_CHECKPOINT"),
+    ({"JKG_RESUME": "2"}, "JKG_RESUME"), ({"JKG_FRAMES": "0"}, "JKG_FRAMES"), ({"JKG_FRAMES": "4,-1"}, "JKG_FRAMES"),
+    ({"JKG_CHECKPOINT": "10", "JKG_SOLVER": "mixed"}, "plain solver only"),
+    ({"JKG_RESUME": "1", "JKG_SOLVER": "warm"}, "plain solver only")])
+def test_checkpoint_and_frame_switches(tmp_path, env, msg):
+    for exe in (DJK, ZJK):
+        p = run_env(exe, env, str(tmp_path / "none"), 8, 4)
+        assert p.returncode != 0 and msg in p.stderr + p.stdout, (env, p.stderr)
+
+
+def test_resume_needs_a_checkpoint(tmp_path):
+    import numpy as np
+    fn = tmp_path / "a"
+    np.zeros(64).tofile(str(fn) + ".CA")  # FN.CU missing
+    np.ones(8, dtype=np.int64).tofile(str(fn) + ".J")
+    p = run_env(DJK, {"JKG_RESUME": "1"}, str(fn), 8, 4)
+    assert p.returncode != 0 and "READ_CHECKPOINT" in p.stderr + p.stdout
+    for name in ("CU", "CZ"):
+        np.zeros(64).tofile(str(fn) + "." + name)
+    np.array([-1], dtype=np.int64).tofile(str(fn) + ".CS")  # a checkpoint being written
+    p = run_env(DJK, {"JKG_RESUME": "1"}, str(fn), 8, 4)
+    assert p.returncode != 0 and "READ_CHECKPOINT" in p.stderr + p.stdout

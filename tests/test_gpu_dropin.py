@@ -190,3 +190,69 @@ def test_dropin_rejects_a_bad_solver_name(tmp_path):
     assert p.returncode != 0
     assert "JKG_SOLVER" in p.stderr
     assert not os.path.exists(fn + ".YU")
+
+
+@pytest.mark.parametrize("case,k", [("c1_r64", 150), ("c3_z16", 60)])
+def test_dropin_checkpoint_and_resume(tmp_path, case, k):
+    """JKG_CHECKPOINT=K: the loop stops every K steps, writes FN.CA/CU/CZ/CS and goes on; the
+    outputs, stdout and CSV are the reference's run.  JKG_RESUME=1 from the last checkpoint
+    left behind: the same outputs, and the CSV from the checkpoint's step on."""
+    g = golden("solve_%s.npz" % case)
+    y, j = g["Y"], g["J"]
+    cplx = np.iscomplexobj(y)
+    n, steps = y.shape[0], int(g["steps"])
+    assert steps > k
+    fn = str(tmp_path / "a")
+    write_inputs(fn, y, j)
+    exe = ZJK if cplx else DJK
+    args = [n, int(g["n2"]), int(g["id_ncp"]), int(g["id_tru"])]
+    p = run(exe, fn, args, JKG_CHECKPOINT=str(k))
+    assert p.returncode == 0, p.stderr[-2000:]
+    out = p.stdout.rstrip("\n").split("\n")
+    assert re.fullmatch(r"Executed %11d steps with transformations in +\d+\.\d{6} s" % steps, out[1]), out[1]
+    rows, lines = csv_rows(p.stderr)
+    assert len(lines) == len(rows) + 1
+    check_trace(rows, g["trace"])
+    outs = read_outputs(fn, n, cplx)
+    for a, b in zip(outs, (g["YU"], g["ZZ"], g["SY"])):
+        assert np.ascontiguousarray(a).tobytes() == np.ascontiguousarray(b).tobytes()
+    # the last checkpoint: after the last whole part of K steps
+    s0 = int(np.fromfile(fn + ".CS", dtype=np.int64)[0])
+    assert s0 == (steps // k) * k
+    dt = np.complex128 if cplx else np.float64
+    for name in ("CA", "CU", "CZ"):
+        assert np.fromfile(fn + "." + name, dtype=dt).size == n * n
+    for name in ("YU", "ZZ", "SY"):
+        os.remove(fn + "." + name)
+    os.remove(fn + ".Y")  # the checkpoint replaces it
+    p = run(exe, fn, args, JKG_RESUME="1")
+    assert p.returncode == 0, p.stderr[-2000:]
+    out = p.stdout.rstrip("\n").split("\n")
+    assert out[0] == census(g["PN"], int(g["TT"]))
+    assert re.fullmatch(r"Executed %11d steps with transformations in +\d+\.\d{6} s" % steps, out[1]), out[1]
+    rows, _ = csv_rows(p.stderr)
+    check_trace(rows, g["trace"][s0:])
+    outs = read_outputs(fn, n, cplx)
+    for a, b in zip(outs, (g["YU"], g["ZZ"], g["SY"])):
+        assert np.ascontiguousarray(a).tobytes() == np.ascontiguousarray(b).tobytes()
+
+
+def test_dropin_frames(tmp_path):
+    """JKG_FRAMES=E,S: FN.FRM holds the S x S pictures of |A| at the steps 0 mod E, as the
+    library records them (jkg_set_frames), also when the loop runs in checkpointed parts."""
+    from jkogb_amd import jkg
+    g = golden("solve_c1_r64.npz")
+    y, j = g["Y"], g["J"]
+    n, steps = y.shape[0], int(g["steps"])
+    with jkg.Context(0) as ctx:
+        ctx.set_frames(every=5, size=16, max_frames=4096)
+        ctx.dsolve(y, j, int(g["n2"]))
+        ref = ctx.frames()
+    assert ref.shape == (steps // 5 + 1, 16, 16)
+    for env in ({"JKG_FRAMES": "5,16"}, {"JKG_FRAMES": "5,16,64", "JKG_CHECKPOINT": "77"}):
+        fn = str(tmp_path / "f")
+        write_inputs(fn, y, j)
+        p = run(DJK, fn, [n, int(g["n2"])], **env)
+        assert p.returncode == 0, p.stderr[-2000:]
+        frm = np.fromfile(fn + ".FRM", dtype=np.uint8)
+        assert frm.size == ref.size and np.array_equal(frm.reshape(ref.shape), ref), env

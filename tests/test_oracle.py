@@ -262,3 +262,87 @@ def test_loop_max_steps_prefix_matches_full_run(orc):
     assert part["info"] == 10
     for f in ["oldlen", "newlen", "trig", "hyp", "tw"]:
         assert np.array_equal(part["trace"][f][:10], full["trace"][f][:10])
+
+
+# ---------------------------------------------------------------- cyclic sweeps (dcyc_loop)
+def reference_sweep_pairs(n, order):
+    """The pair order of the reference's pointwise sweeps, enumerated as its loops do
+    (src/old/jk1.f90:105-116 row-cyclic, :125-136 column-cyclic), 1-based."""
+    if order == 2:
+        return [(i, j) for i in range(1, n) for j in range(i + 1, n + 1)]
+    return [(i, j) for j in range(2, n + 1) for i in range(1, j)]
+
+
+@pytest.mark.parametrize("order", [2, 3])
+def test_cyclic_pointwise_visits_the_reference_order(orc, order):
+    """Step s of the pointwise sweeps transforms pair s mod NN of the reference's order:
+    the loop stopped after s + 1 steps has that pair's off-diagonal entries annihilated
+    (DSTEP_TRANSF zeroes them) -- for steps across the first sweep and into the second."""
+    n = 9
+    rng = np.random.default_rng(3)
+    y = rng.standard_normal((n, n))
+    j = np.ones(n, dtype=np.int64)
+    pairs = reference_sweep_pairs(n, order)
+    nn = len(pairs)
+    for s in list(range(0, nn, 5)) + [nn - 1, nn, nn + 7]:
+        r = orc.solve_cyclic(y, j, max_steps=s + 1, order=order)
+        p, q = pairs[s % nn]
+        a = r["A"]
+        assert a[p - 1, q - 1] == 0.0 and a[q - 1, p - 1] == 0.0, (s, p, q)
+        # a pair the step did not touch keeps a non-zero off-diagonal entry somewhere
+        assert np.count_nonzero(a) > 0
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+@pytest.mark.parametrize("n,seed", [(24, 1), (37, 2)])
+def test_cyclic_svd_matches_lapack(orc, order, n, seed):
+    """J = +1: every pair is trigonometric and the HSVD is the SVD; the singular values of
+    every cyclic order agree with LAPACK's (numpy.linalg.svd) to 8 N eps, relatively, and
+    the factors are orthogonal with Y = YU diag(SY) ZZ to the same order."""
+    rng = np.random.default_rng(seed)
+    y = rng.standard_normal((n, n))
+    j = np.ones(n, dtype=np.int64)
+    r = orc.solve_cyclic(y, j, order=order)
+    assert r["info"] > 0
+    sv = np.linalg.svd(y, compute_uv=False)
+    got = np.sort(np.abs(r["SY"]))[::-1]
+    assert np.max(np.abs(got - sv) / sv) <= 8 * n * EPS
+    yu, zz = r["YU"], r["ZZ"]
+    assert np.linalg.norm(yu.T @ yu - np.eye(n), 2) <= 8 * n * EPS
+    assert np.linalg.norm(y - (yu * r["SY"]) @ zz, 2) <= 8 * n * EPS * sv[0]
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_cyclic_hyperbolic_matches_eig(orc, order):
+    """Mixed signs: sigma^2 = |eig(Y J Y^T)| per sign class, within eig's own error
+    N eps (sigma + ||Y||^2 / sigma), and the J-orthogonality of ZZ.  The input is
+    diagonally dominant: a fixed cyclic order has no guard against the growth of the
+    hyperbolic transforms, and on a general input it can diverge (the dynamic
+    ordering's weights are that guard)."""
+    n = 30
+    rng = np.random.default_rng(11)
+    y = np.diag(rng.uniform(2.0, 4.0, n)) + 0.2 * rng.standard_normal((n, n))
+    j = np.where(rng.random(n) < 0.5, 1, -1).astype(np.int64)
+    r = orc.solve_cyclic(y, j, order=order)
+    assert r["info"] > 0
+    ev = np.linalg.eigvalsh(y @ np.diag(j.astype(float)) @ y.T)
+    y2 = np.linalg.norm(y, 2) ** 2
+    for cls in (1, -1):
+        a = np.sort(np.abs(r["SY"][j == cls]))
+        e = np.sort(np.sqrt(np.abs(ev[ev > 0] if cls > 0 else ev[ev < 0])))
+        assert a.size == e.size
+        assert np.max(np.abs(a - e) / (n * EPS * (e + y2 / e))) <= 1.0
+    jd = np.diag(j.astype(float))
+    zz = r["ZZ"]
+    assert np.linalg.norm(zz @ jd @ zz.T - jd, 2) <= 64 * n * EPS * np.linalg.norm(zz, 2) ** 2
+
+
+def test_cyclic_orders_differ(orc):
+    """The three orders are three trajectories: the same values, other step counts or bits."""
+    n = 20
+    y = np.random.default_rng(4).standard_normal((n, n))
+    j = np.ones(n, dtype=np.int64)
+    rs = [orc.solve_cyclic(y, j, order=o) for o in (1, 2, 3)]
+    assert rs[1]["info"] != rs[2]["info"] or not np.array_equal(bits(rs[1]["YU"]), bits(rs[2]["YU"]))
+    for r in rs:
+        assert np.allclose(np.sort(np.abs(r["SY"])), np.sort(np.abs(rs[0]["SY"])), rtol=1e-12)
