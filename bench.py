@@ -325,23 +325,6 @@ def measured_traffic():
         return None
 
 
-def traffic_workload(y, j, n2):
-    """What the rocprofv3 FETCH_SIZE / WRITE_SIZE runs measure: a plain device copy of 512 MiB
-    (the counters' scale: bytes moved are known) and two stage-profiled steps of each solver."""
-    import torch
-    from jkogb_amd import jkg
-    src = torch.empty(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
-    src.normal_()
-    dst = torch.empty_like(src)
-    for _ in range(3):
-        dst.copy_(src)
-    torch.cuda.synchronize()
-    with jkg.Context(0) as ctx:
-        ctx.dprofile_mixed(y, j, n2, steps=2)
-        ctx.dprofile(y, j, n2, steps=2)
-    print("traffic workload done", file=sys.stderr)
-
-
 def stage_profile(prof):
     ps = max(prof["steps"], 1)
     return {s: v / ps for s, v in prof["stage_ms"].items()}, ps
@@ -407,9 +390,6 @@ def main():
     ap.add_argument("--complex-full", action="store_true", help="also complete complex N=2048 solves")
     ap.add_argument("--batched", action="store_true", help="also the batched configuration (4096 instances)")
     ap.add_argument("--batch-count", type=int, default=4096)
-    ap.add_argument("--traffic-workload", action="store_true",
-                    help="only the workload of the HBM-traffic counter runs: a 512 MiB device copy (three times, "
-                         "the counters' calibration) and two profiled steps of each solver at N=4096")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: the step runs on one GPU (its sharding over several GPUs is a design, not built); "
@@ -421,9 +401,6 @@ def main():
     n = args.n
     y, j = inputs.make("c5", n=n, seed=0)
     n2 = n // 2
-    if args.traffic_workload:
-        traffic_workload(y, j, n2)
-        return
     ctx = jkg.Context(0)
     t0 = time.time()
     log = lambda msg: print("bench: %s (%.0f s)" % (msg, time.time() - t0), file=sys.stderr, flush=True)
