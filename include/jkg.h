@@ -23,7 +23,7 @@ extern "C" {
 #define JKG_OK 0
 #define JKG_EHIP (-1000)     /* a HIP runtime call failed (see jkg_last_hip_error) */
 #define JKG_ENOMEM (-1001)   /* device allocation failed */
-#define JKG_ENOTSUP (-1002)  /* option not implemented on the GPU (e.g. ID_NCP=3) */
+#define JKG_ENOTSUP (-1002)  /* an option or size this build does not support (e.g. N beyond the limits below) */
 #define JKG_EJ (-1003)       /* a J entry outside {-1,+1} (the reference: R%TRU INFO=-2) */
 #define JKG_ENOCONV (-1004)  /* the step loop ran past its guard of 1000 N + 1000 steps (1000 NN + 1000 for
                                 the one-pair cyclic sweeps) */
