@@ -155,11 +155,14 @@ static int ensure_buffers(jkg_ctx *c, int n, int cplx, int trace_cap) {
   ACHK(dalloc(c, &d.plus, n));
   ACHK(dalloc(c, &d.minus, n));
   ACHK(dalloc(c, &d.sort.k0, nn));
-  ACHK(dalloc(c, &d.sort.k1, nn));
   ACHK(dalloc(c, &d.sort.v0, nn + LIST_PAD));
-  ACHK(dalloc(c, &d.sort.v1, nn + LIST_PAD));
+  ACHK(dalloc(c, &d.sort.k1, 2 * nn));
+  ACHK(dalloc(c, &d.sort.k2, 2 * nn));
+  ACHK(dalloc(c, &d.sort.v1, nn));
+  ACHK(dalloc(c, &d.sort.v2, nn));
+  ACHK(dalloc(c, &d.sort.d1, sort_digit_bytes((int)nn)));
+  ACHK(dalloc(c, &d.sort.d2, sort_digit_bytes((int)nn)));
   ACHK(dalloc(c, (char **)&d.sort.scratch, sort_scratch_bytes((int)nn)));
-  d.sort_skip = sort_skip_flags(d.sort, (int)nn);
   for (int k = 0; k < 2; ++k) {
     ACHK(dalloc(c, &d.lists.val[k], nn + LIST_PAD));
     ACHK(dalloc(c, &d.lists.pos[k], nn + LIST_PAD));
@@ -1000,11 +1003,14 @@ static size_t noalloc_layout(DeviceStep &d, double **sigma, char *base, int n, i
   take(&d.plus, n);
   take(&d.minus, n);
   take(&d.sort.k0, nn);
-  take(&d.sort.k1, nn);
   take(&d.sort.v0, nn + LIST_PAD);
-  take(&d.sort.v1, nn + LIST_PAD);
+  take(&d.sort.k1, 2 * nn);
+  take(&d.sort.k2, 2 * nn);
+  take(&d.sort.v1, nn);
+  take(&d.sort.v2, nn);
+  take(&d.sort.d1, sort_digit_bytes((int)nn));
+  take(&d.sort.d2, sort_digit_bytes((int)nn));
   take((char **)&d.sort.scratch, sort_scratch_bytes((int)nn));
-  if (base) d.sort_skip = sort_skip_flags(d.sort, (int)nn);
   for (int k = 0; k < 2; ++k) {
     take(&d.lists.val[k], nn + LIST_PAD);
     take(&d.lists.pos[k], nn + LIST_PAD);
@@ -1308,27 +1314,20 @@ extern "C" int jkg_dbuild(jkg_ctx *c, int n, int n2, const double *a, const int6
   DeviceStep &d = c->d;
   step_begin_launch(d, c->stream);
   weights_launch(d, c->stream);
-  sort_pairs_launch(d.sort, &d.st->nn_sort, d.nn, c->stream, 1);
+  sort_pairs_launch(d.sort, &d.st->nn_sort, d.nn, c->stream, d.n);
   select_launch(d, c->stream);
   HIPCHK(hipGetLastError());
   StepState s;
-  int skip[9];
   HIPCHK(hipMemcpyAsync(&s, d.st, sizeof(s), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipMemcpyAsync(skip, d.sort_skip, sizeof(skip), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  int moved = 0;
-  for (int i = 0; i < 8; ++i) moved += skip[i] ? 0 : 1;
-  const uint64_t *kb = (moved & 1) ? d.sort.k1 : d.sort.k0;
-  const uint32_t *vb = (moved & 1) ? d.sort.v1 : d.sort.v0;
-  std::vector<uint64_t> keys(s.it);
+  // the sorted packed pairs; the keys stay in pair-index order
+  std::vector<uint64_t> keys(d.nn);
   std::vector<uint32_t> vals(s.it);
-  if (s.it > 0) {
-    HIPCHK(hipMemcpy(keys.data(), kb, sizeof(uint64_t) * s.it, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(vals.data(), vb, sizeof(uint32_t) * s.it, hipMemcpyDeviceToHost));
-  }
+  HIPCHK(hipMemcpy(keys.data(), d.sort.k0, sizeof(uint64_t) * d.nn, hipMemcpyDeviceToHost));
+  if (s.it > 0) HIPCHK(hipMemcpy(vals.data(), d.sort.v0, sizeof(uint32_t) * s.it, hipMemcpyDeviceToHost));
   for (int i = 0; i < s.it; ++i) {
     const int p = pq_p(vals[i]) + 1, q = pq_q(vals[i]) + 1;
-    sorted[i].w = jkg_key_weight(keys[i]);
+    sorted[i].w = jkg_key_weight(keys[pair_index(n, p, q)]);
     sorted[i].p = p;
     sorted[i].q = q;
     sorted[i].b = q - p;
@@ -1363,13 +1362,9 @@ __global__ void k_plugin_pairkeys(int n, int nn, const int *pq, uint64_t *k0) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < nn) k0[i] = pair_index(n, pq[2 * i], pq[2 * i + 1]);
 }
-__global__ void k_plugin_wkeys(int nn, const double *w, const uint64_t *kin, const uint32_t *vin, int iota,
-                               uint64_t *k0, uint32_t *v0) {
+__global__ void k_plugin_wkeys(int nn, const double *w, const uint32_t *perm, uint64_t *k0) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nn) return;
-  uint32_t src = iota ? (uint32_t)i : vin[i];
-  k0[i] = weight_key(w[src]);
-  v0[i] = src;
+  if (i < nn) k0[i] = weight_key(w[perm[i]]);
 }
 
 // ASRT: sort by (pair index) then stably by weight key == AW_CMP order.
@@ -1390,10 +1385,12 @@ extern "C" int jkg_srt(jkg_ctx *c, int nn, int nm, jkg_aw *dz, int64_t *info) {
     w[i] = dz[i].w;
     if (dz[i].p < 1 || dz[i].q <= dz[i].p) return -4;
   }
-  // buffers: keys/vals x2, scratch, pq, w, n
+  // buffers: pair keys, weight keys, the first and the final permutation, the sort's records and
+  // scratch, pq, w, n
   const size_t bk = sizeof(uint64_t) * nn, bv = sizeof(uint32_t) * nn;
   const size_t bs = sort_scratch_bytes(nn), bpq = sizeof(int) * 2 * nn, bw = sizeof(double) * nn;
-  int rc = tmp_bytes(c, 2 * bk + 2 * bv + bs + bpq + bw + 64 * 8);
+  const size_t bdg = sort_digit_bytes(nn);
+  int rc = tmp_bytes(c, 6 * bk + 4 * bv + 2 * bdg + bs + bpq + bw + 256 * 15);  // 14 pieces, each 256-byte aligned
   if (rc) return rc;
   char *p = (char *)c->tmp;
   auto take = [&](size_t b) {
@@ -1403,43 +1400,37 @@ extern "C" int jkg_srt(jkg_ctx *c, int nn, int nm, jkg_aw *dz, int64_t *info) {
   };
   SortBuffers sb;
   sb.k0 = (uint64_t *)take(bk);
-  sb.k1 = (uint64_t *)take(bk);
+  uint64_t *wk = (uint64_t *)take(bk);
   sb.v0 = (uint32_t *)take(bv);
+  uint32_t *perm2 = (uint32_t *)take(bv);
+  sb.k1 = (uint64_t *)take(2 * bk);
+  sb.k2 = (uint64_t *)take(2 * bk);
   sb.v1 = (uint32_t *)take(bv);
+  sb.v2 = (uint32_t *)take(bv);
+  sb.d1 = (uint8_t *)take(bdg);
+  sb.d2 = (uint8_t *)take(bdg);
+  sb.vin = nullptr;
   sb.scratch = (int *)take(bs);
   int *dpq = (int *)take(bpq);
   double *dw = (double *)take(bw);
   int *dn = (int *)take(sizeof(int));
-  int *skip = sort_skip_flags(sb, nn);
   HIPCHK(hipMemcpyAsync(dpq, pq.data(), bpq, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(dw, w.data(), bw, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(dn, &nn, sizeof(int), hipMemcpyHostToDevice, c->stream));
   const int g = (nn + 255) / 256;
+  // first by pair index, then stably by weight key over that permutation
   hipLaunchKernelGGL(k_plugin_pairkeys, dim3(g), dim3(256), 0, c->stream, n, nn, dpq, sb.k0);
   sort_pairs_launch(sb, dn, nn, c->stream, 0);
-  int sk[9];
-  HIPCHK(hipMemcpyAsync(sk, skip, sizeof(sk), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  int moved = 0;
-  for (int i = 0; i < 8; ++i) moved += sk[i] ? 0 : 1;
-  // second sort: weight keys over the first permutation, values given
-  const uint64_t *kin = (moved & 1) ? sb.k1 : sb.k0;
-  const uint32_t *vin = (moved & 1) ? sb.v1 : sb.v0;
-  uint64_t *kt = (moved & 1) ? sb.k0 : sb.k1;  // free buffer for the new keys
-  uint32_t *vt = (moved & 1) ? sb.v0 : sb.v1;
-  hipLaunchKernelGGL(k_plugin_wkeys, dim3(g), dim3(256), 0, c->stream, nn, dw, kin, vin, moved == 0 ? 1 : 0, kt, vt);
+  hipLaunchKernelGGL(k_plugin_wkeys, dim3(g), dim3(256), 0, c->stream, nn, dw, sb.v0, wk);
   SortBuffers s2 = sb;
-  s2.k0 = kt;
-  s2.v0 = vt;
-  s2.k1 = (uint64_t *)kin;
-  s2.v1 = (uint32_t *)vin;
-  sort_pairs_launch(s2, dn, nn, c->stream, 1);
-  HIPCHK(hipMemcpyAsync(sk, skip, sizeof(sk), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  moved = 0;
-  for (int i = 0; i < 8; ++i) moved += sk[i] ? 0 : 1;
+  s2.k0 = wk;
+  s2.vin = sb.v0;
+  s2.v0 = perm2;
+  sort_pairs_launch(s2, dn, nn, c->stream, 0);
+  HIPCHK(hipGetLastError());
   std::vector<uint32_t> perm(nn);
-  HIPCHK(hipMemcpy(perm.data(), (moved & 1) ? s2.v1 : s2.v0, bv, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpyAsync(perm.data(), perm2, bv, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   std::vector<jkg_aw> out(nn);
   for (int i = 0; i < nn; ++i) out[i] = dz[perm[i]];
   memcpy(dz, out.data(), sizeof(jkg_aw) * nn);
@@ -1573,24 +1564,29 @@ extern "C" int jkg_ncp3(jkg_ctx *c, int n, int nn, int nm, jkg_aw *dz, int n2, i
   rc = ensure_ncp3(c);
   if (rc) return rc;
   DeviceStep &d = c->d;
-  std::vector<uint64_t> key(nn);
+  // the list as a finished sort: the packed pairs in list order, each key at its pair index (of
+  // the buffers' order nb), and the sort's figures (no NaN weights: they were refused above)
+  std::vector<uint64_t> key(d.nn, ~0ull);
   std::vector<uint32_t> val(nn);
   for (int i = 0; i < nn; ++i) {
-    key[i] = host_weight_key(dz[i].w);
+    uint64_t &k = key[pair_index(nb, dz[i].p, dz[i].q)];
+    if (k != ~0ull) return -5;  // a pair listed twice
+    k = host_weight_key(dz[i].w);
     val[i] = pack_pq(dz[i].p - 1, dz[i].q - 1);
   }
-  // the list as a finished sort: every pass skipped, values given
-  const int skip[9] = {1, 1, 1, 1, 1, 1, 1, 1, 1};
-  HIPCHK(hipMemcpyAsync(d.sort.k0, key.data(), 8 * (size_t)nn, hipMemcpyHostToDevice, c->stream));
+  SortStats ss{};
+  ss.n = nn;
+  ss.nan_count = 0;
+  ss.minkey = ~0ull;
+  for (int i = 0; i < nn; ++i) ss.minkey = std::min<unsigned long long>(ss.minkey, host_weight_key(dz[i].w));
+  HIPCHK(hipMemcpyAsync(d.sort.k0, key.data(), 8 * (size_t)d.nn, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(d.sort.v0, val.data(), 4 * (size_t)nn, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(d.sort_skip, skip, sizeof(skip), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(sort_stats(d.sort), &ss, sizeof(ss), hipMemcpyHostToDevice, c->stream));
   StepState s{};
   s.n2 = n2;
   s.active = 1;
   HIPCHK(hipMemcpyAsync(d.st, &s, sizeof(s), hipMemcpyHostToDevice, c->stream));
-  DeviceStep dd = d;
-  dd.nn = nn;  // the binary search for IT runs over the given list
-  ncp3_launch(dd, c->stream);
+  ncp3_launch(d, c->stream);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(&s, d.st, sizeof(s), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -1721,7 +1717,7 @@ static int profile_common(jkg_ctx *c, int n, int n2, const double *y, const int6
     HIPCHK(hipEventRecord(ev[1], c->stream));
     weights_launch(d, c->stream);
     HIPCHK(hipEventRecord(ev[2], c->stream));
-    sort_pairs_launch(d.sort, &d.st->nn_sort, d.nn, c->stream, 1);
+    sort_pairs_launch(d.sort, &d.st->nn_sort, d.nn, c->stream, d.n, (d.prec32 && d.cut32) ? 32 : 0);
     HIPCHK(hipEventRecord(ev[3], c->stream));
     select_launch(d, c->stream);
     HIPCHK(hipEventRecord(ev[4], c->stream));

@@ -131,6 +131,10 @@ __device__ __forceinline__ void pair_of(int n, uint32_t k, int &p, int &q) {
 __host__ __device__ __forceinline__ uint32_t pack_pq(int p0, int q0) { return (uint32_t)p0 | ((uint32_t)q0 << 16); }
 __host__ __device__ __forceinline__ int pq_p(uint32_t v) { return (int)(v & 0xFFFFu); }
 __host__ __device__ __forceinline__ int pq_q(uint32_t v) { return (int)(v >> 16); }
+// Key of a packed pair: the sort leaves the keys in pair-index order and sorts the pairs.
+__device__ __forceinline__ uint64_t pair_key(const uint64_t *keys, int n, uint32_t v) {
+  return keys[pair_index(n, pq_p(v) + 1, pq_q(v) + 1)];
+}
 
 // The mixed solver's convergence test for one pair: its off-diagonal entries
 // are negligible against the diagonal,

@@ -360,7 +360,7 @@ __global__ void __launch_bounds__(256) k_hyp_weights_gram(int n, const E *__rest
                                                           const double *__restrict__ G, int ldg, int gks,
                                                           const double *__restrict__ coln,
                                                           const double *__restrict__ dg,
-                                                          uint64_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                                                          uint64_t *__restrict__ keys,
                                                           StepState *st, double tol, int cut) {
   if (!st->active) return;
   __shared__ double E1[WT][WT + 1];  // E1[a][b] = A(minus b, plus a)
@@ -414,7 +414,6 @@ __global__ void __launch_bounds__(256) k_hyp_weights_gram(int n, const E *__rest
     }
     const uint32_t k = pair_index(n, p0 + 1, q0 + 1);
     keys[k] = key_cut(key, cut);
-    vals[k] = pack_pq(p0, q0);
   }
   // work counter: pairs whose weight needed the column sums
   for (int o = 32; o > 0; o >>= 1) nfull += __shfl_xor(nfull, o);
@@ -432,7 +431,7 @@ void gram_hyp_weights_launch(DeviceStep &d, hipStream_t s) {
   dim3 g((d.nmi + WT - 1) / WT, (d.npl + WT - 1) / WT);
   auto weights = [&](auto kern, const auto *A, int cut) {
     hipLaunchKernelGGL(kern, g, dim3(256), 0, s, d.n, A, d.lda, d.J, d.plus, d.npl, d.minus, d.nmi, d.gram, d.nmi,
-                       ks, d.coln, d.coln + d.n, d.sort.k0, d.sort.v0, d.st, d.neg_tol, cut);
+                       ks, d.coln, d.coln + d.n, d.sort.k0, d.st, d.neg_tol, cut);
   };
   if (d.prec32) {  // the FP32 warm start: column norms, its end test, the FP32 Gram product
     const float *A = (const float *)d.A32;
@@ -545,7 +544,7 @@ __global__ void __launch_bounds__(256) k_zhyp_weights_gram(int n, const E *__res
                                                            const double *__restrict__ Gim, int ldg, int gks,
                                                            const double *__restrict__ coln,
                                                            const dz *__restrict__ dg, uint64_t *__restrict__ keys,
-                                                           uint32_t *__restrict__ vals, StepState *st, double tol,
+                                                           StepState *st, double tol,
                                                            int cut) {
   if (!st->active) return;
   __shared__ dz E1[ZWT][ZWT + 1];  // E1[a][b] = A(minus b, plus a)
@@ -610,7 +609,6 @@ __global__ void __launch_bounds__(256) k_zhyp_weights_gram(int n, const E *__res
     }
     const uint32_t k = pair_index(n, p0 + 1, q0 + 1);
     keys[k] = key_cut(key, cut);
-    vals[k] = pack_pq(p0, q0);
   }
   for (int o = 32; o > 0; o >>= 1) nfull += __shfl_xor(nfull, o);
   if ((tid & 63) == 0 && nfull) atomicAdd((unsigned long long *)&st->acc_abod, (unsigned long long)nfull);
@@ -624,7 +622,7 @@ void zgram_hyp_weights_launch(DeviceStep &d, hipStream_t s) {
   dim3 g((d.nmi + ZWT - 1) / ZWT, (d.npl + ZWT - 1) / ZWT);
   auto weights = [&](auto kern, const auto *A) {
     hipLaunchKernelGGL(kern, g, dim3(256), 0, s, d.n, A, d.lda, d.J, d.plus, d.npl, d.minus, d.nmi, Gre, Gim, d.nmi,
-                       ks, d.coln, dg, d.sort.k0, d.sort.v0, d.st, d.neg_tol, cut);
+                       ks, d.coln, dg, d.sort.k0, d.st, d.neg_tol, cut);
   };
   if (d.prec32) {
     const fz *A = (const fz *)d.A32;

@@ -66,47 +66,40 @@ __device__ int ncp3_walk(int n, const uint32_t *__restrict__ vals, int it, int s
 
 // The reference's total of a selection: positive weights from the last positive pick
 // back to the first, then the remaining picks in order (weights descend along the list).
-__device__ double ncp3_total(const uint64_t *__restrict__ keys, const int *picks, int np) {
+// (keys in pair-index order, vals the sorted packed pairs, picks positions in vals)
+__device__ double ncp3_total(int n, const uint64_t *__restrict__ keys, const uint32_t *__restrict__ vals,
+                             const int *picks, int np) {
+  auto w = [&](int t) { return key_weight(pair_key(keys, n, vals[picks[t - 1]])); };
   int l = 0;
   for (int t = np; t >= 1; --t)
-    if (key_weight(keys[picks[t - 1]]) > 0.0) {
+    if (w(t) > 0.0) {
       l = t;
       break;
     }
   double wp = 0.0, wn = 0.0;
-  for (int t = l; t >= 1; --t) wp = wp + key_weight(keys[picks[t - 1]]);
-  for (int t = l + 1; t <= np; ++t) wn = wn + key_weight(keys[picks[t - 1]]);
+  for (int t = l; t >= 1; --t) wp = wp + w(t);
+  for (int t = l + 1; t <= np; ++t) wn = wn + w(t);
   return wp + wn;
 }
 
 // IT (the finite keys: position of the first NaN key) and the list length of this step.
-__global__ void k_ncp3_prep(const uint64_t *k0, const uint64_t *k1, const uint32_t *v0, const uint32_t *v1,
-                            const int *sort_skip, int nn, StepState *st) {
+__global__ void k_ncp3_prep(const SortStats *sst, StepState *st) {
   if (!st->active || threadIdx.x != 0) return;
-  SortResult sr = sort_result_dev(k0, k1, v0, v1, sort_skip);
-  int lo = 0, hi = nn;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (key_nan(sr.keys[mid])) hi = mid;
-    else lo = mid + 1;
-  }
-  st->it = lo;
-  st->list_len = lo;
-  st->wmax = (lo > 0) ? key_weight(sr.keys[0]) : 0.0;
+  const int it = sst->n - sst->nan_count;  // the NaN keys sort last
+  st->it = it;
+  st->list_len = it;
+  st->wmax = (it > 0) ? key_weight(sst->minkey) : 0.0;  // the smallest key is the largest weight
   st->sl = 0;
 }
 
 // (the step's candidate values are always materialised: the weight kernels write them)
-__global__ void __launch_bounds__(64 * NCP3_WAVES) k_ncp3_walks(int n, const uint64_t *k0, const uint64_t *k1,
-                                                                const uint32_t *v0, const uint32_t *v1,
-                                                                const int *sort_skip, double *__restrict__ wsum,
+__global__ void __launch_bounds__(64 * NCP3_WAVES) k_ncp3_walks(int n, const uint64_t *keys, const uint32_t *vals,
+                                                                double *__restrict__ wsum,
                                                                 const StepState *st, int n2max) {
   extern __shared__ __attribute__((aligned(16))) unsigned char ncp3_lds[];
   if (!st->active) return;
   const int it = st->it, n2 = (st->n2 < st->it) ? st->n2 : st->it;  // NCP gets MIN(IT, N_2)
   if (n2 <= 0) return;
-  SortResult sr = sort_result_dev(k0, k1, v0, v1, sort_skip);
-  const uint32_t *vals = sr.vals;
   const int wave = threadIdx.x >> 6;
   unsigned char *slice = ncp3_lds + (size_t)wave * ncp3_slice(n, n2max);
   uint8_t *used = slice;
@@ -114,14 +107,13 @@ __global__ void __launch_bounds__(64 * NCP3_WAVES) k_ncp3_walks(int n, const uin
   const int nwaves = gridDim.x * NCP3_WAVES;
   for (int o = blockIdx.x * NCP3_WAVES + wave; o < it; o += nwaves) {
     const int np = ncp3_walk(n, vals, it, o, n2, used, picks);
-    if ((threadIdx.x & 63) == 0) wsum[o] = ncp3_total(sr.keys, picks, np);
+    if ((threadIdx.x & 63) == 0) wsum[o] = ncp3_total(n, keys, vals, picks, np);
     __builtin_amdgcn_wave_barrier();
   }
 }
 
 // One wave: the reference's sequential choice over the offsets, then the winner's walk.
-__global__ void __launch_bounds__(64) k_ncp3_pick(int n, const uint64_t *k0, const uint64_t *k1, const uint32_t *v0,
-                                                  const uint32_t *v1, const int *sort_skip,
+__global__ void __launch_bounds__(64) k_ncp3_pick(int n, const uint32_t *vals,
                                                   const double *__restrict__ wsum, int *piv_p, int *piv_q,
                                                   int *piv_pos, StepState *st) {
   extern __shared__ __attribute__((aligned(16))) unsigned char ncp3_lds[];
@@ -170,8 +162,6 @@ __global__ void __launch_bounds__(64) k_ncp3_pick(int n, const uint64_t *k0, con
     }
     k = __shfl(k, 0);
   }
-  SortResult sr = sort_result_dev(k0, k1, v0, v1, sort_skip);
-  const uint32_t *vals = sr.vals;
   uint8_t *used = ncp3_lds;
   int *picks = (int *)(ncp3_lds + ((n + 1 + 15) / 16) * 16);
   const int np = ncp3_walk(n, vals, it, k, n2, used, picks);
@@ -202,12 +192,10 @@ int ncp3_prepare(int n, int n2max) {
 // Selection of a step under ID_NCP=3 (in place of select_launch); wsum: NN doubles.
 void ncp3_launch(DeviceStep &d, hipStream_t s) {
   const int n2max = d.n2max;
-  hipLaunchKernelGGL(k_ncp3_prep, dim3(1), dim3(64), 0, s, d.sort.k0, d.sort.k1, d.sort.v0, d.sort.v1, d.sort_skip,
-                     d.nn, d.st);
+  hipLaunchKernelGGL(k_ncp3_prep, dim3(1), dim3(64), 0, s, sort_stats(d.sort), d.st);
   hipLaunchKernelGGL(k_ncp3_walks, dim3(1024), dim3(64 * NCP3_WAVES), NCP3_WAVES * ncp3_slice(d.n, n2max), s, d.n,
-                     d.sort.k0, d.sort.k1, d.sort.v0, d.sort.v1, d.sort_skip, d.ncp3_wsum, d.st, n2max);
-  hipLaunchKernelGGL(k_ncp3_pick, dim3(1), dim3(64), ncp3_slice(d.n, n2max), s, d.n, d.sort.k0, d.sort.k1,
-                     d.sort.v0, d.sort.v1, d.sort_skip, d.ncp3_wsum, d.piv_p, d.piv_q, d.piv_pos, d.st);
+                     d.sort.k0, d.sort.v0, d.ncp3_wsum, d.st, n2max);
+  hipLaunchKernelGGL(k_ncp3_pick, dim3(1), dim3(64), ncp3_slice(d.n, n2max), s, d.n, d.sort.v0, d.ncp3_wsum, d.piv_p, d.piv_q, d.piv_pos, d.st);
 }
 
 }  // namespace jkg

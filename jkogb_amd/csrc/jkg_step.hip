@@ -74,7 +74,7 @@ constexpr int TWT = 32;
 template <bool F, class E>
 __global__ void __launch_bounds__(256) k_trig_weights(int n, const E *__restrict__ A, int lda,
                                                        const int *__restrict__ J, uint64_t *__restrict__ keys,
-                                                       uint32_t *__restrict__ vals, const StepState *st,
+                                                       const StepState *st,
                                                        double tol, int cut) {
   const int bp = blockIdx.y, bq = blockIdx.x;
   if (bp > bq || !st->active) return;
@@ -111,7 +111,6 @@ __global__ void __launch_bounds__(256) k_trig_weights(int n, const E *__restrict
     if (tol > 0.0 && go && !order && negligible(aqp, apq, app, aqq, tol)) go = false;
     const uint32_t k = pair_index(n, P + 1, Q + 1);
     keys[k] = key_cut(go ? weight_key(dasum2<F>(aqp, apq)) : KEY_NAN, cut);
-    vals[k] = pack_pq(P, Q);
   }
 }
 
@@ -124,7 +123,7 @@ template <bool F>
 __global__ void __launch_bounds__(256) k_hyp_weights(int n, const double *__restrict__ A, int lda,
                                                       const int *__restrict__ J, const int *__restrict__ plus,
                                                       int npl, const int *__restrict__ minus, int nmi,
-                                                      uint64_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                                                      uint64_t *__restrict__ keys,
                                                       StepState *st) {
   if (!st->active) return;
   __shared__ double L[HR][2 * HT + 1];
@@ -203,7 +202,6 @@ __global__ void __launch_bounds__(256) k_hyp_weights(int n, const double *__rest
   if (valid) {
     const uint32_t k = pair_index(n, p0 + 1, q0 + 1);
     keys[k] = key;
-    vals[k] = pack_pq(p0, q0);
   }
 }
 
@@ -287,9 +285,9 @@ __device__ __forceinline__ int wave_prefix_small(int cnt, int &total) {
 // positions in lp.*[(r - 1) & 1], st->next_len entries).  A round walks at
 // most `limit` entries (0: all) from the start of its list.
 template <bool HAS_POS>
-__global__ void __launch_bounds__(SW_WAVES * 64) k_sel_walk(int n, const uint64_t *k0, const uint64_t *k1,
-                                                            const uint32_t *v0, const uint32_t *v1,
-                                                            const int *sort_skip, ListPtrs lp, uint8_t *used_g,
+__global__ void __launch_bounds__(SW_WAVES * 64) k_sel_walk(int n, const uint64_t *keys,
+                                                            const uint32_t *svals, const SortStats *sst,
+                                                            ListPtrs lp, uint8_t *used_g,
                                                             int *piv_p, int *piv_q, int *piv_pos, StepState *st,
                                                             int round, int limit, int nn, int n2max) {
   // LDS: the queue of alive entries (values, positions; SW_CHUNK each, 16-byte aligned) | n + 1
@@ -304,21 +302,20 @@ __global__ void __launch_bounds__(SW_WAVES * 64) k_sel_walk(int n, const uint64_
   uint32_t *lds = lds_raw + 2 * SW_CHUNK;  // the flag words
   uint8_t *used = (uint8_t *)lds;
   int *bp = (int *)(lds + nw + 1), *bq = bp + n2max, *bpos = bq + n2max;
-  SortResult sr = sort_result_dev(k0, k1, v0, v1, sort_skip);
-  const uint32_t *vals = HAS_POS ? lp.val[(round - 1) & 1] : sr.vals;
+  const uint32_t *vals = HAS_POS ? lp.val[(round - 1) & 1] : svals;
   const uint32_t *poss = HAS_POS ? lp.pos[(round - 1) & 1] : vals;
   if (!HAS_POS) {
     if (wave == 0) {
-      // IT = number of finite weights = position of the first NaN key
-      const uint64_t *keys = sr.keys;
-      const int it = wave_first_true(0, nn, [&](int i) { return key_nan(keys[i]); });
-      const double wmax = (it > 0) ? key_weight(keys[0]) : 0.0;
+      // IT = number of finite weights (the NaN keys sort last); the largest weight is the
+      // smallest key, both found by the sort's histogram pass
+      const int it = sst->n - sst->nan_count;
+      const double wmax = (it > 0) ? key_weight(sst->minkey) : 0.0;
       int len = it;
       // the mixed solver's threshold: walk only the weights >= tau * (largest weight)
       const double tau = st->tau;
       if (tau > 0.0 && it > 0 && wmax > 0.0) {
         const uint64_t kt = weight_key(tau * wmax);  // keys above kt stand for smaller weights
-        len = wave_first_true(0, it, [&](int i) { return keys[i] > kt; });
+        len = wave_first_true(0, it, [&](int i) { return pair_key(keys, n, svals[i]) > kt; });
       }
       if (lane == 0) {
         st->it = it;
@@ -483,9 +480,7 @@ __device__ __forceinline__ void copy_used(uint8_t *used, const uint8_t *used_g, 
 // Filter of round r: survivors of [consumed, list_len) of the round's list whose
 // indices are both free.  fcount[t] = survivors of tile t, fcount[ntiles +
 // FT_WAVES t + w] = survivors of its wave sub-tile w ...
-__global__ void __launch_bounds__(FT_WAVES * 64) k_sel_count(int n, const uint64_t *k0, const uint64_t *k1,
-                                                             const uint32_t *v0, const uint32_t *v1,
-                                                             const int *sort_skip, ListPtrs lp,
+__global__ void __launch_bounds__(FT_WAVES * 64) k_sel_count(int n, const uint32_t *svals, ListPtrs lp,
                                                              const uint8_t *used_g, int *fcount, int ntiles,
                                                              const StepState *st, int round) {
   extern __shared__ __attribute__((aligned(16))) uint8_t used[];
@@ -496,8 +491,7 @@ __global__ void __launch_bounds__(FT_WAVES * 64) k_sel_count(int n, const uint64
   if (base >= len) return;  // counts of tiles past the list are never read
   copy_used(used, used_g, n);
   __syncthreads();
-  SortResult sr = sort_result_dev(k0, k1, v0, v1, sort_skip);
-  const uint32_t *vals = (round == 0) ? sr.vals : lp.val[(round - 1) & 1];
+  const uint32_t *vals = (round == 0) ? svals : lp.val[(round - 1) & 1];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int sub = base + wave * FT_SUB;
   const int end = (sub + FT_SUB < len) ? sub + FT_SUB : len;
@@ -528,9 +522,7 @@ __global__ void __launch_bounds__(FT_WAVES * 64) k_sel_count(int n, const uint64
 // ... and written in order to the next round's list (values and 1-based
 // positions in the sorted list); the tile holding the list's last entry
 // stores the new length in st->next_len.
-__global__ void __launch_bounds__(FT_WAVES * 64) k_sel_write(int n, const uint64_t *k0, const uint64_t *k1,
-                                                             const uint32_t *v0, const uint32_t *v1,
-                                                             const int *sort_skip, ListPtrs lp,
+__global__ void __launch_bounds__(FT_WAVES * 64) k_sel_write(int n, const uint32_t *svals, ListPtrs lp,
                                                              const uint8_t *used_g, const int *fcount, int ntiles,
                                                              StepState *st, int round) {
   extern __shared__ __attribute__((aligned(16))) uint8_t used[];
@@ -553,8 +545,7 @@ __global__ void __launch_bounds__(FT_WAVES * 64) k_sel_write(int n, const uint64
   const int tile_pre = out;
   for (int w = 0; w < wave; ++w) out += fcount[ntiles + FT_WAVES * t + w];
   if (threadIdx.x == 0 && t == (len - 1 - c0) / FT_TILE) st->next_len = tile_pre + fcount[t];
-  SortResult sr = sort_result_dev(k0, k1, v0, v1, sort_skip);
-  const uint32_t *vals = (round == 0) ? sr.vals : lp.val[(round - 1) & 1];
+  const uint32_t *vals = (round == 0) ? svals : lp.val[(round - 1) & 1];
   const uint32_t *poss = (round == 0) ? nullptr : lp.pos[(round - 1) & 1];
   uint32_t *oval = lp.val[round & 1], *opos = lp.pos[round & 1];
   const int sub = base + wave * FT_SUB;
@@ -602,8 +593,7 @@ template <bool F, class E>
 __global__ void __launch_bounds__(256) k_pivots(int n, const E *__restrict__ A, int lda,
                                                  const int *__restrict__ J, const int *__restrict__ piv_p,
                                                  const int *__restrict__ piv_q, const int *__restrict__ piv_pos,
-                                                 const uint64_t *k0, const uint64_t *k1, const uint32_t *v0,
-                                                 const uint32_t *v1, const int *sort_skip, double *piv_w,
+                                                 const uint64_t *keys, double *piv_w,
                                                  int *piv_info, double *piv_V,
                                                  double *piv_W, double *piv_d, Rot *rot_row, Rot *rot_col,
                                                  int *colpiv, StepState *st, double tol) {
@@ -618,9 +608,8 @@ __global__ void __launch_bounds__(256) k_pivots(int n, const E *__restrict__ A, 
   int k[2] = {J[p], J[q]};
   const int pos = piv_pos[i];
   if (pos > 0) {
-    // the pivot's weight (for NORMF2D) from its position in the sorted list
-    SortResult sr = sort_result_dev(k0, k1, v0, v1, sort_skip);
-    piv_w[i] = key_weight(sr.keys[pos - 1]);
+    // the pivot's weight (for NORMF2D): its key, stored at its pair index
+    piv_w[i] = key_weight(keys[pair_index(n, p + 1, q + 1)]);
   } else {
     // cyclic ordering (no list): the pair's off-diagonal mass; a negligible pair whose
     // transform would otherwise be the identity is left alone (INFO 8: no transform, no
@@ -1033,7 +1022,7 @@ void weights_launch(DeviceStep &d, hipStream_t s) {
   const int nt = (d.n + TWT - 1) / TWT;
   const double tol = d.wmode ? d.neg_tol : 0.0;
   auto trig = [&](auto kern, const auto *A, int cut) {
-    hipLaunchKernelGGL(kern, dim3(nt, nt), dim3(256), 0, s, d.n, A, d.lda, d.J, d.sort.k0, d.sort.v0, d.st, tol, cut);
+    hipLaunchKernelGGL(kern, dim3(nt, nt), dim3(256), 0, s, d.n, A, d.lda, d.J, d.sort.k0, d.st, tol, cut);
   };
   if (d.prec32) {  // the FP32 warm start (Gram-form hyperbolic weights only)
     const float *A = (const float *)d.A32;
@@ -1053,10 +1042,10 @@ void weights_launch(DeviceStep &d, hipStream_t s) {
     dim3 g((d.npl + HT - 1) / HT, (d.nmi + HT - 1) / HT);
     if (d.fused)
       hipLaunchKernelGGL(k_hyp_weights<true>, g, dim3(256), 0, s, d.n, A, d.lda, d.J, d.plus, d.npl, d.minus, d.nmi,
-                         d.sort.k0, d.sort.v0, d.st);
+                         d.sort.k0, d.st);
     else
       hipLaunchKernelGGL(k_hyp_weights<false>, g, dim3(256), 0, s, d.n, A, d.lda, d.J, d.plus, d.npl, d.minus,
-                         d.nmi, d.sort.k0, d.sort.v0, d.st);
+                         d.nmi, d.sort.k0, d.st);
   }
 }
 
@@ -1093,18 +1082,16 @@ void select_launch(DeviceStep &d, hipStream_t s) {
   for (int r = 0; r < rounds; ++r) {
     const int lim = (r == rounds - 1) ? 0 : limit;
     if (r == 0)
-      hipLaunchKernelGGL(k_sel_walk<false>, dim3(1), dim3(SW_WAVES * 64), ldsw, s, d.n, d.sort.k0, d.sort.k1,
-                         d.sort.v0, d.sort.v1, d.sort_skip, d.lists, d.used, d.piv_p, d.piv_q, d.piv_pos, d.st, r,
+      hipLaunchKernelGGL(k_sel_walk<false>, dim3(1), dim3(SW_WAVES * 64), ldsw, s, d.n, d.sort.k0, d.sort.v0,
+                         sort_stats(d.sort), d.lists, d.used, d.piv_p, d.piv_q, d.piv_pos, d.st, r,
                          lim, d.nn, d.n2max);
     else
-      hipLaunchKernelGGL(k_sel_walk<true>, dim3(1), dim3(SW_WAVES * 64), ldsw, s, d.n, d.sort.k0, d.sort.k1,
-                         d.sort.v0, d.sort.v1, d.sort_skip, d.lists, d.used, d.piv_p, d.piv_q, d.piv_pos, d.st, r,
+      hipLaunchKernelGGL(k_sel_walk<true>, dim3(1), dim3(SW_WAVES * 64), ldsw, s, d.n, d.sort.k0, d.sort.v0,
+                         sort_stats(d.sort), d.lists, d.used, d.piv_p, d.piv_q, d.piv_pos, d.st, r,
                          lim, d.nn, d.n2max);
     if (r == rounds - 1) break;
-    hipLaunchKernelGGL(k_sel_count, dim3(ntiles), dim3(FT_WAVES * 64), lds, s, d.n, d.sort.k0, d.sort.k1, d.sort.v0,
-                       d.sort.v1, d.sort_skip, d.lists, d.used, d.fcount, ntiles, d.st, r);
-    hipLaunchKernelGGL(k_sel_write, dim3(ntiles), dim3(FT_WAVES * 64), lds, s, d.n, d.sort.k0, d.sort.k1, d.sort.v0,
-                       d.sort.v1, d.sort_skip, d.lists, d.used, d.fcount, ntiles, d.st, r);
+    hipLaunchKernelGGL(k_sel_count, dim3(ntiles), dim3(FT_WAVES * 64), lds, s, d.n, d.sort.v0, d.lists, d.used, d.fcount, ntiles, d.st, r);
+    hipLaunchKernelGGL(k_sel_write, dim3(ntiles), dim3(FT_WAVES * 64), lds, s, d.n, d.sort.v0, d.lists, d.used, d.fcount, ntiles, d.st, r);
   }
 }
 
@@ -1116,7 +1103,7 @@ void pivots_launch(DeviceStep &d, hipStream_t s) {
   const int g = (d.n2max + 255) / 256;
   auto go = [&](auto kern, const auto *A) {
     hipLaunchKernelGGL(kern, dim3(g), dim3(256), 0, s, d.n, A, d.lda, d.J, d.piv_p, d.piv_q, d.piv_pos, d.sort.k0,
-                       d.sort.k1, d.sort.v0, d.sort.v1, d.sort_skip, d.piv_w, d.piv_info, (double *)d.piv_V,
+                       d.piv_w, d.piv_info, (double *)d.piv_V,
                        (double *)d.piv_W, d.piv_d, d.rot_row, d.rot_col, d.colpiv, d.st, d.neg_tol);
   };
   if (d.prec32) {
@@ -1181,7 +1168,7 @@ void step_launch(DeviceStep &d, hipStream_t s) {
   }
   if (d.ctl_mode) controller_launch(d, s);
   weights_launch(d, s);
-  sort_pairs_launch(d.sort, &d.st->nn_sort, d.nn, s, 1);
+  sort_pairs_launch(d.sort, &d.st->nn_sort, d.nn, s, d.n, (d.prec32 && d.cut32) ? 32 : 0);
   select_launch(d, s);
   pivots_launch(d, s);
   rotate_launch(d, s);

@@ -92,7 +92,6 @@ struct DeviceStep {
   int *J;                 // int32 signs
   int *plus, *minus;      // 0-based column indices with J = +1 / -1
   SortBuffers sort;
-  int *sort_skip;
   // selection lists (round >= 1): keys, pair indices, 1-based sorted positions
   ListPtrs lists;
   int *fcount;            // survivor counts of the filter: per tile, then per wave sub-tile

@@ -121,7 +121,7 @@ __device__ __forceinline__ void zrot_ab_apply(const ZRotF &R, fz &x, fz &y) {
 template <bool F, class E>
 __global__ void __launch_bounds__(256) k_ztrig_weights(int n, const E *__restrict__ A, int lda,
                                                         const int *__restrict__ J, uint64_t *__restrict__ keys,
-                                                        uint32_t *__restrict__ vals, const StepState *st, int nn,
+                                                        const StepState *st, int nn,
                                                         double tol, int cut) {
   if (!st->active) return;
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < (uint32_t)nn; k += gridDim.x * blockDim.x) {
@@ -140,7 +140,6 @@ __global__ void __launch_bounds__(256) k_ztrig_weights(int n, const E *__restric
     // tol > 0 (mixed solver only): the convergence test, as for the real pairs
     if (tol > 0.0 && go && !diag && !order && negligible(absz(aqp), absz(apq), app.re, aqq.re, tol)) go = false;
     keys[k] = key_cut(go ? weight_key(dasum4<F>(aqp.re, aqp.im, apq.re, apq.im)) : KEY_NAN, cut);
-    vals[k] = pack_pq(p - 1, q - 1);
   }
 }
 
@@ -151,7 +150,7 @@ template <bool F>
 __global__ void __launch_bounds__(256) k_zhyp_weights(int n, const dz *__restrict__ A, int lda,
                                                        const int *__restrict__ J, const int *__restrict__ plus,
                                                        int npl, const int *__restrict__ minus, int nmi,
-                                                       uint64_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                                                       uint64_t *__restrict__ keys,
                                                        StepState *st) {
   if (!st->active) return;
   __shared__ dz L[ZHR][2 * ZHT + 1];
@@ -227,7 +226,6 @@ __global__ void __launch_bounds__(256) k_zhyp_weights(int n, const dz *__restric
   if (valid) {
     const uint32_t k = pair_index(n, p0 + 1, q0 + 1);
     keys[k] = key;
-    vals[k] = pack_pq(p0, q0);
   }
 }
 
@@ -235,9 +233,8 @@ __global__ void __launch_bounds__(256) k_zhyp_weights(int n, const dz *__restric
 template <bool F, class E>
 __global__ void __launch_bounds__(256) k_zpivots(int n, const E *__restrict__ A, int lda, const int *__restrict__ J,
                                                   const int *__restrict__ piv_p, const int *__restrict__ piv_q,
-                                                  const int *__restrict__ piv_pos, const uint64_t *k0,
-                                                  const uint64_t *k1, const uint32_t *v0, const uint32_t *v1,
-                                                  const int *sort_skip, double *piv_w, int *piv_info,
+                                                  const int *__restrict__ piv_pos, const uint64_t *keys,
+                                                  double *piv_w, int *piv_info,
                                                   double *piv_d, ZRot *zrot, int *colpiv, StepState *st) {
   if (!st->active) return;
   const int sl = st->sl;
@@ -246,10 +243,7 @@ __global__ void __launch_bounds__(256) k_zpivots(int n, const E *__restrict__ A,
   if (i >= sl) return;
   const int p = piv_p[i], q = piv_q[i];
   const size_t P = p, Q = q;
-  {
-    SortResult sr = sort_result_dev(k0, k1, v0, v1, sort_skip);
-    piv_w[i] = key_weight(sr.keys[piv_pos[i] - 1]);
-  }
+  piv_w[i] = key_weight(keys[pair_index(n, p + 1, q + 1)]);  // the pivot's key, at its pair index
   dz b[4] = {ldz(A[P * lda + P]), ldz(A[P * lda + Q]), ldz(A[Q * lda + P]), ldz(A[Q * lda + Q])};
   int k[2] = {J[p], J[q]};
   dz v[4], w[4];
@@ -409,7 +403,7 @@ void zweights_launch(DeviceStep &d, hipStream_t s) {
   const double tol = d.wmode ? d.neg_tol : 0.0;
   const dim3 gt(zgrid(d.nn, 256, 65536));
   auto trig = [&](auto kern, const auto *A) {
-    hipLaunchKernelGGL(kern, gt, dim3(256), 0, s, d.n, A, d.lda, d.J, d.sort.k0, d.sort.v0, d.st, d.nn, tol,
+    hipLaunchKernelGGL(kern, gt, dim3(256), 0, s, d.n, A, d.lda, d.J, d.sort.k0, d.st, d.nn, tol,
                        d.prec32 ? d.cut32 : 0);
   };
   if (d.prec32) {
@@ -430,10 +424,10 @@ void zweights_launch(DeviceStep &d, hipStream_t s) {
     dim3 g((d.npl + ZHT - 1) / ZHT, (d.nmi + ZHT - 1) / ZHT);
     if (d.fused)
       hipLaunchKernelGGL(k_zhyp_weights<true>, g, dim3(256), 0, s, d.n, A, d.lda, d.J, d.plus, d.npl, d.minus,
-                         d.nmi, d.sort.k0, d.sort.v0, d.st);
+                         d.nmi, d.sort.k0, d.st);
     else
       hipLaunchKernelGGL(k_zhyp_weights<false>, g, dim3(256), 0, s, d.n, A, d.lda, d.J, d.plus, d.npl, d.minus,
-                         d.nmi, d.sort.k0, d.sort.v0, d.st);
+                         d.nmi, d.sort.k0, d.st);
   }
 }
 
@@ -441,7 +435,7 @@ void zpivots_launch(DeviceStep &d, hipStream_t s) {
   const int g = (d.n2max + 255) / 256;
   auto go = [&](auto kern, const auto *A) {
     hipLaunchKernelGGL(kern, dim3(g), dim3(256), 0, s, d.n, A, d.lda, d.J, d.piv_p, d.piv_q, d.piv_pos, d.sort.k0,
-                       d.sort.k1, d.sort.v0, d.sort.v1, d.sort_skip, d.piv_w, d.piv_info, d.piv_d, (ZRot *)d.zrot,
+                       d.piv_w, d.piv_info, d.piv_d, (ZRot *)d.zrot,
                        d.colpiv, d.st);
   };
   if (d.prec32) {
