@@ -562,7 +562,7 @@ static int finish_run(jkg_ctx *c, jkg_result *res, const LoopState *ls = nullptr
   res->sum_it = (int64_t)s.acc_it;
   res->sum_b2 = (int64_t)s.acc_b2;
   res->sum_b4 = (int64_t)s.acc_b4;
-  res->sum_abod = (int64_t)s.acc_abod;
+  res->sum_abod = (int64_t)acc_abod_total(s);
   return JKG_OK;
 }
 
@@ -1135,7 +1135,7 @@ extern "C" int jkg_dsolve_noalloc(jkg_ctx *c, int n, int n2, const double *y, co
   res->sum_it = (int64_t)st.acc_it;
   res->sum_b2 = (int64_t)st.acc_b2;
   res->sum_b4 = (int64_t)st.acc_b4;
-  res->sum_abod = (int64_t)st.acc_abod;
+  res->sum_abod = (int64_t)acc_abod_total(st);
   return JKG_OK;
 }
 
@@ -1746,7 +1746,7 @@ static int profile_common(jkg_ctx *c, int n, int n2, const double *y, const int6
   res->sum_it = (int64_t)st.acc_it;
   res->sum_b2 = (int64_t)st.acc_b2;
   res->sum_b4 = (int64_t)st.acc_b4;
-  res->sum_abod = (int64_t)st.acc_abod;
+  res->sum_abod = (int64_t)acc_abod_total(st);
   double tot = 0;
   for (int k = 0; k < NST; ++k) tot += stage_ms[k];
   res->seconds = tot * 1e-3;

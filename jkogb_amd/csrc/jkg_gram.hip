@@ -351,7 +351,7 @@ __global__ void __launch_bounds__(256) k_colnorm_off(int n, const E *__restrict_
 // gathered through LDS so that every global read runs down a column; the
 // diagonal comes from dg.  The branch structure is DMAGF2H's; only the
 // ABODNDF2 sum is replaced by the quadratic form above.
-constexpr int WT = 64;
+constexpr int WT = 32;
 
 template <bool F, class E>
 __global__ void __launch_bounds__(256) k_hyp_weights_gram(int n, const E *__restrict__ A, int lda,
@@ -417,7 +417,11 @@ __global__ void __launch_bounds__(256) k_hyp_weights_gram(int n, const E *__rest
   }
   // work counter: pairs whose weight needed the column sums
   for (int o = 32; o > 0; o >>= 1) nfull += __shfl_xor(nfull, o);
-  if ((tid & 63) == 0 && nfull) atomicAdd((unsigned long long *)&st->acc_abod, (unsigned long long)nfull);
+  __shared__ unsigned wfull[4];
+  if ((tid & 63) == 0) wfull[tid >> 6] = nfull;
+  __syncthreads();
+  if (tid == 0 && wfull[0] + wfull[1] + wfull[2] + wfull[3])
+    acc_abod_add(st, (unsigned long long)(wfull[0] + wfull[1] + wfull[2] + wfull[3]));
 }
 
 static void zgemm_tn_f32_launch(int k, const float *X, int ldx, const int *xcols, int m, const int *ycols, int nc,
@@ -611,7 +615,11 @@ __global__ void __launch_bounds__(256) k_zhyp_weights_gram(int n, const E *__res
     keys[k] = key_cut(key, cut);
   }
   for (int o = 32; o > 0; o >>= 1) nfull += __shfl_xor(nfull, o);
-  if ((tid & 63) == 0 && nfull) atomicAdd((unsigned long long *)&st->acc_abod, (unsigned long long)nfull);
+  __shared__ unsigned wfull[4];
+  if ((tid & 63) == 0) wfull[tid >> 6] = nfull;
+  __syncthreads();
+  if (tid == 0 && wfull[0] + wfull[1] + wfull[2] + wfull[3])
+    acc_abod_add(st, (unsigned long long)(wfull[0] + wfull[1] + wfull[2] + wfull[3]));
 }
 
 void zgram_hyp_weights_launch(DeviceStep &d, hipStream_t s) {

@@ -172,7 +172,7 @@ __global__ void __launch_bounds__(256) k_hyp_weights(int n, const double *__rest
   __syncthreads();
   if (need) atomicAdd(&nneed, 1);
   if (__syncthreads_or(need)) {
-    if (threadIdx.x == 0) atomicAdd((unsigned long long *)&st->acc_abod, (unsigned long long)nneed);
+    if (threadIdx.x == 0) acc_abod_add(st, (unsigned long long)nneed);
     const int xc = xplus ? a : HT + b, yc = xplus ? HT + b : a;
     const int s1c = sel1 ? yc : xc, t1c = sel1 ? xc : yc;
     const bool eq = (sel1 == sel2);
@@ -234,7 +234,11 @@ __global__ void __launch_bounds__(256) k_hyp_weights(int n, const double *__rest
 constexpr int SW_WAVES = 16;
 constexpr int SW_ITEMS = 4;
 constexpr int SW_CHUNK = SW_WAVES * 64 * SW_ITEMS;  // entries per super-chunk
-static_assert(SW_ITEMS == 4, "one 16-byte load per lane and array");
+static_assert(SW_ITEMS % 4 == 0 && SW_ITEMS <= 16, "16-byte loads; per-lane counts below 32");
+static_assert(SW_CHUNK <= (1 << 14), "queue indices in the low 14 bits of first[]");
+constexpr int SW_QCAP = SW_CHUNK;  // queue entries: every alive entry of a super-chunk
+constexpr int LD_MIN = 192;        // alive entries of a super-chunk up to which wave 0 resolves them alone
+constexpr int LD_MAXIT = 12;       // locally dominant rounds per super-chunk before wave 0 takes the rest
 constexpr int FT_WAVES = 4;                          // waves of a filter workgroup
 constexpr int FT_SUB = 2048;                         // entries per wave sub-tile
 constexpr int FT_TILE = FT_WAVES * FT_SUB;           // entries per filter workgroup
@@ -280,6 +284,48 @@ __device__ __forceinline__ int wave_prefix_small(int cnt, int &total) {
   return __popcll(b0 & lt) + 2 * __popcll(b1 & lt) + 4 * __popcll(b2 & lt);
 }
 
+// Wave 0 resolves a queue of entries (packed pairs qv, tags qp) in order: the free-entry test
+// against used[] gives a wave-uniform mask of alive lanes; each acceptance is the first set bit,
+// a readlane of its pair and a ballot of the lanes that share an index with it (cleared from the
+// mask), all in scalar registers; then the accepted lanes mark their indices in used[] and
+// either record their pivots (pair, tag = 1-based list position) from index acc - acc0 on, as
+// long as acc < n2, or (taken != null) set taken[tag].  Returns the new acc.
+__device__ int resolve_queue(const uint32_t *qv, const uint32_t *qp, int total, uint8_t *used, int n, int acc,
+                             int acc0, int n2, int *bp, int *bq, int *bpos, uint8_t *taken) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t sent = (uint32_t)n * 0x10001u;  // the pair (n, n): dead through used[n] = 1
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int b = 0; b < total && acc < n2; b += 64) {
+    const int k = b + lane;
+    const uint32_t x = (k < total) ? qv[k] : sent;
+    const uint32_t xp = (k < total) ? qp[k] : 0u;
+    uint64_t A = __ballot((used[x & 0xFFFFu] | used[x >> 16]) == 0);
+    uint64_t M = 0;
+    int room = n2 - acc;
+    while (A != 0 && room > 0) {
+      const int L = __ffsll((unsigned long long)A) - 1;
+      const uint32_t gv = __builtin_amdgcn_readlane(x, L);
+      M |= 1ull << L;
+      --room;
+      A &= ~__ballot(pair_hits(x, (gv & 0xFFFFu) * 0x10001u, (gv >> 16) * 0x10001u) != 0);
+    }
+    if ((M >> lane) & 1ull) {
+      if (taken) {
+        taken[xp] = 1;
+      } else {
+        const int r = acc - acc0 + __popcll(M & lt);
+        bp[r] = (int)(x & 0xFFFFu);
+        bq[r] = (int)(x >> 16);
+        bpos[r] = (int)xp;
+      }
+      used[x & 0xFFFFu] = 1;
+      used[x >> 16] = 1;
+    }
+    acc += __popcll(M);
+  }
+  return acc;
+}
+
 // Round r of the walk.  Round 0 walks the sorted list (positions implicit);
 // round r > 0 the list the filter of round r - 1 wrote (values and 1-based
 // positions in lp.*[(r - 1) & 1], st->next_len entries).  A round walks at
@@ -290,24 +336,29 @@ __global__ void __launch_bounds__(SW_WAVES * 64) k_sel_walk(int n, const uint64_
                                                             ListPtrs lp, uint8_t *used_g,
                                                             int *piv_p, int *piv_q, int *piv_pos, StepState *st,
                                                             int round, int limit, int nn, int n2max) {
-  // LDS: the queue of alive entries (values, positions; SW_CHUNK each, 16-byte aligned) | n + 1
-  // flag bytes rounded up to words | this round's pivots (p, q, position; n2max each)
+  // LDS: the queue (values, tags; SW_QCAP each, 16-byte aligned) | n + 1 flag bytes rounded up to
+  // words | this round's pivots (p, q, position; n2max each) | first[] of the locally dominant
+  // rounds (n + 1 words) | taken flags of a super-chunk (SW_CHUNK bytes)
   extern __shared__ __attribute__((aligned(16))) uint32_t lds_raw[];
   __shared__ int wtot[2][SW_WAVES];
-  __shared__ int sh_len, sh_acc;
+  __shared__ int sh_len;
   if (st->sel_done || st->sel_round > round) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nw = n >> 2;
-  uint32_t *qv = lds_raw, *qp = lds_raw + SW_CHUNK;
-  uint32_t *lds = lds_raw + 2 * SW_CHUNK;  // the flag words
+  uint32_t *qv = lds_raw, *qp = lds_raw + SW_QCAP;
+  uint32_t *lds = lds_raw + 2 * SW_QCAP;  // the flag words
   uint8_t *used = (uint8_t *)lds;
   int *bp = (int *)(lds + nw + 1), *bq = bp + n2max, *bpos = bq + n2max;
+  uint32_t *first = (uint32_t *)(bpos + n2max);
+  uint8_t *taken = (uint8_t *)(first + n + 1);
+  for (int i = tid; i <= n; i += SW_WAVES * 64) first[i] = 0xFFFFFFFFu;
+  for (int i = tid; i < SW_CHUNK / 4; i += SW_WAVES * 64) ((uint32_t *)taken)[i] = 0u;
   const uint32_t *vals = HAS_POS ? lp.val[(round - 1) & 1] : svals;
   const uint32_t *poss = HAS_POS ? lp.pos[(round - 1) & 1] : vals;
   if (!HAS_POS) {
     if (wave == 0) {
       // IT = number of finite weights (the NaN keys sort last); the largest weight is the
-      // smallest key, both found by the sort's histogram pass
+      // smallest key, both found by the sort's first pass
       const int it = sst->n - sst->nan_count;
       const double wmax = (it > 0) ? key_weight(sst->minkey) : 0.0;
       int len = it;
@@ -344,28 +395,79 @@ __global__ void __launch_bounds__(SW_WAVES * 64) k_sel_walk(int n, const uint64_
   const int stop = (limit > 0 && len > limit) ? limit : len;
   // Buffer loads whose range ends at `stop` rounded up to whole 16-byte groups (inside the
   // allocations): loads past it read zeros without touching memory, so every lane loads, and
-  // entries at or past `stop` are masked by position.
+  // entries at or past `stop` are masked by position.  (The range is made wave-uniform: a buffer
+  // resource in vector registers would make every load a loop.)
   typedef uint32_t u4 __attribute__((ext_vector_type(4)));
-  const int range = ((stop + 3) >> 2) << 4;
+  const int range = __builtin_amdgcn_readfirstlane(((stop + 3) >> 2) << 4);
   const __amdgpu_buffer_rsrc_t rsv = __builtin_amdgcn_make_buffer_rsrc((void *)vals, (short)0, range, 0x00020000);
   const __amdgpu_buffer_rsrc_t rsp = __builtin_amdgcn_make_buffer_rsrc((void *)poss, (short)0, range, 0x00020000);
   const int loff = tid * SW_ITEMS * 4;
-  auto load = [&](u4 &dv, u4 &dp, int base) {
-    dv = __builtin_amdgcn_raw_buffer_load_b128(rsv, loff + base * 4, 0, 0);
-    if constexpr (HAS_POS) dp = __builtin_amdgcn_raw_buffer_load_b128(rsp, loff + base * 4, 0, 0);
+  struct Chunk {
+    u4 v[SW_ITEMS / 4], p[SW_ITEMS / 4];
+  };
+  auto load = [&](Chunk &ch, int base) {
+#pragma unroll
+    for (int g = 0; g < SW_ITEMS / 4; ++g) {
+      ch.v[g] = __builtin_amdgcn_raw_buffer_load_b128(rsv, loff + 16 * g + base * 4, 0, 0);
+      if constexpr (HAS_POS) ch.p[g] = __builtin_amdgcn_raw_buffer_load_b128(rsp, loff + 16 * g + base * 4, 0, 0);
+    }
   };
   bool run = stop > 0 && acc < n2;
   int par = 0;
-  auto chunk = [&](const u4 &cv, const u4 &cp) {
-    const uint32_t vv[SW_ITEMS] = {cv.x, cv.y, cv.z, cv.w};
-    uint32_t pv[SW_ITEMS];
+  int ld_round = 0;  // locally dominant rounds so far (the tag of first[])
+  // Block-wide exclusive prefix of a per-lane count in 0..31 over the entries' order (lane-major
+  // within a wave, then wave by wave), and the total: five ballots per wave, one word per wave in
+  // LDS (double-buffered), one barrier.
+  auto block_prefix = [&](int cnt, int &total) -> int {
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int pre = 0, tot = 0;
+#pragma unroll
+    for (int b = 0; b < 5; ++b) {
+      const uint64_t bb = __ballot((cnt >> b) & 1);
+      pre += __popcll(bb & lt) << b;
+      tot += __popcll(bb) << b;
+    }
+    if (lane == 0) wtot[par][wave] = tot;
+    lds_barrier();
+    int base = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < SW_WAVES; ++w) {
+      const int t = wtot[par][w];
+      base += (w < wave) ? t : 0;
+      all += t;
+    }
+    par ^= 1;
+    total = all;
+    return base + pre;
+  };
+  // whether any lane of the workgroup has a nonzero value (cheaper than __syncthreads_or)
+  auto block_any = [&](uint32_t v) -> bool {
+    const uint64_t b = __ballot(v != 0u);
+    if (lane == 0) wtot[par][wave] = (b != 0ull) ? 1 : 0;
+    lds_barrier();
+    int t = 0;
+#pragma unroll
+    for (int w = 0; w < SW_WAVES; ++w) t |= wtot[par][w];
+    par ^= 1;
+    return t != 0;
+  };
+  auto chunk = [&](const Chunk &ch) {
+    uint32_t vv[SW_ITEMS], pv[SW_ITEMS];
     const int e0 = c + tid * SW_ITEMS;
-    if constexpr (HAS_POS) {
-      pv[0] = cp.x;
-      pv[1] = cp.y;
-      pv[2] = cp.z;
-      pv[3] = cp.w;
-    } else {
+#pragma unroll
+    for (int g = 0; g < SW_ITEMS / 4; ++g) {
+      vv[4 * g] = ch.v[g].x;
+      vv[4 * g + 1] = ch.v[g].y;
+      vv[4 * g + 2] = ch.v[g].z;
+      vv[4 * g + 3] = ch.v[g].w;
+      if constexpr (HAS_POS) {
+        pv[4 * g] = ch.p[g].x;
+        pv[4 * g + 1] = ch.p[g].y;
+        pv[4 * g + 2] = ch.p[g].z;
+        pv[4 * g + 3] = ch.p[g].w;
+      }
+    }
+    if constexpr (!HAS_POS) {
 #pragma unroll
       for (int i = 0; i < SW_ITEMS; ++i) pv[i] = (uint32_t)(e0 + i + 1);
     }
@@ -373,82 +475,126 @@ __global__ void __launch_bounds__(SW_WAVES * 64) k_sel_walk(int n, const uint64_
 #pragma unroll
     for (int i = 0; i < SW_ITEMS; ++i)
       alive |= ((e0 + i < stop) && (used[vv[i] & 0xFFFFu] | used[vv[i] >> 16]) == 0 ? 1u : 0u) << i;
-    int wsum;
-    const int pre = wave_prefix_small(__popc(alive), wsum);
-    if (lane == 0) wtot[par][wave] = wsum;
-    lds_barrier();
-    int base = 0, total = 0;
-#pragma unroll
-    for (int w = 0; w < SW_WAVES; ++w) {
-      const int t = wtot[par][w];
-      base += (w < wave) ? t : 0;
-      total += t;
-    }
-    par ^= 1;
-    int o = base + pre;
-#pragma unroll
-    for (int i = 0; i < SW_ITEMS; ++i)
-      if ((alive >> i) & 1u) {
-        qv[o] = vv[i];
-        qp[o] = pv[i];
-        ++o;
-      }
+    int total;
+    const int pre = block_prefix(__popc(alive), total);
     c = (c + SW_CHUNK < stop) ? c + SW_CHUNK : stop;
-    if (total > 0) {
-      lds_barrier();
-      if (wave == 0) {
-        // the queue 64 entries at a time, one per lane: the free-entry test against used[] gives a
-        // wave-uniform mask of alive lanes; each acceptance is the first set bit, a readlane of its
-        // pair and a ballot of the lanes that share an index with it (cleared from the mask), all in
-        // scalar registers; then the accepted lanes record their pivots (in lane order, which is
-        // list order) and mark their indices in used[] together
-        const uint32_t sent = (uint32_t)n * 0x10001u;  // the pair (n, n): dead through used[n] = 1
-        const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-        for (int b = 0; b < total && acc < n2; b += 64) {
-          const int k = b + lane;
-          const uint32_t x = (k < total) ? qv[k] : sent;
-          const uint32_t xp = (k < total) ? qp[k] : 0u;
-          uint64_t A = __ballot((used[x & 0xFFFFu] | used[x >> 16]) == 0);
-          uint64_t M = 0;
-          int room = n2 - acc;
-          while (A != 0 && room > 0) {
-            const int L = __ffsll((unsigned long long)A) - 1;
-            const uint32_t gv = __builtin_amdgcn_readlane(x, L);
-            M |= 1ull << L;
-            --room;
-            A &= ~__ballot(pair_hits(x, (gv & 0xFFFFu) * 0x10001u, (gv >> 16) * 0x10001u) != 0);
-          }
-          if ((M >> lane) & 1ull) {
-            const int r = acc - acc0 + __popcll(M & lt);
-            bp[r] = (int)(x & 0xFFFFu);
-            bq[r] = (int)(x >> 16);
-            bpos[r] = (int)xp;
-            used[x & 0xFFFFu] = 1;
-            used[x >> 16] = 1;
-          }
-          acc += __popcll(M);
+    if (total > 0 && total <= LD_MIN) {
+      // few entries alive: the queue, resolved in order by wave 0 (each acceptance is a few
+      // scalar instructions; rounds over the whole workgroup cost more here)
+      int o = pre;
+#pragma unroll
+      for (int i = 0; i < SW_ITEMS; ++i)
+        if ((alive >> i) & 1u) {
+          qv[o] = vv[i];
+          qp[o] = pv[i];
+          ++o;
         }
-        if (lane == 0) sh_acc = acc;
-      }
       lds_barrier();
-      acc = sh_acc;
+      if (wave == 0) acc = resolve_queue(qv, qp, total, used, n, acc, acc0, n2, bp, bq, bpos, nullptr);
+      if (wave == 0 && lane == 0) wtot[par][0] = acc;
+      lds_barrier();
+      acc = wtot[par][0];
+      par ^= 1;
+    } else if (total > 0) {
+      // Locally dominant rounds over the whole super-chunk: an alive entry that comes first (in
+      // list order) among the alive entries at both of its indices is taken by the sequential
+      // walk, and every entry sharing an index with a taken one is dead.  first[x] holds the
+      // least super-chunk index of an alive entry at index x, tagged with a counter that falls
+      // with every round, so values of earlier rounds never win and first[] is never cleared.
+      uint32_t al = alive, accm = 0;
+      int it = 0;
+      while (true) {
+#pragma unroll
+        for (int i = 0; i < SW_ITEMS; ++i)
+          if (((al >> i) & 1u) && (used[vv[i] & 0xFFFFu] | used[vv[i] >> 16])) al &= ~(1u << i);
+        if (!block_any(al) || it == LD_MAXIT) break;
+        if (ld_round == 0xFFFF) {  // the tags ran out: first[] anew
+          for (int i = tid; i <= n; i += SW_WAVES * 64) first[i] = 0xFFFFFFFFu;
+          ld_round = 0;
+          lds_barrier();
+        }
+        const uint32_t tag = (0xFFFFu - (uint32_t)ld_round) << 14;
+#pragma unroll
+        for (int i = 0; i < SW_ITEMS; ++i)
+          if ((al >> i) & 1u) {
+            const uint32_t me = tag | (uint32_t)(tid * SW_ITEMS + i);
+            atomicMin(&first[vv[i] & 0xFFFFu], me);
+            atomicMin(&first[vv[i] >> 16], me);
+          }
+        lds_barrier();
+        uint32_t win = 0;
+#pragma unroll
+        for (int i = 0; i < SW_ITEMS; ++i)
+          if ((al >> i) & 1u) {
+            const uint32_t me = tag | (uint32_t)(tid * SW_ITEMS + i);
+            if (first[vv[i] & 0xFFFFu] == me && first[vv[i] >> 16] == me) win |= 1u << i;
+          }
+#pragma unroll
+        for (int i = 0; i < SW_ITEMS; ++i)
+          if ((win >> i) & 1u) {
+            used[vv[i] & 0xFFFFu] = 1;
+            used[vv[i] >> 16] = 1;
+          }
+        accm |= win;
+        al &= ~win;
+        lds_barrier();
+        ++it;
+        ++ld_round;
+      }
+      if (it == LD_MAXIT && block_any(al)) {
+        // rounds exhausted (a long chain of conflicts): the undecided entries, in list order,
+        // resolved by wave 0; an entry taken by a round is never at an index of an earlier
+        // undecided entry, so the flags the rounds set kill none of them wrongly
+        int total2;
+        int o = block_prefix(__popc(al), total2);
+#pragma unroll
+        for (int i = 0; i < SW_ITEMS; ++i)
+          if ((al >> i) & 1u) {
+            qv[o] = vv[i];
+            qp[o] = (uint32_t)(tid * SW_ITEMS + i);
+            ++o;
+          }
+        lds_barrier();
+        if (wave == 0) resolve_queue(qv, qp, total2, used, n, 0, 0, 0x7FFFFFFF, nullptr, nullptr, nullptr, taken);
+        lds_barrier();
+#pragma unroll
+        for (int i = 0; i < SW_ITEMS; ++i)
+          if (((al >> i) & 1u) && taken[tid * SW_ITEMS + i]) {
+            accm |= 1u << i;
+            taken[tid * SW_ITEMS + i] = 0;
+          }
+      }
+      // record the super-chunk's pivots in list order, as many as N_2 leaves room for
+      int tacc;
+      int r = acc + block_prefix(__popc(accm), tacc);
+#pragma unroll
+      for (int i = 0; i < SW_ITEMS; ++i)
+        if ((accm >> i) & 1u) {
+          if (r < n2) {
+            bp[r - acc0] = (int)(vv[i] & 0xFFFFu);
+            bq[r - acc0] = (int)(vv[i] >> 16);
+            bpos[r - acc0] = (int)pv[i];
+          }
+          ++r;
+        }
+      acc = (acc + tacc < n2) ? acc + tacc : n2;
     }
     run = c < stop && acc < n2;
   };
   // two super-chunks in registers, ping-pong: the next one's loads are issued before the current
-  // one is tested, so they have a whole super-chunk of work to arrive in
-  u4 va, pa, vb, pb;
+  // one is tested
+  Chunk ca, cb;
   int nb = 0;
-  load(va, pa, nb);
+  load(ca, nb);
   nb += SW_CHUNK;
   while (run) {
-    load(vb, pb, nb);
+    load(cb, nb);
     nb += SW_CHUNK;
-    chunk(va, pa);
+    chunk(ca);
     if (!run) break;
-    load(va, pa, nb);
+    load(ca, nb);
     nb += SW_CHUNK;
-    chunk(vb, pb);
+    chunk(cb);
   }
   lds_barrier();
   {
@@ -644,7 +790,8 @@ __global__ void __launch_bounds__(256) k_pivots(int n, const E *__restrict__ A, 
   piv_d[2 * i + 1] = b[3];
   rot_row[i] = rot_ba(v);
   rot_col[i] = rot_ab(w);
-  if (info & 2) atomicAdd(&st->nrow, 1);
+  // only whether any pivot has a row transform is read (a flag, not a count: no atomics)
+  if (info & 2) st->nrow = 1;
 }
 
 // ------------------------------------------------------------------ (e) rotate
@@ -1050,7 +1197,8 @@ void weights_launch(DeviceStep &d, hipStream_t s) {
 }
 
 static size_t select_lds_bytes(const DeviceStep &d) {
-  return (((size_t)d.n >> 2) + 1) * 4 + (size_t)3 * d.n2max * sizeof(int) + (size_t)2 * SW_CHUNK * sizeof(uint32_t);
+  return (((size_t)d.n >> 2) + 1) * 4 + (size_t)3 * d.n2max * sizeof(int) + (size_t)2 * SW_QCAP * sizeof(uint32_t) +
+         ((size_t)d.n + 1) * sizeof(uint32_t) + SW_CHUNK;
 }
 
 int select_prepare(DeviceStep &d) {

@@ -81,7 +81,22 @@ struct StepState {
   int quiet;  // cyclic ordering: consecutive steps without a transform
   // work counters summed over the steps (bytes and flops of the run)
   unsigned long long acc_sl, acc_it, acc_b2, acc_b4, acc_abod;
+  // hyperbolic weights that needed the column sums, added per workgroup into one of 64 slots
+  // (blockIdx mod 64; one address for every workgroup would queue their atomics): the run's
+  // count is acc_abod plus their sum
+  unsigned long long acc_abod_part[64];
 };
+
+// Adds v to the work counter of hyperbolic pairs (one call per workgroup).
+__device__ __forceinline__ void acc_abod_add(StepState *st, unsigned long long v) {
+  const unsigned b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  atomicAdd(&st->acc_abod_part[b & 63], v);
+}
+inline unsigned long long acc_abod_total(const StepState &s) {
+  unsigned long long t = s.acc_abod;
+  for (int i = 0; i < 64; ++i) t += s.acc_abod_part[i];
+  return t;
+}
 
 struct DeviceStep {
   int n, lda, cplx, fused;

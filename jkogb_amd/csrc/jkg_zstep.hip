@@ -198,7 +198,7 @@ __global__ void __launch_bounds__(256) k_zhyp_weights(int n, const dz *__restric
   __syncthreads();
   if (need) atomicAdd(&nneed, 1);
   if (__syncthreads_or(need)) {
-    if (threadIdx.x == 0) atomicAdd((unsigned long long *)&st->acc_abod, (unsigned long long)nneed);
+    if (threadIdx.x == 0) acc_abod_add(st, (unsigned long long)nneed);
     const int xc = xplus ? a : ZHT + b, yc = xplus ? ZHT + b : a;
     for (int r0 = 0; r0 < n; r0 += ZHR) {
       __syncthreads();
@@ -262,7 +262,7 @@ __global__ void __launch_bounds__(256) k_zpivots(int n, const E *__restrict__ A,
   const dz vh[4] = {zconj(v[0]), zconj(v[2]), zconj(v[1]), zconj(v[3])};
   zrot[3 * i + 1] = zrot_ab<F>(vh);
   zrot[3 * i + 2] = zrot_ab<F>(w);
-  if (info & 2) atomicAdd(&st->nrow, 1);
+  if (info & 2) st->nrow = 1;  // a flag: whether any pivot has a row transform
 }
 
 // ---------------------------------------------------------------- rotate
