@@ -796,6 +796,7 @@ __global__ void __launch_bounds__(256) k_pivots(int n, const E *__restrict__ A, 
 
 // ------------------------------------------------------------------ (e) rotate
 // E = float: the FP32 warm start (descriptors rounded once per pivot, products in FP32).
+constexpr int ROT_RPF = 4;  // rows per lane held in registers (N <= 4096 entirely)
 template <bool F, class E>
 __global__ void __launch_bounds__(1024) k_rotate(int n, E *__restrict__ A, int lda, E *__restrict__ U,
                                                 E *__restrict__ Z, const int *__restrict__ piv_p,
@@ -817,10 +818,43 @@ __global__ void __launch_bounds__(1024) k_rotate(int n, E *__restrict__ A, int l
   const bool col_rot = (j >= 0) && err == 0 && info >= 0 && (info & 4);
   const bool diag_set = (j >= 0) && err == 0 && info >= 0 && !(info & 8);
   const int tid = threadIdx.x, bs = blockDim.x;
-  if (nrow > 0 || col_rot || diag_set) {
-    E *col0 = A + (size_t)c * lda;
-    E *col1 = (c1 >= 0) ? A + (size_t)c1 * lda : nullptr;
-    for (int r = tid; r < n; r += bs) {
+  const bool a_phase = nrow > 0 || col_rot || diag_set;
+  const bool do_u = j >= 0 && info >= 0 && (info & 2), do_z = j >= 0 && info >= 0 && (info & 4);
+  E *col0 = A + (size_t)c * lda;
+  E *col1 = (c1 >= 0) ? A + (size_t)c1 * lda : nullptr;
+  E *u0 = U + (size_t)c * lda, *u1 = (c1 >= 0) ? U + (size_t)c1 * lda : nullptr;
+  E *z0 = Z + (size_t)c * lda, *z1 = (c1 >= 0) ? Z + (size_t)c1 * lda : nullptr;
+  // The first ROT_RPF rows per lane of A's two columns, then of the pivot's U and Z columns, are
+  // loaded together up front, so the U and Z loads are in flight while A is transformed (A's
+  // loads are issued first: waiting for them does not wait for U and Z); rows past ROT_RPF * bs
+  // (N > 4096) go through the same steps in plain loops.
+  E ra0[ROT_RPF], ra1[ROT_RPF], ru0[ROT_RPF], ru1[ROT_RPF], rz0[ROT_RPF], rz1[ROT_RPF];
+#pragma unroll
+  for (int k = 0; k < ROT_RPF; ++k) {
+    const int r = tid + k * bs;
+    const bool in = r < n;
+    ra0[k] = (a_phase && in) ? col0[r] : (E)0;
+    ra1[k] = (a_phase && in && col1) ? col1[r] : (E)0;
+  }
+#pragma unroll
+  for (int k = 0; k < ROT_RPF; ++k) {
+    const int r = tid + k * bs;
+    const bool in = r < n;
+    ru0[k] = (do_u && in) ? u0[r] : (E)0;
+    ru1[k] = (do_u && in) ? u1[r] : (E)0;
+    rz0[k] = (do_z && in) ? z0[r] : (E)0;
+    rz1[k] = (do_z && in) ? z1[r] : (E)0;
+  }
+  if (a_phase) {
+#pragma unroll
+    for (int k = 0; k < ROT_RPF; ++k) {
+      const int r = tid + k * bs;
+      if (r < n) {
+        L[r] = ra0[k];
+        if (col1) L[n + r] = ra1[k];
+      }
+    }
+    for (int r = tid + ROT_RPF * bs; r < n; r += bs) {
       L[r] = col0[r];
       if (col1) L[n + r] = col1[r];
     }
@@ -866,26 +900,42 @@ __global__ void __launch_bounds__(1024) k_rotate(int n, E *__restrict__ A, int l
     }
   }
   // phase 1 on U (columns p,q times V^T) and Z (columns p,q times W)
-  if (j >= 0 && info >= 0) {
-    if (info & 2) {
-      const auto R = as_rot(rot_row[j], U);
-      E *u0 = U + (size_t)c * lda, *u1 = U + (size_t)c1 * lda;
-      for (int r = tid; r < n; r += bs) {
-        E x = u0[r], y = u1[r];
+  if (do_u) {
+    const auto R = as_rot(rot_row[j], U);
+#pragma unroll
+    for (int k = 0; k < ROT_RPF; ++k) {
+      const int r = tid + k * bs;
+      if (r < n) {
+        E x = ru0[k], y = ru1[k];
         rot_apply<F>(R, x, y);
         u0[r] = x;
         u1[r] = y;
       }
     }
-    if (info & 4) {
-      const auto R = as_rot(rot_col[j], Z);
-      E *z0 = Z + (size_t)c * lda, *z1 = Z + (size_t)c1 * lda;
-      for (int r = tid; r < n; r += bs) {
-        E x = z0[r], y = z1[r];
+    for (int r = tid + ROT_RPF * bs; r < n; r += bs) {
+      E x = u0[r], y = u1[r];
+      rot_apply<F>(R, x, y);
+      u0[r] = x;
+      u1[r] = y;
+    }
+  }
+  if (do_z) {
+    const auto R = as_rot(rot_col[j], Z);
+#pragma unroll
+    for (int k = 0; k < ROT_RPF; ++k) {
+      const int r = tid + k * bs;
+      if (r < n) {
+        E x = rz0[k], y = rz1[k];
         rot_apply<F>(R, x, y);
         z0[r] = x;
         z1[r] = y;
       }
+    }
+    for (int r = tid + ROT_RPF * bs; r < n; r += bs) {
+      E x = z0[r], y = z1[r];
+      rot_apply<F>(R, x, y);
+      z0[r] = x;
+      z1[r] = y;
     }
   }
 }
