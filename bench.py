@@ -28,8 +28,12 @@ Printed as one JSON line:
                the mixed solver and the trained controller by default (also the FP32
                warm start with --c5-variants mixed,learned,warm): device time, steps,
                the singular values against the plain (reference-arithmetic) run in
-               tests/golden/solve_c5_n4096_plain_sy.npy in units of N eps, and the
-               residuals; the plain solver's wall time is its measured per-step time
+               tests/golden/solve_c5_n4096_plain_sy.npy in units of N eps, the
+               residuals and the componentwise condition of the singular values
+               (kappa / N: how far the exact values move with one rounding of each
+               entry of Y); the reference's two builds' spread; with --c5-noise also
+               the mixed solver on Y moved by one ulp per entry (the method's own
+               spread); the plain solver's wall time is its measured per-step time
                times the 52 000 steps of that run, a labelled lower bound (it does
                not meet the reference's termination at this size)
   full_solve   complete HSVDs at N = 1024 (C2 input): plain, mixed, the FP32 warm
@@ -55,8 +59,9 @@ Printed as one JSON line:
 
 Algorithmic bytes of one step (N, NN = N(N-1)/2, IT finite weights, SL
 pivots, b2/b4 pivots with a row/column transform):
-  (a) weights  8 N^2 (read A) + 12 NN (one key and one pair value per pair)
-  (b) sort     24 NN (read + write one 8-byte key and 4-byte value)
+  (a) weights  8 N^2 (read A) + 8 NN (one key per pair, stored at the pair's index)
+  (b) sort     24 NN (one pass reading and writing an 8-byte key and a 4-byte value: the
+               reference figure; the sort itself makes eight passes over shrinking records)
   (c) select   12 IT (walk the sorted values once)
   (d) 2x2      112 SL
   (e) rotate   16 N^2 (A read + written once) + 32 N (b2 + b4) (U, Z columns)
@@ -82,7 +87,7 @@ FP64_PEAK_TFLOPS = 78.6
 
 def step_bytes(n, it, sl, b2, b4):
     nn = n * (n - 1) // 2
-    a = 8.0 * n * n + 12.0 * nn
+    a = 8.0 * n * n + 8.0 * nn
     b = 24.0 * nn
     c = 12.0 * it
     d = 112.0 * sl
@@ -165,12 +170,20 @@ def accuracy(y, j, r, ref, eps):
     out["res_rec"] = float(torch.linalg.norm(Y - (U * S[None, :]) @ ZZ) / torch.linalg.norm(Y))
     out["res_orth"] = float(torch.linalg.norm(U.T @ U - I))
     out["res_junit"] = float(torch.linalg.norm((ZZ * Jd[None, :]) @ ZZ.T - torch.diag(Jd)))
-    del Y, U, ZZ, S, I
+    # componentwise relative condition of each sigma_i under |dY| <= eps |Y|: first order,
+    # |d sigma_i| / sigma_i <= eps (|u_i|^T |Y| |z_i|) / sigma_i with Z = J ZZ^T J (Y Z = U S), in units
+    # of N eps: how far the exact values of the FP64 data move with one rounding of each entry
+    Z = (Jd[:, None] * ZZ.T) * Jd[None, :]
+    kap = ((U.abs() * (Y.abs() @ Z.abs())).sum(0) / S.abs()).cpu().numpy() / n
+    imin = int(np.argmin(np.abs(r["SY"])))
+    out["kappa_over_N"] = {"at_sigma_min": round(float(kap[imin]), 2), "max": round(float(kap.max()), 2),
+                           "median": round(float(np.median(kap)), 4)}
+    del Y, U, ZZ, S, I, Z
     torch.cuda.empty_cache()
     return out
 
 
-def c5_full_run(ctx, y, j, n2, fused, plain_ms, log, variants):
+def c5_full_run(ctx, y, j, n2, fused, plain_ms, log, variants, noise=False):
     """Complete solves of the N=4096 input (the headline wall time): the mixed solver, its FP32 warm
     start, the trained controller, and both together.  Accuracy against the plain solver's run of the
     same input (the reference's arithmetic; it never meets the reference's own termination at this size,
@@ -197,6 +210,13 @@ def c5_full_run(ctx, y, j, n2, fused, plain_ms, log, variants):
            "accuracy_reference": "tests/golden/solve_c5_n4096_plain_sy.npy (plain solver, 52000 steps)"
            if ref is not None else None,
            "bound_64_N_eps": 64}
+    fma_ref = os.path.join(ROOT, "tests", "golden", "solve_c5_n4096_plain_fma_sy.npy")
+    if ref is not None and os.path.exists(fma_ref):
+        spread = np.abs(np.load(fma_ref) - ref) / np.abs(ref) / (n * eps)
+        out["reference_builds_spread"] = {"what": "the plain solver with FMAD fused against unfused (the reference's "
+                                                  "two builds): per-index relative difference in N eps",
+                                          "max_N_eps": round(float(spread.max()), 1),
+                                          "over_64_N_eps": int(np.sum(spread > 64))}
     for name, kw in variants:
         kw = dict(kw)
         if kw.pop("learned", False):
@@ -212,6 +232,20 @@ def c5_full_run(ctx, y, j, n2, fused, plain_ms, log, variants):
                            "newton_schulz": [w["ns_u"], w["ns_z"]], "fallback": w["fallback"]}
         rec.update(accuracy(y, j, r, ref, eps))
         out[name] = rec
+        if name == "mixed" and noise:
+            # the same solver on Y moved to a neighbouring double in every entry (relative change <= eps):
+            # the spread of the method's own trajectory, against which a bound on the distance to one
+            # run of the reference can be read
+            rng = np.random.default_rng(7)
+            yp = np.where(rng.random(y.shape) < 0.5, np.nextafter(y, np.inf), np.nextafter(y, -np.inf))
+            rp = ctx.dsolve_mixed(yp, j, n2, fused=fused, trace_cap=1, progress=prog)
+            rel = np.abs(rp["SY"] - r["SY"]) / np.abs(r["SY"]) / (n * eps)
+            out["one_ulp_noise"] = {"what": "mixed solver on Y with every entry moved by one ulp, against the mixed "
+                                            "solver on Y: per-index relative difference in N eps",
+                                    "steps": rp["steps"], "max_N_eps": round(float(rel.max()), 1),
+                                    "median_N_eps": round(float(np.median(rel)), 4),
+                                    "over_64_N_eps": int(np.sum(rel > 64))}
+            log("c5 one-ulp noise: max %.1f N eps" % rel.max())
         log("c5 %s: %d steps, %.1f s, max %.1f N eps" % (name, r["steps"], r["seconds"], rec.get("sy_max_N_eps", -1)))
     return out
 
@@ -384,6 +418,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-full", action="store_true", help="skip the complete N=1024 solves")
     ap.add_argument("--no-c5-full", action="store_true", help="skip the complete N=4096 solves (minutes)")
+    ap.add_argument("--c5-noise", action="store_true",
+                    help="also the mixed solver on Y moved by one ulp per entry (the method's own spread)")
     ap.add_argument("--c5-variants", default="mixed,learned",
                     help="complete N=4096 solves: any of mixed, warm, learned, warm_learned")
     ap.add_argument("--no-complex", action="store_true")
@@ -430,7 +466,7 @@ def main():
         table = {"mixed": {}, "warm": {"fp32_warm": 1, "key_bits": 32}, "learned": {"learned": True},
                  "warm_learned": {"fp32_warm": 1, "key_bits": 32, "learned": True}}
         variants = [(v, table[v]) for v in args.c5_variants.split(",") if v in table]
-        c5 = c5_full_run(ctx, y, j, n2, args.fused, plain["ms_per_step"], log, variants)
+        c5 = c5_full_run(ctx, y, j, n2, args.fused, plain["ms_per_step"], log, variants, noise=args.c5_noise)
     full = None
     if not args.no_full:
         yc, jc = inputs.make("c2", n=1024, seed=0)
@@ -529,11 +565,14 @@ def main():
             {"bound": "hbm", "kernel": "k_rotate", "achieved": round(rot_gbs, 1), "peak": HBM_PEAK_GBS,
              "unit": "GB/s", "frac": round(rot_gbs / HBM_PEAK_GBS, 4), "traffic": (traffic or {}).get("rotate"),
              "bytes_per_launch": int(stage_bytes["rotate"]), "ms_per_launch": round(pm["rotate"], 4)},
-            {"bound": "hbm", "kernel": "select stage (k_sel_walk, k_sel_count, k_sel_write rounds)",
+            {"bound": "hbm", "kernel": "select stage (k_sel_walk rounds: locally dominant rounds over dense "
+                                       "super-chunks, wave-0 resolution of sparse ones; k_sel_count + "
+                                       "k_sel_write filters between rounds)",
              "achieved": round(sel_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
              "frac": round(sel_gbs / HBM_PEAK_GBS, 5), "traffic": (traffic or {}).get("select"),
              "bytes_per_launch": int(stage_bytes["select"]), "ms_per_launch": round(pm["select"], 4)},
-            {"bound": "hbm", "kernel": "sort stage (8 passes of k_sort_hist, k_sort_scan, k_sort_scatter)",
+            {"bound": "hbm", "kernel": "sort stage (8 passes of k_sort_count, k_sort_scan, k_sort_scatter over "
+                                       "records of 16, 8 and 4 bytes)",
              "achieved": round(sort_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
              "frac": round(sort_gbs / HBM_PEAK_GBS, 5), "traffic": (traffic or {}).get("sort"),
              "bytes_per_launch": int(stage_bytes["sort"]), "ms_per_launch": round(pm["sort"], 4)},
