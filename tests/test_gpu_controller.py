@@ -101,7 +101,7 @@ def test_phase_table_controller(ctx):
 
 @pytest.mark.parametrize("cfg,n", [("c1", 128), ("c2", 160)])
 def test_controlled_solve_accuracy(ctx, cfg, n):
-    # a controlled run stays within the spread of the reference's two FMA builds (+64 N eps)
+    # a controlled run stays within 64 N eps of the plain solver (the reference's arithmetic)
     from oracle import mlp_ref
     p = mlp_ref.init(13, scale=0.5)
     d = mlp_ref.unpack(p)
@@ -114,14 +114,22 @@ def test_controlled_solve_accuracy(ctx, cfg, n):
     assert out["info"] >= 0
     spread = np.max(np.abs(fused["SY"] - plain["SY"]) / np.abs(plain["SY"]))
     rel = np.max(np.abs(out["SY"] - plain["SY"]) / np.abs(plain["SY"]))
-    assert rel <= 4 * spread + 64 * n * EPS, (rel, spread)
+    print("%s N=%d: %.2f N eps from the plain solver; the reference builds' spread %.2f N eps"
+          % (cfg, n, rel / (n * EPS), spread / (n * EPS)))
+    assert rel <= 64 * n * EPS, (rel, spread)
 
 
 def test_trained_controller_gain(ctx):
     """What the trained network (jkogb_amd/ctl_params.npy) gains over the mixed solver without it, on an
     input of the learned-controller family that training and validation did not see (seed 7, N=512);
-    measured against the plain full-sort ordering: equal accuracy (within the spread of the reference's
-    two FMA builds + 64 N eps) and fewer steps in less device time."""
+    measured against the plain full-sort ordering: fewer steps in less device time, at equal accuracy.
+
+    Equal accuracy here: within 64 N eps of the plain solver (the reference's arithmetic) or within twice
+    the method's own spread on this input, whichever is larger -- the spread being how far the mixed
+    solver without a controller moves when every entry of Y moves by one ulp.  On this column-graded
+    input (scales 10^U(-6,6)) that spread is about 70 N eps and the reference's own two builds are 43 N eps
+    apart: a controller changes the trajectory, and a different trajectory lands that far from one run of
+    the reference whatever it does (handing back earlier or capping the threshold measured no closer)."""
     import os
     from jkogb_amd import ctl_train
     p = np.load(os.path.join(os.path.dirname(ctl_train.__file__), "ctl_params.npy"))
@@ -134,6 +142,13 @@ def test_trained_controller_gain(ctx):
     print("trained controller: %.3f s (%d steps) vs %.3f s (%d steps) without it: x%.3f; plain %.3f s (%d steps)"
           % (mlp["seconds"], mlp["steps"], base["seconds"], base["steps"], gain, plain["seconds"], plain["steps"]))
     spread = np.max(np.abs(fused["SY"] - plain["SY"]) / np.abs(plain["SY"]))
+    rng = np.random.default_rng(7)
+    Yp = np.where(rng.random(Y.shape) < 0.5, np.nextafter(Y, np.inf), np.nextafter(Y, -np.inf))
+    pert = ctx.dsolve_mixed(Yp, J, 256)
+    noise = np.max(np.abs(pert["SY"] - base["SY"]) / np.abs(base["SY"]))
     rel = np.max(np.abs(mlp["SY"] - plain["SY"]) / np.abs(plain["SY"]))
-    assert rel <= 4 * spread + 64 * 512 * EPS, (rel, spread)
+    print("trained controller N=512: %.2f N eps from the plain solver; the mixed solver on Y moved by one ulp: "
+          "%.2f N eps; the reference builds' spread %.2f N eps" % (rel / (512 * EPS), noise / (512 * EPS),
+                                                                  spread / (512 * EPS)))
+    assert rel <= max(64 * 512 * EPS, 2 * noise), (rel, noise)
     assert mlp["steps"] < base["steps"] and gain > 1.02, gain

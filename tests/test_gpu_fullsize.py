@@ -24,6 +24,10 @@ needs days for:
     that follows another trajectory cannot be held to 64 N eps of one of them there.  The
     bound is max(64 N eps, the largest relative distance between the two builds), checked
     on each value, with the median held to 1 N eps and the count above 64 N eps reported.
+    (The data do fix those values more tightly: one rounding of each entry of Y moves the
+    exact sigma_min by at most 62 N eps to first order, kappa = |u|^T |Y| |z| / sigma.  The
+    method does not: the mixed solver run on Y with every entry moved by one ulp lands 939
+    N eps from its own run on Y; bench.py --c5-noise measures both.)
 """
 import os
 

@@ -155,8 +155,8 @@ def test_mixed_graph_follows_options(ctx):
 
 @pytest.mark.parametrize("cfg,n", [("c1", 128), ("c2", 160)])
 def test_mixed_threshold(ctx, cfg, n):
-    # a walk cut at tau * (largest weight) changes the trajectory; the result stays within the spread
-    # between two valid builds of the reference (unfused and fused FMAD) plus 64 N eps
+    # a walk cut at tau * (largest weight) changes the trajectory; the result stays within 64 N eps of
+    # the plain solver (the reference's arithmetic)
     Y, J = inputs(cfg, n, 6)
     plain = ctx.dsolve(Y, J, n // 2)
     fused = ctx.dsolve(Y, J, n // 2, fused=True)
@@ -166,7 +166,9 @@ def test_mixed_threshold(ctx, cfg, n):
     assert thr["sum_it"] != base["sum_it"] or thr["steps"] != base["steps"]
     spread = np.max(np.abs(fused["SY"] - plain["SY"]) / np.abs(plain["SY"]))
     rel = np.max(np.abs(thr["SY"] - plain["SY"]) / np.abs(plain["SY"]))
-    assert rel <= 4 * spread + 64 * n * EPS, (rel, spread)
+    print("%s N=%d: %.2f N eps from the plain solver; the reference builds' spread %.2f N eps"
+          % (cfg, n, rel / (n * EPS), spread / (n * EPS)))
+    assert rel <= 64 * n * EPS, (rel, spread)
     rec, orth, jorth = residuals(Y, J, thr)
     prec, porth, pjorth = residuals(Y, J, plain)
     assert rec <= 4 * prec + 2 * n * EPS and orth <= 4 * porth + 2 * n * EPS, (rec, orth)

@@ -5,10 +5,10 @@ Tolerances, as for the real mixed solver:
     to the oracle; the Gram-form hyperbolic weights within 32 N eps of the
     per-pair scale s_pq = (c_p + c_q)(1 + |W|_F^2) + |A_qp|^2 + |A_pq|^2,
     c = off-diagonal column norms^2, W = the pair's 2x2 column factor;
-  - whole solves: the hyperbolic singular values within 4x the spread between
-    the reference's unfused and fused builds (the plain solver run both ways)
-    + 64 N eps of the plain solver's, index by index and sorted within each J
-    class; reconstruction and unitarity residuals within 4x the plain
+  - whole solves: the hyperbolic singular values within 64 N eps of the plain
+    solver's (the reference's arithmetic), index by index and sorted within each
+    J class (the spread between the reference's unfused and fused builds is
+    printed beside it); reconstruction and unitarity residuals within 4x the plain
     solver's + 2 N eps and 64 N eps, the J-unitarity residual within 4x the
     plain solver's + 2 N eps.
 """
@@ -104,8 +104,10 @@ def test_zmixed_solve_matches_plain(ctx, n, tol):
     mix = ctx.dsolve_mixed(Y, J, n // 2, tol=tol)
     assert plain["info"] >= 0 and mix["info"] >= 0
     spread = np.max(np.abs(fused["SY"] - plain["SY"]) / np.abs(plain["SY"]))
-    bound = 4 * spread + 64 * n * EPS
+    bound = 64 * n * EPS
     rel = np.abs(mix["SY"] - plain["SY"]) / np.abs(plain["SY"])
+    print("N=%d: %.3f N eps from the plain solver; the reference builds' spread %.3f N eps"
+          % (n, rel.max() / (n * EPS), spread / (n * EPS)))
     assert rel.max() <= bound, (rel.max(), spread)
     for a, b in zip(sorted_by_class(mix["SY"], J), sorted_by_class(plain["SY"], J)):
         assert np.max(np.abs(a - b) / np.abs(b)) <= bound
@@ -154,8 +156,10 @@ def check_against_plain(ctx, Y, J, out):
     plain = ctx.dsolve(Y, J, n // 2)
     fused = ctx.dsolve(Y, J, n // 2, fused=True)
     spread = np.max(np.abs(fused["SY"] - plain["SY"]) / np.abs(plain["SY"]))
-    bound = 4 * spread + 64 * n * EPS
+    bound = 64 * n * EPS
     rel = np.max(np.abs(out["SY"] - plain["SY"]) / np.abs(plain["SY"]))
+    print("N=%d: %.3f N eps from the plain solver; the reference builds' spread %.3f N eps"
+          % (n, rel / (n * EPS), spread / (n * EPS)))
     assert rel <= bound, (rel, spread)
     rec, orth, jorth = zresiduals(Y, J, out)
     prec, porth, pjorth = zresiduals(Y, J, plain)
