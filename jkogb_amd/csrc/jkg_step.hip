@@ -1029,53 +1029,44 @@ __global__ void __launch_bounds__(256) k_step_end(const int *__restrict__ J, con
   }
   const int err = st->err;
   if (threadIdx.x < 5) cnt[threadIdx.x] = 0;
-  __syncthreads();
-  {
-    int nb2 = 0, nb4 = 0;
-    for (int i = threadIdx.x; i < sl; i += blockDim.x) {
-      const int l = piv_info[i];
-      if (l >= 0 && (l & 2)) ++nb2;
-      if (l >= 0 && (l & 4)) ++nb4;
-    }
-    atomicAdd(&cnt[3], nb2);
-    atomicAdd(&cnt[4], nb4);
-  }
-  if (err == 0) {
-    int nl = 0, nt = 0, nh = 0;
-    for (int i = threadIdx.x; i < sl; i += blockDim.x) {
-      const int l = piv_info[i];
-      if ((l & 1) && ((l & 2) || (l & 4))) ++nl;
-      if ((l & 2) || (l & 4)) {
-        if (J[piv_p[i]] == J[piv_q[i]]) ++nt;
-        else ++nh;
-      }
-    }
-    atomicAdd(&cnt[0], nl);
-    atomicAdd(&cnt[1], nt);
-    atomicAdd(&cnt[2], nh);
-  }
-  // TW: the weights of the pivots with a transform, summed in pivot order from
-  // -0 (one lane, as the reference's loop), staged through LDS in chunks; a
-  // NaN marks a pivot without a transform (pivot weights are finite).
+  // One pass over the pivots in chunks: the counts (NL, trigonometric and hyperbolic
+  // transforms, row and column transforms) and TW, the weights of the pivots with a
+  // transform summed in pivot order from -0 by one lane (as the reference's loop).  A
+  // pivot without a transform (or with a NaN weight) adds -0, which leaves every sum
+  // unchanged (x + -0 = x for all x), so the chain is one add per pivot.  All loads of
+  // an entry are issued together.
   __shared__ double wbuf[1024];
   double tw = -0.0;
-  if (err == 0)
-    for (int c0 = 0; c0 < sl; c0 += 1024) {
-      __syncthreads();
-      for (int i = threadIdx.x; i < 1024 && c0 + i < sl; i += blockDim.x) {
-        const int l = piv_info[c0 + i];
-        wbuf[i] = ((l & 2) || (l & 4)) ? piv_w[c0 + i] : __builtin_nan("");
-      }
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        const int m = (sl - c0 < 1024) ? sl - c0 : 1024;
-#pragma unroll 8
-        for (int i = 0; i < m; ++i) {
-          const double w = wbuf[i];
-          tw = (w == w) ? tw + w : tw;
+  int nb2 = 0, nb4 = 0, nlc = 0, nt = 0, nh = 0;
+  for (int c0 = 0; c0 < sl; c0 += 1024) {
+    const int m = (sl - c0 < 1024) ? sl - c0 : 1024;
+    __syncthreads();  // wbuf free, cnt cleared
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+      const int l = piv_info[c0 + i];
+      const double w = piv_w[c0 + i];
+      const bool tr = (l & 2) || (l & 4);
+      if (l >= 0 && (l & 2)) ++nb2;
+      if (l >= 0 && (l & 4)) ++nb4;
+      if (err == 0) {
+        if ((l & 1) && tr) ++nlc;
+        if (tr) {
+          if (J[piv_p[c0 + i]] == J[piv_q[c0 + i]]) ++nt;
+          else ++nh;
         }
       }
+      wbuf[i] = (tr && w == w) ? w : -0.0;
     }
+    __syncthreads();
+    if (threadIdx.x == 0 && err == 0) {
+#pragma unroll 16
+      for (int i = 0; i < m; ++i) tw += wbuf[i];
+    }
+  }
+  atomicAdd(&cnt[0], nlc);
+  atomicAdd(&cnt[1], nt);
+  atomicAdd(&cnt[2], nh);
+  atomicAdd(&cnt[3], nb2);
+  atomicAdd(&cnt[4], nb4);
   __syncthreads();
   if (threadIdx.x == 0) {
     const int nl = (err == 0) ? cnt[0] : err;
