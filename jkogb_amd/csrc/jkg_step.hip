@@ -215,15 +215,11 @@ __global__ void __launch_bounds__(256) k_hyp_weights(int n, const double *__rest
 // k_sel_walk: one workgroup of SW_WAVES waves walks the list in super-chunks
 // of SW_CHUNK entries (SW_ITEMS consecutive entries per lane, one 16-byte
 // buffer load per array, the next super-chunk's loads in flight).  All lanes
-// test their entries against used[] (n + 1 flag bytes in LDS; used[n] = 1) and
-// the entries still alive are compacted, in list order, into an LDS queue;
-// then wave 0 alone resolves the queue 64 entries at a time: it tests them
-// again (earlier acceptances of this super-chunk may have killed them), and
-// each acceptance is a ballot for the first alive lane, a readlane, two LDS
-// byte stores by lane 0 and a packed 16-bit compare on every lane that drops
-// the entries sharing an index with the accepted pair.  used[] only grows, so
-// an entry dead at the start of its super-chunk stays dead: the queue holds
-// every entry the sequential walk could take, in its order.
+// test their entries against used[] (n + 1 flag bytes in LDS; used[n] = 1).
+// used[] only grows, so an entry dead at the start of its super-chunk stays
+// dead.  The entries still alive are decided by locally dominant rounds: over
+// the whole workgroup when many are alive, else compacted, in list order, into
+// an LDS queue that wave 0 alone resolves 64 entries at a time.
 //
 // Between walk rounds the rest of the list is compacted, in order, to the
 // entries whose two indices are both free (k_sel_count: survivors per wave
@@ -322,6 +318,73 @@ __device__ int resolve_queue(const uint32_t *qv, const uint32_t *qp, int total, 
       used[x >> 16] = 1;
     }
     acc += __popcll(M);
+  }
+  return acc;
+}
+
+// The queue resolved by wave 0 in locally dominant rounds over each group of 64
+// entries (the workgroup's rounds below, within one wave, so without barriers: a
+// wave's LDS accesses are carried out in the order issued).  An alive entry that
+// comes first among the group's alive entries at both of its indices is taken by
+// the sequential walk (every earlier entry at those indices is dead), and every
+// entry sharing an index with a taken one dies; a round costs a few LDS round trips
+// however many entries it takes.  first[] and the round counter (its tags) are the
+// workgroup rounds' own.  The taken entries are recorded in list order, as many as
+// N_2 leaves room for (the capped walk is a prefix of the uncapped one).
+__device__ int resolve_queue_ld(const uint32_t *qv, const uint32_t *qp, int total, uint8_t *used, uint32_t *first,
+                                int n, int &ld_round, int acc, int acc0, int n2, int *bp, int *bq, int *bpos) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t sent = (uint32_t)n * 0x10001u;  // the pair (n, n): dead through used[n] = 1
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int b = 0; b < total && acc < n2; b += 64) {
+    const int k = b + lane;
+    const uint32_t x = (k < total) ? qv[k] : sent;
+    const uint32_t xp = (k < total) ? qp[k] : 0u;
+    const uint32_t xa = x & 0xFFFFu, xb = x >> 16;
+    bool alive = (used[xa] | used[xb]) == 0;
+    uint64_t T = 0;
+    for (;;) {
+      const uint64_t A = __ballot(alive);
+      if (A == 0) break;
+      if ((A & (A - 1)) == 0) {  // one entry left: nothing conflicts with it
+        T |= A;
+        if (alive) {
+          used[xa] = 1;
+          used[xb] = 1;
+        }
+        break;
+      }
+      if (ld_round == 0xFFFF) {  // the tags ran out: first[] anew
+        for (int i = lane; i <= n; i += 64) first[i] = 0xFFFFFFFFu;
+        ld_round = 0;
+      }
+      const uint32_t me = ((0xFFFFu - (uint32_t)ld_round) << 14) | (uint32_t)k;
+      ++ld_round;
+      if (alive) {
+        atomicMin(&first[xa], me);
+        atomicMin(&first[xb], me);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      const bool win = alive && first[xa] == me && first[xb] == me;
+      if (win) {
+        used[xa] = 1;
+        used[xb] = 1;
+      }
+      T |= __ballot(win);
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      alive = alive && !win && (used[xa] | used[xb]) == 0;
+    }
+    const int room = n2 - acc;
+    if ((T >> lane) & 1ull) {
+      const int r = __popcll(T & lt);
+      if (r < room) {
+        bp[acc - acc0 + r] = (int)xa;
+        bq[acc - acc0 + r] = (int)xb;
+        bpos[acc - acc0 + r] = (int)xp;
+      }
+    }
+    const int nt = __popcll(T);
+    acc += (nt < room) ? nt : room;
   }
   return acc;
 }
@@ -479,8 +542,8 @@ __global__ void __launch_bounds__(SW_WAVES * 64) k_sel_walk(int n, const uint64_
     const int pre = block_prefix(__popc(alive), total);
     c = (c + SW_CHUNK < stop) ? c + SW_CHUNK : stop;
     if (total > 0 && total <= LD_MIN) {
-      // few entries alive: the queue, resolved in order by wave 0 (each acceptance is a few
-      // scalar instructions; rounds over the whole workgroup cost more here)
+      // few entries alive: the queue, resolved by wave 0 alone (rounds over the whole
+      // workgroup cost more here: a barrier each)
       int o = pre;
 #pragma unroll
       for (int i = 0; i < SW_ITEMS; ++i)
@@ -490,10 +553,14 @@ __global__ void __launch_bounds__(SW_WAVES * 64) k_sel_walk(int n, const uint64_
           ++o;
         }
       lds_barrier();
-      if (wave == 0) acc = resolve_queue(qv, qp, total, used, n, acc, acc0, n2, bp, bq, bpos, nullptr);
-      if (wave == 0 && lane == 0) wtot[par][0] = acc;
+      if (wave == 0) acc = resolve_queue_ld(qv, qp, total, used, first, n, ld_round, acc, acc0, n2, bp, bq, bpos);
+      if (wave == 0 && lane == 0) {
+        wtot[par][0] = acc;
+        wtot[par][1] = ld_round;
+      }
       lds_barrier();
       acc = wtot[par][0];
+      ld_round = wtot[par][1];
       par ^= 1;
     } else if (total > 0) {
       // Locally dominant rounds over the whole super-chunk: an alive entry that comes first (in
