@@ -346,10 +346,10 @@ __global__ void __launch_bounds__(256) k_colnorm_off(int n, const E *__restrict_
 }
 
 // ------------------------------------------------------------------ weights
-// One workgroup per 64 (J = +1) x 64 (J = -1) tile of hyperbolic pairs.  The
+// One workgroup per 32 (J = +1) x 32 (J = -1) tile of hyperbolic pairs.  The
 // two off-diagonal blocks A(minus, plus) and A(plus, minus) of the tile are
 // gathered through LDS so that every global read runs down a column; the
-// diagonal comes from dg.  The branch structure is DMAGF2H's; only the
+// diagonal comes from dg, staged with the columns' norms and signs.  The branch structure is DMAGF2H's; only the
 // ABODNDF2 sum is replaced by the quadratic form above.
 constexpr int WT = 32;
 
@@ -365,34 +365,73 @@ __global__ void __launch_bounds__(256) k_hyp_weights_gram(int n, const E *__rest
   if (!st->active) return;
   __shared__ double E1[WT][WT + 1];  // E1[a][b] = A(minus b, plus a)
   __shared__ double E2[WT][WT + 1];  // E2[b][a] = A(plus a, minus b)
-  __shared__ int pc[WT], mc[WT];
+  __shared__ int pc[WT], mc[WT];     // the tile's columns (-1: none), plus then minus
+  __shared__ double cdg[2 * WT], cnr[2 * WT];  // their diagonal entries and off-diagonal norms^2
+  __shared__ int cj[2 * WT];                   // and signs
   const int i0 = blockIdx.y * WT, j0 = blockIdx.x * WT, tid = threadIdx.x;
-  if (tid < WT) pc[tid] = (i0 + tid < npl) ? plus[i0 + tid] : -1;
-  else if (tid < 2 * WT) mc[tid - WT] = (j0 + tid - WT < nmi) ? minus[j0 + tid - WT] : -1;
+  // Every global load is issued early and unconditionally (indices clamped into their arrays,
+  // what lies outside masked afterwards): the Gram entries of this lane's pairs first (they
+  // need no column index), then the column data, then the two blocks.
+  constexpr int PL = WT * WT / 256;  // pairs per lane
+  constexpr int GPF = 2;             // Gram planes prefetched (the rest, if any, read in place)
+  const size_t plane = (size_t)npl * nmi;
+  double gv[PL][GPF];
+#pragma unroll
+  for (int i = 0; i < PL; ++i) {
+    const int e = tid + 256 * i, a = e / WT, b = e % WT;
+    const size_t gi = (size_t)min(i0 + a, npl - 1) * ldg + min(j0 + b, nmi - 1);
+#pragma unroll
+    for (int z = 0; z < GPF; ++z) gv[i][z] = (z < gks) ? G[gi + z * plane] : 0.0;
+  }
+  if (tid < 2 * WT) {
+    const bool isp = tid < WT;
+    const int g = isp ? i0 + tid : j0 + tid - WT;
+    const bool in = g < (isp ? npl : nmi);
+    const int c = isp ? plus[min(g, npl - 1)] : minus[min(g, nmi - 1)];
+    const double vd = dg[c], vn = coln[c];
+    const int vj = J[c];
+    if (isp) pc[tid] = in ? c : -1;
+    else mc[tid - WT] = in ? c : -1;
+    cdg[tid] = vd;
+    cnr[tid] = vn;
+    cj[tid] = vj;
+  }
   __syncthreads();
-  for (int e = tid; e < WT * WT; e += 256) {
-    const int hi = e / WT, lo = e % WT;
-    const int c1 = pc[hi], r1 = mc[lo];  // column plus hi, row minus lo
-    E1[hi][lo] = (c1 >= 0 && r1 >= 0) ? (double)A[(size_t)c1 * lda + r1] : 0.0;
-    const int c2 = mc[hi], r2 = pc[lo];  // column minus hi, row plus lo
-    E2[hi][lo] = (c2 >= 0 && r2 >= 0) ? (double)A[(size_t)c2 * lda + r2] : 0.0;
+  {
+    double v1[PL], v2[PL];
+#pragma unroll
+    for (int i = 0; i < PL; ++i) {
+      const int e = tid + 256 * i, hi = e / WT, lo = e % WT;
+      const int c1 = pc[hi], r1 = mc[lo];  // column plus hi, row minus lo
+      const int c2 = mc[hi], r2 = pc[lo];  // column minus hi, row plus lo
+      v1[i] = (double)A[(size_t)max(c1, 0) * lda + max(r1, 0)];
+      v2[i] = (double)A[(size_t)max(c2, 0) * lda + max(r2, 0)];
+    }
+#pragma unroll
+    for (int i = 0; i < PL; ++i) {
+      const int e = tid + 256 * i, hi = e / WT, lo = e % WT;
+      E1[hi][lo] = (pc[hi] >= 0 && mc[lo] >= 0) ? v1[i] : 0.0;
+      E2[hi][lo] = (mc[hi] >= 0 && pc[lo] >= 0) ? v2[i] : 0.0;
+    }
   }
   __syncthreads();
   unsigned nfull = 0;
-  for (int e = tid; e < WT * WT; e += 256) {
-    const int a = e / WT, b = e % WT;
+#pragma unroll
+  for (int i = 0; i < PL; ++i) {
+    const int e = tid + 256 * i, a = e / WT, b = e % WT;
     const int cp = pc[a], cm = mc[b];
     if (cp < 0 || cm < 0) continue;
-    const int p0 = cp < cm ? cp : cm, q0 = cp < cm ? cm : cp;
+    const bool pfirst = cp < cm;  // the plus column is p (the smaller index)
+    const int p0 = pfirst ? cp : cm, q0 = pfirst ? cm : cp;
     const double e1 = E1[a][b], e2 = E2[b][a];  // A(cm, cp), A(cp, cm)
-    const double aqp = cp < cm ? e1 : e2, apq = cp < cm ? e2 : e1;
-    const double app = dg[p0], aqq = dg[q0];
+    const double aqp = pfirst ? e1 : e2, apq = pfirst ? e2 : e1;
+    const double app = pfirst ? cdg[a] : cdg[WT + b], aqq = pfirst ? cdg[WT + b] : cdg[a];
     uint64_t key = KEY_NAN;
     const bool order = sign_neg(app) || sign_neg(aqq);
     const bool drop = tol > 0.0 && !order && negligible(aqp, apq, app, aqq, tol);
     if ((aqp != 0.0 || apq != 0.0 || order) && !drop) {
       double a2[4] = {app, aqp, apq, aqq}, u2[4], z2[4];
-      int j2[2] = {J[p0], J[q0]};
+      int j2[2] = {pfirst ? cj[a] : cj[WT + b], pfirst ? cj[WT + b] : cj[a]};
       const int info = dhsvd2<F>(a2, j2, u2, z2, TH1FIX);
       if (info < 0) key = KEY_NAN;
       else if ((info & 1) == 0) key = weight_key(0.0);
@@ -402,10 +441,11 @@ __global__ void __launch_bounds__(256) k_hyp_weights_gram(int n, const E *__rest
         const double aq = (info & 8) ? a2[1] : 0.0, ap = (info & 8) ? a2[2] : 0.0;
         double d = fmad<F>(aqp - aq, aqp + aq, 0.0);
         d = fmad<F>(apq - ap, apq + ap, d);
-        const double gxx = coln[p0] - aqp * aqp, gyy = coln[q0] - apq * apq;
-        const size_t gi = (size_t)(i0 + a) * ldg + j0 + b, plane = (size_t)npl * nmi;
-        double gxy = G[gi];
-        for (int z = 1; z < gks; ++z) gxy += G[gi + z * plane];
+        const double cnp = pfirst ? cnr[a] : cnr[WT + b], cnq = pfirst ? cnr[WT + b] : cnr[a];
+        const double gxx = cnp - aqp * aqp, gyy = cnq - apq * apq;
+        double gxy = gv[i][0];
+        for (int z = 1; z < gks; ++z)
+          gxy += (z < GPF) ? gv[i][z < GPF ? z : 0] : G[(size_t)(i0 + a) * ldg + j0 + b + z * plane];
         const double b11 = z2[0], b21 = z2[1], b12 = z2[2], b22 = z2[3];
         const double w = d + gxx * (1.0 - b11 * b11 - b12 * b12) + gyy * (1.0 - b21 * b21 - b22 * b22) -
                          2.0 * gxy * (b11 * b21 + b12 * b22);

@@ -83,19 +83,37 @@ __global__ void __launch_bounds__(256) k_trig_weights(int n, const E *__restrict
   __shared__ double dgp[TWT], dgq[TWT];
   __shared__ int jp[TWT], jq[TWT];
   const int P0 = bp * TWT, Q0 = bq * TWT, t = threadIdx.x;
-  for (int e = t; e < TWT * TWT; e += 256) {
-    const int r = e % TWT, c = e / TWT;
-    X[r][c] = (P0 + c < n && Q0 + r < n) ? (double)A[(size_t)(P0 + c) * lda + Q0 + r] : 0.0;
-    Y[r][c] = (P0 + r < n && Q0 + c < n) ? (double)A[(size_t)(Q0 + c) * lda + P0 + r] : 0.0;
+  // every load issued before any is used: addresses clamped into the matrix, entries outside
+  // it cleared when the tile is written to LDS
+  constexpr int TL = TWT * TWT / 256;
+  E xv[TL], yv[TL];
+#pragma unroll
+  for (int i = 0; i < TL; ++i) {
+    const int e = t + 256 * i, r = e % TWT, c = e / TWT;
+    const int pc = min(P0 + c, n - 1), qr = min(Q0 + r, n - 1), qc = min(Q0 + c, n - 1), pr = min(P0 + r, n - 1);
+    xv[i] = A[(size_t)pc * lda + qr];
+    yv[i] = A[(size_t)qc * lda + pr];
+  }
+  const int dq = (t < TWT) ? P0 + t : Q0 + t - TWT;  // lanes 0..63: the diagonal of the P, then the Q block
+  const int dqc = min(dq, n - 1);
+  E dv = 0;
+  int jv = 0;
+  if (t < 2 * TWT) {
+    dv = A[(size_t)dqc * lda + dqc];
+    jv = J[dqc];
+  }
+#pragma unroll
+  for (int i = 0; i < TL; ++i) {
+    const int e = t + 256 * i, r = e % TWT, c = e / TWT;
+    X[r][c] = (P0 + c < n && Q0 + r < n) ? (double)xv[i] : 0.0;
+    Y[r][c] = (P0 + r < n && Q0 + c < n) ? (double)yv[i] : 0.0;
   }
   if (t < TWT) {
-    const int P = P0 + t;
-    dgp[t] = (P < n) ? (double)A[(size_t)P * lda + P] : 0.0;
-    jp[t] = (P < n) ? J[P] : 0;
+    dgp[t] = (dq < n) ? (double)dv : 0.0;
+    jp[t] = (dq < n) ? jv : 0;
   } else if (t < 2 * TWT) {
-    const int Q = Q0 + t - TWT;
-    dgq[t - TWT] = (Q < n) ? (double)A[(size_t)Q * lda + Q] : 0.0;
-    jq[t - TWT] = (Q < n) ? J[Q] : 0;
+    dgq[t - TWT] = (dq < n) ? (double)dv : 0.0;
+    jq[t - TWT] = (dq < n) ? jv : 0;
   }
   __syncthreads();
   // slot s: diagonal dl = s / 32 - 31 (q - Q0 = p - P0 + dl), a = p - P0 = s % 32
