@@ -28,6 +28,11 @@
 // 8+13 + 1 + 12+13 + 1 + 12+9 + 1 + 8+9 + 1 + 8+9 + 1 + 8+9 + 1 + 8+5 + 1 + 4+4
 // = 133 bytes per key, against 8 x (8 + 24) = 256 for eight passes of
 // (key, value) pairs with a histogram read of the keys in each.
+// For the step (pair values) the first pass leaves the NaN keys out -- the
+// walk stops at the first of them -- and the later passes carry only the
+// others (their count read on the device), which late in a run, when most
+// pairs are negligible, is a small part of the list.  Stability keeps the
+// finite keys in the order a sort of the whole list gives them.
 // Every launch is the same for every step (a fixed sequence a graph replays);
 // the key count is read on the device.
 #include "jkg_sort.hpp"
@@ -81,16 +86,17 @@ __global__ void k_sort_reset(const int *n_ptr, int *scratch) {
     ss->nan_count = 0;
     ss->n = *n_ptr;
     ss->minkey = ~0ull;
+    ss->nsorted = *n_ptr;
   }
 }
 
 // Tile histogram of the pass's digit.  FIRST: from the keys (digit at bit
-// `shift`), with the NaN count and the smallest key; otherwise from the digit
-// bytes the previous pass wrote.
+// `shift`), with the NaN count and the smallest key (drop: the NaN keys are not
+// counted in the histogram); otherwise from the digit bytes the previous pass wrote.
 template <bool FIRST>
 __global__ void __launch_bounds__(SORT_THREADS) k_sort_count(const uint64_t *__restrict__ keys,
                                                              const uint8_t *__restrict__ dig, const int *n_ptr,
-                                                             int *scratch, int ntiles, int shift) {
+                                                             int *scratch, int ntiles, int shift, int drop) {
   __shared__ uint32_t h[RADIX];
   __shared__ int snan;
   __shared__ unsigned long long smin;
@@ -117,8 +123,9 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_count(const uint64_t *__r
 #pragma unroll
       for (int r = 0; r < SORT_ITEMS; ++r) {
         if (base + r * SORT_THREADS + (int)threadIdx.x < n) {
-          atomicAdd(&h[(int)((k[r] >> shift) & 0xFF)], 1u);
-          nan += (k[r] >= 0xFFFFFFFF00000000ull) ? 1 : 0;
+          const bool isnan = key_nan(k[r]);
+          if (!(drop && isnan)) atomicAdd(&h[(int)((k[r] >> shift) & 0xFF)], 1u);
+          nan += isnan ? 1 : 0;
           mn = (k[r] < mn) ? k[r] : mn;
         }
       }
@@ -152,8 +159,8 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_count(const uint64_t *__r
 }
 
 // one workgroup per digit value; first: workgroup 0 also sums the NaN counts and finds the
-// smallest key over the tiles of the first pass
-__global__ void __launch_bounds__(SORT_THREADS) k_sort_scan(int *scratch, int ntiles, int first) {
+// smallest key over the tiles of the first pass (drop: the later passes carry the other keys)
+__global__ void __launch_bounds__(SORT_THREADS) k_sort_scan(int *scratch, int ntiles, int first, int drop) {
   __shared__ int part[SORT_THREADS];
   __shared__ unsigned long long pmin[SORT_THREADS];
   const int d = blockIdx.x;
@@ -179,6 +186,7 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scan(int *scratch, int nt
       SortStats *ss = sc_stats(scratch);
       ss->nan_count = part[0];
       ss->minkey = pmin[0];
+      if (drop) ss->nsorted = ss->n - part[0];
     }
     __syncthreads();
   }
@@ -242,6 +250,7 @@ struct PassArgs {
   uint32_t *vout;
   uint8_t *dout;
   int pair_n;
+  int drop; // IN == F_ORIG: the NaN keys are left out
   int ds;   // bit of this pass's digit in the input record
   int sh;   // bits dropped from the record (OUT != F_VAL)
   int vsh;  // bit of the value in the input record (OUT == F_VAL)
@@ -259,6 +268,7 @@ __global__ void __launch_bounds__(SC_THREADS) k_sort_scatter(PassArgs a, const i
   __shared__ uint64_t lanes[NW][RADIX]; // per-wave: the lanes whose key of the current round has digit d
   __shared__ int gdelta[RADIX];         // global position of a staged slot of digit d, minus the slot
   __shared__ int wsum[RADIX / 64];
+  __shared__ int staged;                // keys of the tile written (all, or those not dropped)
   const int n = *n_ptr;
   const int tiles = (n + SORT_TILE - 1) / SORT_TILE;
   const int t = blockIdx.x;
@@ -270,11 +280,13 @@ __global__ void __launch_bounds__(SC_THREADS) k_sort_scatter(PassArgs a, const i
 
   uint64_t L[SC_ITEMS];
   uint32_t H[SC_ITEMS];
+  uint32_t vmask = 0;  // the items ranked and written
 #pragma unroll
   for (int r = 0; r < SC_ITEMS; ++r) {
     const int li = wbase + r * 64 + lane;
     const int idx = t * SORT_TILE + li;
     const bool valid = li < tlen;
+    vmask |= (valid ? 1u : 0u) << r;
     if constexpr (IN == F_ORIG) {
       L[r] = valid ? a.k0[idx] : ~0ull;
       H[r] = valid ? (a.vin ? a.vin[idx] : (uint32_t)idx) : 0u;
@@ -289,6 +301,13 @@ __global__ void __launch_bounds__(SC_THREADS) k_sort_scatter(PassArgs a, const i
     } else {
       L[r] = valid ? (uint64_t)a.vin_rec[idx] : ~0ull;
       H[r] = 0u;
+    }
+  }
+  if constexpr (IN == F_ORIG) {
+    if (a.drop) {
+#pragma unroll
+      for (int r = 0; r < SC_ITEMS; ++r)
+        if (key_nan(L[r])) vmask &= ~(1u << r);
     }
   }
 #pragma unroll
@@ -306,7 +325,7 @@ __global__ void __launch_bounds__(SC_THREADS) k_sort_scatter(PassArgs a, const i
   int rank[SC_ITEMS];
 #pragma unroll
   for (int r = 0; r < SC_ITEMS; ++r) {
-    const bool valid = wbase + r * 64 + lane < tlen;
+    const bool valid = (vmask >> r) & 1u;
     const int d = (int)((L[r] >> ds) & 0xFF);
     if (valid) __hip_atomic_fetch_or(&lanes[wave][d], me, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
     __builtin_amdgcn_wave_barrier();
@@ -337,6 +356,7 @@ __global__ void __launch_bounds__(SC_THREADS) k_sort_scatter(PassArgs a, const i
       }
     }
     const int ls = block_excl_scan256(c, wsum);  // first slot of digit d in the staged tile
+    if (d == RADIX - 1) staged = ls + c;
     if (d < RADIX) {
 #pragma unroll
       for (int w = 0; w < NW; ++w) wcnt[w][d] += ls;
@@ -349,7 +369,7 @@ __global__ void __launch_bounds__(SC_THREADS) k_sort_scatter(PassArgs a, const i
   // the digit (an output record needs at most 88 bits, so its high word at most 24)
 #pragma unroll
   for (int r = 0; r < SC_ITEMS; ++r) {
-    if (wbase + r * 64 + lane < tlen) {
+    if ((vmask >> r) & 1u) {
       const int d = (int)((L[r] >> ds) & 0xFF);
       const int slot = wcnt[wave][d] + rank[r];
       const unsigned __int128 rec = (unsigned __int128)L[r] | ((unsigned __int128)H[r] << 64);
@@ -375,7 +395,8 @@ __global__ void __launch_bounds__(SC_THREADS) k_sort_scatter(PassArgs a, const i
   __syncthreads();
   // write out in staged order: neighbouring lanes hold neighbouring slots; the low byte of an
   // output record is the next pass's digit
-  for (int i = threadIdx.x; i < tlen; i += SC_THREADS) {
+  const int nst = staged;
+  for (int i = threadIdx.x; i < nst; i += SC_THREADS) {
     const uint32_t hd = sH[i];
     const uint64_t l = sL[i];
     const int pos = gdelta[hd >> 24] + i;
@@ -429,6 +450,8 @@ void sort_pairs_launch(SortBuffers &b, const int *n_dev, int nmax, hipStream_t s
   uint64_t *kb[2] = {b.k1, b.k2};
   uint32_t *vb[2] = {b.v1, b.v2};
   uint8_t *db[2] = {b.d1, b.d2};
+  const int drop = pair_n > 0 ? 1 : 0;
+  const int *n_rest = &sort_stats(b)->nsorted;  // the passes after the first
   hipLaunchKernelGGL(k_sort_reset, dim3(1), dim3(64), 0, st, n_dev, b.scratch);
   for (int pass = first; pass < 8; ++pass) {
     const int r = pass - first;  // passes run before this one
@@ -437,11 +460,12 @@ void sort_pairs_launch(SortBuffers &b, const int *n_dev, int nmax, hipStream_t s
     const int out = (pass == 7) ? F_VAL : fmt_for_width(64 - 8 * (pass + 1) + vbits);
     if (r == 0)
       hipLaunchKernelGGL(k_sort_count<true>, dim3(ntiles), dim3(SORT_THREADS), 0, st, (const uint64_t *)b.k0,
-                         (const uint8_t *)nullptr, n_dev, b.scratch, ntiles, 8 * pass);
+                         (const uint8_t *)nullptr, n_dev, b.scratch, ntiles, 8 * pass, drop);
     else
       hipLaunchKernelGGL(k_sort_count<false>, dim3(ntiles), dim3(SORT_THREADS), 0, st, (const uint64_t *)nullptr,
-                         (const uint8_t *)db[(r - 1) & 1], n_dev, b.scratch, ntiles, 0);
-    hipLaunchKernelGGL(k_sort_scan, dim3(RADIX), dim3(SORT_THREADS), 0, st, b.scratch, ntiles, r == 0 ? 1 : 0);
+                         (const uint8_t *)db[(r - 1) & 1], n_rest, b.scratch, ntiles, 0, 0);
+    hipLaunchKernelGGL(k_sort_scan, dim3(RADIX), dim3(SORT_THREADS), 0, st, b.scratch, ntiles, r == 0 ? 1 : 0,
+                       drop);
     PassArgs a;
     a.k0 = b.k0;
     a.vin = b.vin;
@@ -452,10 +476,11 @@ void sort_pairs_launch(SortBuffers &b, const int *n_dev, int nmax, hipStream_t s
     a.vout = vb[r & 1];
     a.dout = db[r & 1];
     a.pair_n = pair_n;
+    a.drop = drop;
     a.ds = 8 * pass - lo_in;
     a.sh = 8 * (pass + 1) - lo_in;
     a.vsh = 64 - lo_in;
-    launch_scatter(in, out, a, n_dev, b.scratch, ntiles, st);
+    launch_scatter(in, out, a, r == 0 ? n_dev : n_rest, b.scratch, ntiles, st);
   }
 }
 

@@ -25,8 +25,9 @@ struct SortBuffers {
 // Per-sort figures the first pass finds on the way, for the selection.
 struct SortStats {
   int nan_count;             // keys >= 0xFFFFFFFF00000000 (the step's NaN / dropped keys, sorted last)
-  int n;                     // keys sorted
+  int n;                     // keys given
   unsigned long long minkey; // smallest key (~0 when there are none)
+  int nsorted;               // keys the passes after the first carry: n, or n - nan_count (pair_n > 0)
 };
 
 size_t sort_scratch_bytes(int nmax);
@@ -37,7 +38,9 @@ inline SortStats *sort_stats(const SortBuffers &b) { return (SortStats *)b.scrat
 // grid) by their bits [lo_bits, 64) -- lo_bits = 0 or 32 (keys whose low half
 // is zero) -- and writes the sorted values to b.v0.  pair_n > 0: the values are
 // pair indices k of an N = pair_n problem (the input values must be implicit)
-// and leave as packed pairs pack_pq(p - 1, q - 1); pair_n = 0: values as given.
+// and leave as packed pairs pack_pq(p - 1, q - 1), and the NaN keys are dropped
+// by the first pass (only the first n - nan_count values are written: the step
+// never reads past them); pair_n = 0: values as given, every key sorted.
 void sort_pairs_launch(SortBuffers &b, const int *n_dev, int nmax, hipStream_t st, int pair_n, int lo_bits = 0);
 
 }  // namespace jkg

@@ -127,6 +127,30 @@ def test_build_bit_exact_vs_oracle(ctx, orc, cfg, n, seed, n2):
     assert check_build(ctx, orc, y, j, n2) > 0
 
 
+@pytest.mark.parametrize("n", [300, 700])
+def test_build_with_dropped_pairs(ctx, orc, n):
+    """Most pairs are dropped before the sort (src/D/dstep.F90 DSTEP_BUILD keeps the
+    finite weights only): a diagonal that is already ordered in each sign class
+    (non-increasing for J = +1, non-decreasing for J = -1, both positive) and
+    about 10 % of the off-diagonal entries nonzero, so a pair with both of its
+    off-diagonal entries zero has a NaN weight.  The step's sort leaves those keys
+    out after its first pass; the finite part must come out in the oracle's order
+    and give the oracle's selection."""
+    rng = np.random.default_rng(41 + n)
+    npl = n // 2 + 20
+    j = np.array([1] * npl + [-1] * (n - npl), dtype=np.int64)
+    d = np.concatenate([np.sort(rng.uniform(1.0, 2.0, npl))[::-1], np.sort(rng.uniform(1.0, 2.0, n - npl))])
+    a = np.zeros((n, n))
+    mask = rng.random((n, n)) < 0.1
+    a[mask] = rng.standard_normal(int(mask.sum())) * 1e-2
+    a[np.diag_indices(n)] = d
+    a = np.asfortranarray(a)
+    nn = n * (n - 1) // 2
+    srt, _ = ctx.dbuild(a, j, n // 2)
+    assert 0.1 * nn < len(srt) < 0.3 * nn  # most keys are NaN
+    assert check_build(ctx, orc, a, j, n // 2) > 0
+
+
 def test_build_deep_walk_uses_filter_rounds(ctx, orc):
     """N = 700: NN = 244650 candidates, more than one walk round (65536
     entries), so the filter/compaction rounds decide the tail of the walk."""
