@@ -49,10 +49,12 @@ constexpr int SORT_ITEMS = SORT_TILE / SORT_THREADS;
 constexpr int SC_THREADS = 512;
 constexpr int SC_ITEMS = SORT_TILE / SC_THREADS;
 
-// record formats: the input keys (and values); records of up to 96 bits as 16-byte pairs of
-// words (one array), of up to 64 bits as words, of up to 32 bits as 32-bit words; the values
+// record formats: the input keys (and values); records of up to 96 bits as three 32-bit words
+// (12 bytes, one array), of up to 64 bits as words, of up to 32 bits as 32-bit words; the values
 enum : int { F_ORIG = 0, F_96 = 1, F_64 = 2, F_32 = 3, F_VAL = 4 };
-typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+struct R12 {
+  uint32_t w0, w1, w2;  // bits 0-31, 32-63, 64-95
+};
 
 // scratch layout (bytes): SortStats | digit totals | per-digit rows of tile counts | per tile of
 // the first pass: NaN keys, smallest key
@@ -291,10 +293,10 @@ __global__ void __launch_bounds__(SC_THREADS) k_sort_scatter(PassArgs a, const i
       L[r] = valid ? a.k0[idx] : ~0ull;
       H[r] = valid ? (a.vin ? a.vin[idx] : (uint32_t)idx) : 0u;
     } else if constexpr (IN == F_96) {
-      u64x2 v = {~0ull, 0ull};
-      if (valid) v = ((const u64x2 *)a.kin)[idx];
-      L[r] = v.x;
-      H[r] = (uint32_t)v.y;
+      R12 v = {~0u, ~0u, 0u};
+      if (valid) v = ((const R12 *)a.kin)[idx];
+      L[r] = (uint64_t)v.w0 | ((uint64_t)v.w1 << 32);
+      H[r] = v.w2;
     } else if constexpr (IN == F_64) {
       L[r] = valid ? a.kin[idx] : ~0ull;
       H[r] = 0u;
@@ -401,7 +403,7 @@ __global__ void __launch_bounds__(SC_THREADS) k_sort_scatter(PassArgs a, const i
     const uint64_t l = sL[i];
     const int pos = gdelta[hd >> 24] + i;
     if constexpr (OUT == F_96) {
-      ((u64x2 *)a.kout)[pos] = u64x2{l, (uint64_t)(hd & 0x00FFFFFFu)};
+      ((R12 *)a.kout)[pos] = R12{(uint32_t)l, (uint32_t)(l >> 32), hd & 0x00FFFFFFu};
     } else if constexpr (OUT == F_64) {
       a.kout[pos] = l;
     } else if constexpr (OUT == F_32) {

@@ -10,12 +10,12 @@ namespace jkg {
 // sorted values only, and the key of sorted entry i is the input key at the
 // index the value names (for the dynamic-ordering step: the pair index of the
 // packed pair).  Between passes each key travels as one record holding its
-// value above the key bits not sorted yet, in 16, 8 or 4 bytes (k1/k2 or
+// value above the key bits not sorted yet, in 12, 8 or 4 bytes (k1/k2 or
 // v1/v2), and its next digit as one byte (d1/d2) for the next pass's counts.
 struct SortBuffers {
   uint64_t *k0;        // input keys (read, never written)
   uint32_t *v0;        // output: the sorted values (capacity n + padding of the caller's choice)
-  uint64_t *k1, *k2;   // ping-pong records of 16 or 8 bytes (2 nmax words each)
+  uint64_t *k1, *k2;   // ping-pong records of 12 or 8 bytes (2 nmax words each)
   uint32_t *v1, *v2;   // ping-pong records of 4 bytes
   uint8_t *d1, *d2;    // ping-pong next digits (sort_digit_bytes(nmax) bytes each)
   const uint32_t *vin; // input values (null: the value of key i is i)
