@@ -99,10 +99,15 @@ template <bool FIRST>
 __global__ void __launch_bounds__(SORT_THREADS) k_sort_count(const uint64_t *__restrict__ keys,
                                                              const uint8_t *__restrict__ dig, const int *n_ptr,
                                                              int *scratch, int ntiles, int shift, int drop) {
-  __shared__ uint32_t h[RADIX];
+  // HC copies of the histogram, lane l adding to copy l mod HC: the top digits (sign and
+  // exponent bits) fall into few bins, and same-address LDS atomics of a wave are serialized
+  constexpr int HC = 4;
+  __shared__ uint32_t hc[HC][RADIX];
   __shared__ int snan;
   __shared__ unsigned long long smin;
-  h[threadIdx.x] = 0u;
+#pragma unroll
+  for (int c = 0; c < HC; ++c) hc[c][threadIdx.x] = 0u;
+  uint32_t *h = hc[threadIdx.x & (HC - 1)];
   if (FIRST && threadIdx.x == 0) {
     snan = 0;
     smin = ~0ull;
@@ -152,7 +157,10 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_count(const uint64_t *__r
     }
   }
   __syncthreads();
-  sc_tiles(scratch)[threadIdx.x * ntiles + t] = h[threadIdx.x];  // zero for tiles past n
+  uint32_t hsum = 0u;
+#pragma unroll
+  for (int c = 0; c < HC; ++c) hsum += hc[c][threadIdx.x];
+  sc_tiles(scratch)[threadIdx.x * ntiles + t] = hsum;  // zero for tiles past n
   if (FIRST && threadIdx.x == 0) {
     // per tile (one global atomic per workgroup on one address would queue them all)
     sc_tnan(scratch, ntiles)[t] = snan;
@@ -279,6 +287,12 @@ __global__ void __launch_bounds__(SC_THREADS) k_sort_scatter(PassArgs a, const i
   const int tlen = (n - t * SORT_TILE < SORT_TILE) ? n - t * SORT_TILE : SORT_TILE;
   const int wbase = wave * (SORT_TILE / NW);
   const int ds = a.ds;
+  // the digit totals and this tile's offsets (thread d < 256: digit d), loaded with the records
+  int dtot = 0, doff = 0;
+  if (threadIdx.x < RADIX) {
+    dtot = (int)sc_total(scratch)[threadIdx.x];
+    doff = (int)sc_tiles(scratch)[(size_t)threadIdx.x * ntiles + t];
+  }
 
   uint64_t L[SC_ITEMS];
   uint32_t H[SC_ITEMS];
@@ -363,8 +377,8 @@ __global__ void __launch_bounds__(SC_THREADS) k_sort_scatter(PassArgs a, const i
 #pragma unroll
       for (int w = 0; w < NW; ++w) wcnt[w][d] += ls;
     }
-    const int gs = block_excl_scan256(d < RADIX ? (int)sc_total(scratch)[d] : 0, wsum);
-    if (d < RADIX) gdelta[d] = gs + (int)sc_tiles(scratch)[(size_t)d * ntiles + t] - ls;
+    const int gs = block_excl_scan256(dtot, wsum);
+    if (d < RADIX) gdelta[d] = gs + doff - ls;
   }
   __syncthreads();
   // stage in digit order: the output record (or value) and, in the top byte of the high word,
