@@ -267,15 +267,32 @@ struct PassArgs {
 };
 
 // One pass over the tiles: IN/OUT record formats.
+// LDS of a scatter: the staged tile -- records of 8 bytes (OUT = F_96, F_64) or 4, then per slot
+// the high word with the digit in its top byte (F_96) or the digit alone -- shares its bytes with
+// the ranking's lane masks (used before the staging), so a pass writing 8-byte records or less
+// fits three workgroups on a CU (a 12-byte one two)
+template <int OUT>
+struct ScatterLds {
+  static constexpr size_t REC = (OUT == F_96 || OUT == F_64) ? 8 : 4;
+  static constexpr size_t HI = (OUT == F_96) ? 4 : 1;
+  static constexpr size_t STAGE = (size_t)SORT_TILE * (REC + HI);
+  static constexpr size_t LANES = (size_t)(SC_THREADS / 64) * RADIX * 8;
+  static constexpr size_t BYTES = STAGE > LANES ? STAGE : LANES;
+  static constexpr int WAVES_PER_SIMD = (OUT == F_96) ? 4 : 6;  // two or three workgroups per CU
+};
+
 template <int IN, int OUT>
-__global__ void __launch_bounds__(SC_THREADS) k_sort_scatter(PassArgs a, const int *n_ptr, int *scratch,
-                                                             int ntiles) {
+__global__ void __launch_bounds__(SC_THREADS, ScatterLds<OUT>::WAVES_PER_SIMD)
+    k_sort_scatter(PassArgs a, const int *n_ptr, int *scratch, int ntiles) {
   constexpr int NW = SC_THREADS / 64;
-  // 73 KB of LDS: two workgroups (16 waves) share a CU
-  __shared__ uint64_t sL[SORT_TILE];
-  __shared__ uint32_t sH[SORT_TILE];
+  typedef ScatterLds<OUT> SL;
+  __shared__ __attribute__((aligned(16))) uint8_t sbuf[SL::BYTES];
+  uint64_t(*lanes)[RADIX] = (uint64_t(*)[RADIX])sbuf;  // per wave: the lanes whose key of the current round has digit d
+  uint64_t *sL8 = (uint64_t *)sbuf;                     // staged records (after the ranking)
+  uint32_t *sL4 = (uint32_t *)sbuf;
+  uint32_t *sH4 = (uint32_t *)(sbuf + (size_t)SORT_TILE * SL::REC);
+  uint8_t *sD = sbuf + (size_t)SORT_TILE * SL::REC;
   __shared__ int wcnt[NW][RADIX];       // per-wave digit counters, then the first slot of wave w's keys of digit d
-  __shared__ uint64_t lanes[NW][RADIX]; // per-wave: the lanes whose key of the current round has digit d
   __shared__ int gdelta[RADIX];         // global position of a staged slot of digit d, minus the slot
   __shared__ int wsum[RADIX / 64];
   __shared__ int staged;                // keys of the tile written (all, or those not dropped)
@@ -404,8 +421,16 @@ __global__ void __launch_bounds__(SC_THREADS) k_sort_scatter(PassArgs a, const i
         oL = (uint64_t)o;
         if constexpr (OUT == F_96) oH = (uint32_t)(o >> 64);
       }
-      sL[slot] = oL;
-      sH[slot] = oH | ((uint32_t)d << 24);
+      if constexpr (OUT == F_96) {
+        sL8[slot] = oL;
+        sH4[slot] = oH | ((uint32_t)d << 24);
+      } else if constexpr (OUT == F_64) {
+        sL8[slot] = oL;
+        sD[slot] = (uint8_t)d;
+      } else {
+        sL4[slot] = (uint32_t)oL;
+        sD[slot] = (uint8_t)d;
+      }
     }
   }
   __syncthreads();
@@ -413,8 +438,8 @@ __global__ void __launch_bounds__(SC_THREADS) k_sort_scatter(PassArgs a, const i
   // output record is the next pass's digit
   const int nst = staged;
   for (int i = threadIdx.x; i < nst; i += SC_THREADS) {
-    const uint32_t hd = sH[i];
-    const uint64_t l = sL[i];
+    const uint32_t hd = (OUT == F_96) ? sH4[i] : (uint32_t)sD[i] << 24;
+    const uint64_t l = (OUT == F_96 || OUT == F_64) ? sL8[i] : (uint64_t)sL4[i];
     const int pos = gdelta[hd >> 24] + i;
     if constexpr (OUT == F_96) {
       ((R12 *)a.kout)[pos] = R12{(uint32_t)l, (uint32_t)(l >> 32), hd & 0x00FFFFFFu};
