@@ -1079,7 +1079,7 @@ __device__ void controller_features(StepState *st, int s, int n, int nn, int sl,
   }
 }
 
-__global__ void __launch_bounds__(256) k_step_end(const int *__restrict__ J, const int *__restrict__ piv_p,
+__global__ void __launch_bounds__(1024) k_step_end(const int *__restrict__ J, const int *__restrict__ piv_p,
                                                    const int *__restrict__ piv_q, const int *__restrict__ piv_info,
                                                    const double *__restrict__ piv_w, StepState *st, StepRec *trace,
                                                    int n, int nn, const int *__restrict__ piv_pos, int ctl_on,
@@ -1412,7 +1412,7 @@ void rotate_launch(DeviceStep &d, hipStream_t s) {
 void step_end_launch(DeviceStep &d, hipStream_t s) {
   // cyclic orderings: the steps of one sweep (a sweep without a transform ends the run)
   const int sweep = d.cyclic == 1 ? d.n + (d.n & 1) - 1 : (d.cyclic ? (int)d.nn : 0);
-  hipLaunchKernelGGL(k_step_end, dim3(1), dim3(256), 0, s, d.J, d.piv_p, d.piv_q, d.piv_info, d.piv_w, d.st,
+  hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1024), 0, s, d.J, d.piv_p, d.piv_q, d.piv_info, d.piv_w, d.st,
                      d.trace, d.n, d.nn, d.piv_pos, d.ctl_mode != 0 ? 1 : 0, d.ctl_trace, d.ctl_trace_cap, sweep);
 }
 
