@@ -1,5 +1,5 @@
 """The device radix sort through the ASRT plug-in (jkg_srt), on lists large enough for thousands of
-tiles and on key distributions that exercise every record width and the skipped passes.
+tiles and on key distributions that exercise every record width and constant digits.
 
 The expected order is AW_CMP's (src/shared/atypes.F90:85-120): weight descending with -0 == +0,
 then band q - p descending, then p descending -- computed here independently by numpy's lexsort
@@ -57,11 +57,11 @@ def expected_order(dz, n):
 DISTS = {
     # every digit varies: all eight passes run, records shrink from 12 to 8 to 4 bytes
     "wide": lambda rng, m: rng.standard_normal(m) * 10.0 ** rng.uniform(-150, 150, m),
-    # a handful of values: the low six digits are the same in every key (skipped passes)
+    # a handful of values: the low six digits are the same in every key (long equal-digit runs)
     "ties": lambda rng, m: rng.integers(-3, 4, m).astype(np.float64) * 0.5,
-    # FP32 values: the low 29 mantissa bits are zero, the lowest three digits skip
+    # FP32 values: the low 29 mantissa bits are zero, the lowest three digits constant
     "fp32": lambda rng, m: rng.standard_normal(m).astype(np.float32).astype(np.float64),
-    # one value: every pass is an identity, the last one runs to write the values
+    # one value: every pass keeps the input order (ties by pair index only)
     "const": lambda rng, m: np.full(m, 2.5),
     # +-0 and tiny subnormals next to ordinary values
     "zeros": lambda rng, m: np.where(rng.random(m) < 0.5, np.where(rng.random(m) < 0.5, 0.0, -0.0),
