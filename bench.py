@@ -25,8 +25,9 @@ Printed as one JSON line:
   rooflines    each stage against its bound (weights: FP64 MFMA; rotate, select,
                sort: HBM)
   c5_full      complete solves of the same N = 4096 input (the headline wall time):
-               the mixed solver and the trained controller by default (also the FP32
-               warm start with --c5-variants mixed,learned,warm): device time, steps,
+               the mixed solver, the trained controller, and the controller after the
+               FP32 warm start by default (also the warm start alone with
+               --c5-variants mixed,learned,warm_learned,warm): device time, steps,
                the singular values against the plain (reference-arithmetic) run in
                tests/golden/solve_c5_n4096_plain_sy.npy in units of N eps, the
                residuals and the componentwise condition of the singular values
@@ -420,7 +421,7 @@ def main():
     ap.add_argument("--no-c5-full", action="store_true", help="skip the complete N=4096 solves (minutes)")
     ap.add_argument("--c5-noise", action="store_true",
                     help="also the mixed solver on Y moved by one ulp per entry (the method's own spread)")
-    ap.add_argument("--c5-variants", default="mixed,learned",
+    ap.add_argument("--c5-variants", default="mixed,learned,warm_learned",
                     help="complete N=4096 solves: any of mixed, warm, learned, warm_learned")
     ap.add_argument("--no-complex", action="store_true")
     ap.add_argument("--complex-full", action="store_true", help="also complete complex N=2048 solves")

@@ -137,13 +137,14 @@ def c5_reference():
 
 
 @pytest.mark.timeout(400)
-@pytest.mark.parametrize("variant", ["learned", "warm"])
+@pytest.mark.parametrize("variant", ["learned", "warm", "warm_learned"])
 def test_c5_full_size(ctx, variant):
     from jkogb_amd import inputs
     Y, J = inputs.make("c5", seed=0)
     ref, fma, spread = c5_reference()
-    kw = {"learned": dict(ctl_params=np.load(os.path.join(ROOT, "jkogb_amd", "ctl_params.npy"))),
-          "warm": dict(fp32_warm=1, key_bits=32)}[variant]
+    ctl = dict(ctl_params=np.load(os.path.join(ROOT, "jkogb_amd", "ctl_params.npy")))
+    kw = {"learned": ctl, "warm": dict(fp32_warm=1, key_bits=32),
+          "warm_learned": dict(fp32_warm=1, key_bits=32, **ctl)}[variant]
     out = ctx.dsolve_mixed(Y, J, 2048, trace_cap=1, **kw)
     assert out["info"] >= 0
     r = rel_n_eps(out["SY"], ref)
