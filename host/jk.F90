@@ -20,6 +20,8 @@
 !           JKG_CTL_PARAMS (default: jkogb_amd/ctl_params.f32 next to this
 !           program's directory) -- real problems only
 !   warm    the mixed solver starting with FP32 steps (fp32_warm)
+!   warm_learned  both: FP32 steps first, the trained controller throughout
+!           (real problems only)
 ! Other environment: JKG_DEVICE (default 0), JKG_FUSED=1 selects FMAD = fma.
 !
 ! Checkpoint and resume (plain solver; jkg_dsolve_from / jkg_zsolve_from):
@@ -86,7 +88,8 @@ PROGRAM jk
   CALL GET_ENVIRONMENT_VARIABLE('JKG_SOLVER', ENVV, STATUS=INFO)
   IF (INFO .EQ. 0) SOLVER = ADJUSTL(ENVV)
   IF ((SOLVER .NE. 'plain') .AND. (SOLVER .NE. 'mixed') .AND. (SOLVER .NE. 'learned') .AND. &
-      (SOLVER .NE. 'warm')) ERROR STOP 'JKG_SOLVER: plain, mixed, learned or warm'
+      (SOLVER .NE. 'warm') .AND. (SOLVER .NE. 'warm_learned')) &
+       ERROR STOP 'JKG_SOLVER: plain, mixed, learned, warm or warm_learned'
   IF ((SOLVER .NE. 'plain') .AND. (ID_NCP .EQ. 3)) ERROR STOP 'JKG_SOLVER: ID_NCP=3 needs the plain solver'
   CKPT = 0
   CALL GET_ENVIRONMENT_VARIABLE('JKG_CHECKPOINT', ENVV, STATUS=INFO)
@@ -206,8 +209,8 @@ PROGRAM jk
   TRACE_PRINTED = 0
 
   CALL jkg_mixed_opts_default(MOPTS)
-  IF (SOLVER .EQ. 'warm') MOPTS%fp32_warm = 1
-  IF (SOLVER .EQ. 'learned') THEN
+  IF ((SOLVER .EQ. 'warm') .OR. (SOLVER .EQ. 'warm_learned')) MOPTS%fp32_warm = 1
+  IF ((SOLVER .EQ. 'learned') .OR. (SOLVER .EQ. 'warm_learned')) THEN
      CALL GET_ENVIRONMENT_VARIABLE('JKG_CTL_PARAMS', PATH, STATUS=INFO)
      IF (INFO .NE. 0) THEN
         CALL GET_COMMAND_ARGUMENT(0, PATH)

@@ -114,9 +114,9 @@ def test_dropin_fused_policy(tmp_path, orc, case):
     assert np.max(np.abs(sy - g["SY"]) / np.abs(g["SY"])) <= 64 * n * np.finfo(np.float64).eps
 
 
-@pytest.mark.parametrize("solver", ["mixed", "learned", "warm"])
+@pytest.mark.parametrize("solver", ["mixed", "learned", "warm", "warm_learned"])
 def test_dropin_fast_solvers(tmp_path, solver):
-    """JKG_SOLVER selects the mixed solver, the trained controller or the FP32 warm start;
+    """JKG_SOLVER selects the mixed solver, the trained controller, the FP32 warm start or both;
     their singular values agree with the plain run of the same program to 64 N eps, the factors
     are orthogonal and J-orthogonal to the same order, and the CSV has one line per step."""
     from jkogb_amd import inputs
