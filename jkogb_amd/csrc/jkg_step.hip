@@ -256,6 +256,7 @@ constexpr int LD_MAXIT = 12;       // locally dominant rounds per super-chunk be
 constexpr int FT_WAVES = 4;                          // waves of a filter workgroup
 constexpr int FT_SUB = 2048;                         // entries per wave sub-tile
 constexpr int FT_TILE = FT_WAVES * FT_SUB;           // entries per filter workgroup
+constexpr int FT_G = FT_SUB / 256;                   // groups of 4 entries per lane in a sub-tile
 
 __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
@@ -730,10 +731,17 @@ __global__ void __launch_bounds__(FT_WAVES * 64) k_sel_count(int n, const uint32
   int cnt = 0;
   // c0 is a multiple of SW_CHUNK (the walk stops at whole super-chunks or at the end of its
   // list), so every group of 4 entries is 16-byte aligned; groups reach at most 3 entries past
-  // the list, inside the padded allocation
-  for (int e0 = sub + 4 * lane; e0 < end; e0 += 256) {
-    const u4 v = *(const u4 *)(vals + e0);
-    const uint32_t vv[4] = {v.x, v.y, v.z, v.w};
+  // the list, inside the padded allocation.  A lane's FT_G groups are loaded before any is
+  // tested (one memory latency per wave, not one per group), unconditionally: a group past the
+  // end reads the last group (its entries are masked by position).
+  const int glast = (end > 0) ? (end - 1) & ~3 : 0;
+  u4 gv[FT_G];
+#pragma unroll
+  for (int g = 0; g < FT_G; ++g) gv[g] = *(const u4 *)(vals + min(sub + 4 * lane + 256 * g, glast));
+#pragma unroll
+  for (int g = 0; g < FT_G; ++g) {
+    const int e0 = sub + 4 * lane + 256 * g;
+    const uint32_t vv[4] = {gv[g].x, gv[g].y, gv[g].z, gv[g].w};
 #pragma unroll
     for (int i = 0; i < 4; ++i) cnt += (e0 + i < end && (used[vv[i] & 0xFFFFu] | used[vv[i] >> 16]) == 0) ? 1 : 0;
   }
@@ -782,30 +790,34 @@ __global__ void __launch_bounds__(FT_WAVES * 64) k_sel_write(int n, const uint32
   const int sub = base + wave * FT_SUB;
   const int end = (sub + FT_SUB < len) ? sub + FT_SUB : len;
   typedef uint32_t u4 __attribute__((ext_vector_type(4)));
-  for (int g = sub; g < end; g += 256) {  // uniform over the wave
-    const int e0 = g + 4 * lane;
-    uint32_t vv[4] = {0u, 0u, 0u, 0u}, pv[4];
-    uint32_t alive = 0;
-    if (e0 < end) {
-      const u4 v = *(const u4 *)(vals + e0);
-      vv[0] = v.x;
-      vv[1] = v.y;
-      vv[2] = v.z;
-      vv[3] = v.w;
-      if (poss) {
-        const u4 p = *(const u4 *)(poss + e0);
-        pv[0] = p.x;
-        pv[1] = p.y;
-        pv[2] = p.z;
-        pv[3] = p.w;
-      } else {
+  // a lane's groups (values and positions) loaded before any is tested, as in k_sel_count
+  const int glast = (end > 0) ? (end - 1) & ~3 : 0;
+  u4 gv[FT_G], gp[FT_G];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) pv[i] = (uint32_t)(e0 + i + 1);
-      }
+  for (int g = 0; g < FT_G; ++g) gv[g] = *(const u4 *)(vals + min(sub + 4 * lane + 256 * g, glast));
+  if (poss) {
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
-        alive |= ((e0 + i < end) && (used[vv[i] & 0xFFFFu] | used[vv[i] >> 16]) == 0 ? 1u : 0u) << i;
+    for (int g = 0; g < FT_G; ++g) gp[g] = *(const u4 *)(poss + min(sub + 4 * lane + 256 * g, glast));
+  }
+#pragma unroll
+  for (int g = 0; g < FT_G; ++g) {
+    if (sub + 256 * g >= end) break;  // uniform over the wave
+    const int e0 = sub + 4 * lane + 256 * g;
+    const uint32_t vv[4] = {gv[g].x, gv[g].y, gv[g].z, gv[g].w};
+    uint32_t pv[4];
+    if (poss) {
+      pv[0] = gp[g].x;
+      pv[1] = gp[g].y;
+      pv[2] = gp[g].z;
+      pv[3] = gp[g].w;
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) pv[i] = (uint32_t)(e0 + i + 1);
     }
+    uint32_t alive = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      alive |= ((e0 + i < end) && (used[vv[i] & 0xFFFFu] | used[vv[i] >> 16]) == 0 ? 1u : 0u) << i;
     int tot;
     int o = out + wave_prefix_small(__popc(alive), tot);
 #pragma unroll
