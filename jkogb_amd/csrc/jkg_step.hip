@@ -703,9 +703,18 @@ __global__ void __launch_bounds__(SW_WAVES * 64) k_sel_walk(int n, const uint64_
 }
 
 // The n flag bytes of used_g (a 256-byte aligned allocation) into LDS, by words.
+// Four words per lane are loaded before any is stored (from clamped addresses).
 __device__ __forceinline__ void copy_used(uint8_t *used, const uint8_t *used_g, int n) {
-  const int nw = n >> 2;
-  for (int i = threadIdx.x; i < nw; i += blockDim.x) ((uint32_t *)used)[i] = ((const uint32_t *)used_g)[i];
+  const int nw = n >> 2, bd = blockDim.x;
+  const uint32_t *src = (const uint32_t *)used_g;
+  for (int b = 0; b < nw; b += 4 * bd) {
+    uint32_t w[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) w[k] = src[min(b + k * bd + (int)threadIdx.x, nw > 0 ? nw - 1 : 0)];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (b + k * bd + (int)threadIdx.x < nw) ((uint32_t *)used)[b + k * bd + threadIdx.x] = w[k];
+  }
   if ((int)threadIdx.x < (n & 3)) used[4 * nw + threadIdx.x] = used_g[4 * nw + threadIdx.x];
 }
 
@@ -775,7 +784,13 @@ __global__ void __launch_bounds__(FT_WAVES * 64) k_sel_write(int n, const uint32
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   // survivors of the tiles before this one (each tile sums them itself: no pass over the counts)
   int s = 0;
-  for (int i = threadIdx.x; i < t; i += blockDim.x) s += fcount[i];
+  for (int b = 0; b < t; b += 4 * (int)blockDim.x) {  // four counts per lane in flight
+    int c[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) c[k] = fcount[min(b + k * (int)blockDim.x + (int)threadIdx.x, t - 1)];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) s += (b + k * (int)blockDim.x + (int)threadIdx.x < t) ? c[k] : 0;
+  }
   for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o);
   if (lane == 0) red[wave] = s;
   __syncthreads();
