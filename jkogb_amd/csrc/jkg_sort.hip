@@ -47,6 +47,8 @@ constexpr int SORT_THREADS = 256;
 constexpr int SORT_ITEMS = SORT_TILE / SORT_THREADS;
 // scatter: 512 threads x 8 keys
 constexpr int SC_THREADS = 512;
+// scan: counts per lane loaded together
+constexpr int SCAN_B = 8;
 constexpr int SC_ITEMS = SORT_TILE / SC_THREADS;
 
 // record formats: the input keys (and values); records of up to 96 bits as three 32-bit words
@@ -177,10 +179,21 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scan(int *scratch, int nt
   if (first && d == 0) {
     int nan = 0;
     unsigned long long mn = ~0ull;
-    for (int i = threadIdx.x; i < ntiles; i += SORT_THREADS) {
-      nan += sc_tnan(scratch, ntiles)[i];
-      const unsigned long long m = sc_tmin(scratch, ntiles)[i];
-      mn = (m < mn) ? m : mn;
+    for (int b = 0; b < ntiles; b += SCAN_B * SORT_THREADS) {  // SCAN_B tiles per lane in flight
+      int tn[SCAN_B];
+      unsigned long long tm[SCAN_B];
+#pragma unroll
+      for (int k = 0; k < SCAN_B; ++k) {
+        const int i = min(b + k * SORT_THREADS + (int)threadIdx.x, ntiles - 1);
+        tn[k] = sc_tnan(scratch, ntiles)[i];
+        tm[k] = sc_tmin(scratch, ntiles)[i];
+      }
+#pragma unroll
+      for (int k = 0; k < SCAN_B; ++k)
+        if (b + k * SORT_THREADS + (int)threadIdx.x < ntiles) {
+          nan += tn[k];
+          mn = (tm[k] < mn) ? tm[k] : mn;
+        }
     }
     part[threadIdx.x] = nan;
     pmin[threadIdx.x] = mn;
@@ -203,9 +216,21 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scan(int *scratch, int nt
   uint32_t *row = sc_tiles(scratch) + (size_t)d * ntiles;
   const int per = (ntiles + SORT_THREADS - 1) / SORT_THREADS;
   const int lo = threadIdx.x * per;
+  // a lane's counts SCAN_B at a time, kept in registers for the prefix pass when they fit
+  int v[SCAN_B];
+  auto load_counts = [&](int b) {
+#pragma unroll
+    for (int k = 0; k < SCAN_B; ++k) {
+      const int i = lo + b + k;
+      v[k] = (b + k < per && i < ntiles) ? (int)row[min(i, ntiles - 1)] : 0;
+    }
+  };
   int s = 0;
-  for (int i = 0; i < per; ++i)
-    if (lo + i < ntiles) s += (int)row[lo + i];
+  for (int b = 0; b < per; b += SCAN_B) {
+    load_counts(b);
+#pragma unroll
+    for (int k = 0; k < SCAN_B; ++k) s += v[k];
+  }
   part[threadIdx.x] = s;
   __syncthreads();
   // inclusive Hillis-Steele scan of the 256 partial sums
@@ -216,12 +241,15 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scan(int *scratch, int nt
     __syncthreads();
   }
   int run = part[threadIdx.x] - s;
-  for (int i = 0; i < per; ++i)
-    if (lo + i < ntiles) {
-      const int v = (int)row[lo + i];
-      row[lo + i] = (uint32_t)run;
-      run += v;
-    }
+  for (int b = 0; b < per; b += SCAN_B) {
+    if (per > SCAN_B) load_counts(b);  // else v holds the lane's counts already
+#pragma unroll
+    for (int k = 0; k < SCAN_B; ++k)
+      if (b + k < per && lo + b + k < ntiles) {
+        row[lo + b + k] = (uint32_t)run;
+        run += v[k];
+      }
+  }
   if (threadIdx.x == SORT_THREADS - 1) sc_total(scratch)[d] = (uint32_t)part[SORT_THREADS - 1];
 }
 
