@@ -573,7 +573,7 @@ def main():
              "frac": round(sel_gbs / HBM_PEAK_GBS, 5), "traffic": (traffic or {}).get("select"),
              "bytes_per_launch": int(stage_bytes["select"]), "ms_per_launch": round(pm["select"], 4)},
             {"bound": "hbm", "kernel": "sort stage (8 passes of k_sort_count, k_sort_scan, k_sort_scatter over "
-                                       "records of 16, 8 and 4 bytes)",
+                                       "records of 12, 8 and 4 bytes)",
              "achieved": round(sort_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
              "frac": round(sort_gbs / HBM_PEAK_GBS, 5), "traffic": (traffic or {}).get("sort"),
              "bytes_per_launch": int(stage_bytes["sort"]), "ms_per_launch": round(pm["sort"], 4)},
