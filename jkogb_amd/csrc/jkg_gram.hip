@@ -334,11 +334,21 @@ __global__ void __launch_bounds__(256) k_colnorm_off(int n, const E *__restrict_
   const int col = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (col >= n) return;
   const E *a = A + (size_t)col * lda;
+  // a lane's rows lane, lane + 64, ... summed in that order; CB of them loaded at a time (from
+  // clamped rows) before any is added
+  constexpr int CB = 16;
   double s = 0.0;
-  for (int r = lane; r < n; r += 64) {
-    const double v = (double)a[r];
-    if (r == col) dg[col] = v;
-    else s = __builtin_fma(v, v, s);
+  for (int r0 = lane; r0 < n; r0 += 64 * CB) {
+    E v[CB];
+#pragma unroll
+    for (int t = 0; t < CB; ++t) v[t] = a[min(r0 + 64 * t, n - 1)];
+#pragma unroll
+    for (int t = 0; t < CB; ++t) {
+      const int r = r0 + 64 * t;
+      const double x = (double)v[t];
+      if (r == col) dg[col] = x;
+      else if (r < n) s = __builtin_fma(x, x, s);
+    }
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
