@@ -528,11 +528,20 @@ __global__ void __launch_bounds__(256) k_zcolnorm_off(int n, const E *__restrict
   const int col = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (col >= n) return;
   const E *a = A + (size_t)col * lda;
+  // as k_colnorm_off: the rows in the same order, ZCB of them loaded at a time from clamped rows
+  constexpr int ZCB = 8;
   double s = 0.0;
-  for (int r = lane; r < n; r += 64) {
-    const dz v = ldz(a[r]);
-    if (r == col) dg[col] = v;
-    else s = __builtin_fma(v.im, v.im, __builtin_fma(v.re, v.re, s));
+  for (int r0 = lane; r0 < n; r0 += 64 * ZCB) {
+    E v[ZCB];
+#pragma unroll
+    for (int t = 0; t < ZCB; ++t) v[t] = a[min(r0 + 64 * t, n - 1)];
+#pragma unroll
+    for (int t = 0; t < ZCB; ++t) {
+      const int r = r0 + 64 * t;
+      const dz x = ldz(v[t]);
+      if (r == col) dg[col] = x;
+      else if (r < n) s = __builtin_fma(x.im, x.im, __builtin_fma(x.re, x.re, s));
+    }
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
@@ -550,10 +559,25 @@ __global__ void __launch_bounds__(256) k_warm_offnorm(int n, const double *__res
                                                       StepState *st, double switch_tol) {
   if (!st->active) return;
   __shared__ double so[256], sd[256];
+  // a lane's entries i, i + 256, ... added in that order, OB of them loaded at a time
+  constexpr int OB = 8;
   double a = 0.0, b = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) {
-    a += c[i];
-    b = sq_acc(dg[i], b);
+  for (int i0 = threadIdx.x; i0 < n; i0 += 256 * OB) {
+    double cv[OB];
+    DG dv[OB];
+#pragma unroll
+    for (int t = 0; t < OB; ++t) {
+      const int i = min(i0 + 256 * t, n - 1);
+      cv[t] = c[i];
+      dv[t] = dg[i];
+    }
+#pragma unroll
+    for (int t = 0; t < OB; ++t) {
+      if (i0 + 256 * t < n) {
+        a += cv[t];
+        b = sq_acc(dv[t], b);
+      }
+    }
   }
   so[threadIdx.x] = a;
   sd[threadIdx.x] = b;
