@@ -84,16 +84,6 @@ __device__ __forceinline__ unsigned long long *sc_tmin(int *s, int ntiles) {
                                 ((SC_TILES + (size_t)RADIX * ntiles * 4 + (size_t)ntiles * 4 + 7) & ~(size_t)7));
 }
 
-__global__ void k_sort_reset(const int *n_ptr, int *scratch) {
-  if (threadIdx.x == 0) {
-    SortStats *ss = sc_stats(scratch);
-    ss->nan_count = 0;
-    ss->n = *n_ptr;
-    ss->minkey = ~0ull;
-    ss->nsorted = *n_ptr;
-  }
-}
-
 // Tile histogram of the pass's digit.  FIRST: from the keys (digit at bit
 // `shift`), with the NaN count and the smallest key (drop: the NaN keys are not
 // counted in the histogram); otherwise from the digit bytes the previous pass wrote.
@@ -171,8 +161,10 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_count(const uint64_t *__r
 }
 
 // one workgroup per digit value; first: workgroup 0 also sums the NaN counts and finds the
-// smallest key over the tiles of the first pass (drop: the later passes carry the other keys)
-__global__ void __launch_bounds__(SORT_THREADS) k_sort_scan(int *scratch, int ntiles, int first, int drop) {
+// smallest key over the tiles of the first pass, and sets every field of the sort's statistics
+// (drop: the later passes carry the other keys)
+__global__ void __launch_bounds__(SORT_THREADS) k_sort_scan(int *scratch, int ntiles, int first, int drop,
+                                                            const int *n_ptr) {
   __shared__ int part[SORT_THREADS];
   __shared__ unsigned long long pmin[SORT_THREADS];
   const int d = blockIdx.x;
@@ -207,9 +199,11 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scan(int *scratch, int nt
     }
     if (threadIdx.x == 0) {
       SortStats *ss = sc_stats(scratch);
+      const int n = *n_ptr;
+      ss->n = n;
       ss->nan_count = part[0];
       ss->minkey = pmin[0];
-      if (drop) ss->nsorted = ss->n - part[0];
+      ss->nsorted = drop ? n - part[0] : n;
     }
     __syncthreads();
   }
@@ -521,7 +515,6 @@ void sort_pairs_launch(SortBuffers &b, const int *n_dev, int nmax, hipStream_t s
   uint8_t *db[2] = {b.d1, b.d2};
   const int drop = pair_n > 0 ? 1 : 0;
   const int *n_rest = &sort_stats(b)->nsorted;  // the passes after the first
-  hipLaunchKernelGGL(k_sort_reset, dim3(1), dim3(64), 0, st, n_dev, b.scratch);
   for (int pass = first; pass < 8; ++pass) {
     const int r = pass - first;  // passes run before this one
     const int lo_in = (r == 0) ? 0 : 8 * pass;
@@ -534,7 +527,7 @@ void sort_pairs_launch(SortBuffers &b, const int *n_dev, int nmax, hipStream_t s
       hipLaunchKernelGGL(k_sort_count<false>, dim3(ntiles), dim3(SORT_THREADS), 0, st, (const uint64_t *)nullptr,
                          (const uint8_t *)db[(r - 1) & 1], n_rest, b.scratch, ntiles, 0, 0);
     hipLaunchKernelGGL(k_sort_scan, dim3(RADIX), dim3(SORT_THREADS), 0, st, b.scratch, ntiles, r == 0 ? 1 : 0,
-                       drop);
+                       drop, n_dev);
     PassArgs a;
     a.k0 = b.k0;
     a.vin = b.vin;
